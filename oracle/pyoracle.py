@@ -1,0 +1,247 @@
+"""ctypes binding of the oracle (TEST INFRASTRUCTURE ONLY).
+
+Only the test suite, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module, and only as the checker.  It binds
+``oracle/liboracle.so`` (the scalar restatement of the reference CPU backend)
+and, when present, ``oracle/_ref/libi360ref.so`` (the reference's own
+hot-path sources compiled by ``make -C oracle ref``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC = None
+_REF = None
+
+u16p = C.POINTER(C.c_uint16)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+i32p = C.POINTER(C.c_int32)
+u8p = C.POINTER(C.c_uint8)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def lib():
+    global _ORC
+    if _ORC is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run: make -C oracle)")
+        _ORC = C.CDLL(path)
+    return _ORC
+
+
+def ref():
+    """The compiled reference sources, or None where they were not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libi360ref.so")
+        if not os.path.exists(path):
+            return None
+        _REF = C.CDLL(path)
+    return _REF
+
+
+def _ptr_array(planes, t=u16p):
+    arr = (t * len(planes))()
+    for i, p in enumerate(planes):
+        arr[i] = p.ctypes.data_as(t)
+    return arr
+
+
+# ----------------------------------------------------------------- oracle --
+def median_stack(planes, sigma=2.0):
+    planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+    out = np.zeros_like(planes[0])
+    lib().orc_median_stack_u16(_ptr_array(planes), len(planes), _p(out, u16p), C.c_longlong(out.size),
+                               C.c_float(sigma))
+    return out
+
+
+def warp_perspective(src, dst_w, dst_h, invH, off_x=0, off_y=0):
+    src = np.ascontiguousarray(src, dtype=np.uint16)
+    h, w = src.shape[:2]
+    m = np.ascontiguousarray(invH, dtype=np.float32).reshape(9)
+    out = np.zeros((dst_h, dst_w, 3), np.uint16)
+    lib().orc_warp_perspective_u16(_p(src, u16p), w, h, _p(out, u16p), dst_w, dst_h, _p(m, f32p), off_x, off_y)
+    return out
+
+
+def apply_gain(data, gain):
+    d = np.ascontiguousarray(data, dtype=np.uint16).copy()
+    lib().orc_apply_gain_u16(_p(d, u16p), C.c_longlong(d.size), C.c_float(gain))
+    return d
+
+
+def demosaic(cfa, black, cam_mul, rgb_cam, filters, f32=False):
+    cfa = np.ascontiguousarray(cfa, dtype=np.uint16)
+    h, w = cfa.shape
+    cm = np.ascontiguousarray(cam_mul, dtype=np.float32)
+    rc = np.ascontiguousarray(rgb_cam, dtype=np.float32).reshape(12)
+    if f32:
+        out = np.zeros((h, w, 3), np.float32)
+        lib().orc_demosaic_bayer_f32(_p(cfa, u16p), w, h, C.c_uint16(black), _p(cm, f32p), _p(rc, f32p),
+                                     C.c_uint32(filters), _p(out, f32p))
+    else:
+        out = np.zeros((h, w, 3), np.uint16)
+        lib().orc_demosaic_bayer_u16(_p(cfa, u16p), w, h, C.c_uint16(black), _p(cm, f32p), _p(rc, f32p),
+                                     C.c_uint32(filters), _p(out, u16p))
+    return out
+
+
+def invert_homography(hm):
+    m = np.ascontiguousarray(hm, dtype=np.float32).reshape(9)
+    out = np.zeros(9, np.float32)
+    ok = lib().orc_invert_homography(_p(m, f32p), _p(out, f32p))
+    return out if ok else None
+
+
+def invert_homography_f64(hm):
+    m = np.ascontiguousarray(hm, dtype=np.float64).reshape(9)
+    out = np.zeros(9, np.float64)
+    ok = lib().orc_invert_homography_f64(_p(m, f64p), _p(out, f64p))
+    return out if ok else None
+
+
+def coverage_distance(cov):
+    cov = np.ascontiguousarray(cov, dtype=np.uint8)
+    h, w = cov.shape
+    out = np.zeros((h, w), np.float32)
+    lib().orc_coverage_distance(_p(cov, u8p), w, h, _p(out, f32p))
+    return out
+
+
+def blend_chunk(warped, num_bands=5, want_owner=False):
+    warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
+    h, w = warped[0].shape[:2]
+    out = np.zeros((h, w, 3), np.uint16)
+    owner = np.zeros((h, w), np.int32)
+    lib().orc_blend_chunk_u16(_ptr_array(warped), len(warped), w, h, num_bands, _p(out, u16p), _p(owner, i32p))
+    return (out, owner) if want_owner else out
+
+
+def multiband_compose(warped, owner, num_bands=5):
+    warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
+    h, w = warped[0].shape[:2]
+    covs = [np.ascontiguousarray((x != 0).any(axis=2).astype(np.uint8)) for x in warped]
+    owner = np.ascontiguousarray(owner, dtype=np.int32)
+    out = np.zeros((h, w, 3), np.uint16)
+    lib().orc_multiband_compose_u16(_ptr_array(warped), _ptr_array(covs, u8p), _p(owner, i32p), len(warped), w, h,
+                                    num_bands, _p(out, u16p))
+    return out
+
+
+def clamp_bands(w, h, n):
+    return lib().orc_clamp_bands(w, h, n)
+
+
+def render_chunk(srcs, hinv, gains, x0, y0, w, h, num_bands, fp64=True, owner_in=None, want_owner=False):
+    """Float chunk render; srcs are float32 RGB frames, hinv canvas->source."""
+    srcs = [np.ascontiguousarray(s, dtype=np.float32) for s in srcs]
+    c = len(srcs)
+    sw = np.array([s.shape[1] for s in srcs], np.int32)
+    sh = np.array([s.shape[0] for s in srcs], np.int32)
+    g = np.ascontiguousarray(gains, dtype=np.float32)
+    out = np.zeros((h, w, 3), np.float32)
+    owner = np.zeros((h, w), np.int32)
+    oin = None if owner_in is None else np.ascontiguousarray(owner_in, dtype=np.int32)
+    if fp64:
+        m = np.ascontiguousarray(hinv, dtype=np.float64).reshape(c * 9)
+        lib().orc_render_chunk_f64(_ptr_array(srcs, f32p), _p(sw, i32p), _p(sh, i32p), _p(m, f64p), _p(g, f32p), c,
+                                   x0, y0, w, h, num_bands, _p(out, f32p), _p(owner, i32p),
+                                   None if oin is None else _p(oin, i32p))
+    else:
+        m = np.ascontiguousarray(hinv, dtype=np.float32).reshape(c * 9)
+        lib().orc_render_chunk_f32(_ptr_array(srcs, f32p), _p(sw, i32p), _p(sh, i32p), _p(m, f32p), _p(g, f32p), c,
+                                   x0, y0, w, h, num_bands, _p(out, f32p), _p(owner, i32p),
+                                   None if oin is None else _p(oin, i32p))
+    return (out, owner) if want_owner else out
+
+
+# -------------------------------------------------------------- reference --
+def ref_tier():
+    r = ref()
+    return -1 if r is None else r.ref_simd_tier()
+
+
+def ref_median_stack(planes, sigma=2.0, tier=0):
+    planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+    out = np.zeros_like(planes[0])
+    n = out.size // 3
+    ref().ref_median_stack(tier, _ptr_array(planes), len(planes), _p(out, u16p), n, 1, C.c_float(sigma))
+    return out
+
+
+def ref_warp_perspective(src, dst_w, dst_h, invH, off_x=0, off_y=0, tier=0):
+    src = np.ascontiguousarray(src, dtype=np.uint16)
+    h, w = src.shape[:2]
+    m = np.ascontiguousarray(invH, dtype=np.float32).reshape(9)
+    out = np.zeros((dst_h, dst_w, 3), np.uint16)
+    ref().ref_warp_perspective(tier, _p(src, u16p), w, h, _p(out, u16p), dst_w, dst_h, _p(m, f32p), off_x, off_y)
+    return out
+
+
+def ref_demosaic(cfa, black, cam_mul, rgb_cam, filters, tier=0):
+    cfa = np.ascontiguousarray(cfa, dtype=np.uint16)
+    h, w = cfa.shape
+    cm = np.ascontiguousarray(cam_mul, dtype=np.float32)
+    rc = np.ascontiguousarray(rgb_cam, dtype=np.float32).reshape(12)
+    out = np.zeros((h, w, 3), np.uint16)
+    ref().ref_demosaic_bayer(tier, _p(cfa, u16p), w, h, C.c_uint16(black), _p(cm, f32p), _p(rc, f32p),
+                             C.c_uint32(filters), _p(out, u16p))
+    return out
+
+
+def ref_apply_gain(data, gain):
+    d = np.ascontiguousarray(data, dtype=np.uint16).copy()
+    ref().ref_apply_gain(_p(d, u16p), d.size, C.c_float(gain))
+    return d
+
+
+def ref_invert_homography(hm):
+    m = np.ascontiguousarray(hm, dtype=np.float32).reshape(9)
+    out = np.zeros(9, np.float32)
+    ok = ref().ref_invert_homography(_p(m, f32p), _p(out, f32p))
+    return out if ok else None
+
+
+def ref_blend_chunk(warped):
+    warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
+    h, w = warped[0].shape[:2]
+    out = np.zeros((h, w, 3), np.uint16)
+    ref().ref_blend_chunk(_ptr_array(warped), len(warped), w, h, _p(out, u16p))
+    return out
+
+
+def ref_blend_owner(warped):
+    warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
+    h, w = warped[0].shape[:2]
+    owner = np.zeros((h, w), np.int32)
+    ref().ref_blend_owner(_ptr_array(warped), len(warped), w, h, _p(owner, i32p))
+    return owner
+
+
+def ref_coverage_distance(cov):
+    cov = np.ascontiguousarray(cov, dtype=np.uint8)
+    h, w = cov.shape
+    out = np.zeros((h, w), np.float32)
+    ref().ref_coverage_distance(_p(cov, u8p), w, h, _p(out, f32p))
+    return out
+
+
+def ref_multiband_compose(warped, owner, num_bands):
+    warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
+    h, w = warped[0].shape[:2]
+    dists = [ref_coverage_distance((x != 0).any(axis=2).astype(np.uint8)) for x in warped]
+    owner = np.ascontiguousarray(owner, dtype=np.int32)
+    out = np.zeros((h, w, 3), np.uint16)
+    ref().ref_multiband_compose(_ptr_array(warped), _ptr_array(dists, f32p), _p(owner, i32p), len(warped), w, h,
+                                num_bands, _p(out, u16p))
+    return out
