@@ -1,0 +1,95 @@
+"""ctypes binding of the C ABI declared in ``include/i360hip.h``.
+
+The library is loaded from ``image360_amd/lib/libi360hip.so`` (built in tree
+by ``python -m image360_amd.build``).  There is no fallback: a missing
+library raises immediately, and so does a missing gfx950 device when a
+context is created.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libi360hip.so")
+HEADER_DIR = os.path.join(ROOT, "include")
+
+SUCCESS, NO_GPU, INVALID_PARAM, OUT_OF_MEMORY, KERNEL_LAUNCH_FAILED, DEVICE_ERROR, NOT_SUPPORTED = range(7)
+RESULT_NAMES = {0: "SUCCESS", 1: "NO_GPU", 2: "INVALID_PARAM", 3: "OUT_OF_MEMORY", 4: "KERNEL_LAUNCH_FAILED",
+                5: "DEVICE_ERROR", 6: "NOT_SUPPORTED"}
+
+_LIB = None
+
+vp = C.c_void_p
+i32 = C.c_int
+u16p = C.POINTER(C.c_uint16)
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+sz = C.c_size_t
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("device_id", C.c_int), ("name", C.c_char * 256), ("total_memory", C.c_size_t),
+                ("free_memory", C.c_size_t), ("compute_major", C.c_int), ("compute_minor", C.c_int),
+                ("max_threads_per_block", C.c_int), ("multiprocessor_count", C.c_int),
+                ("has_matrix_cores", C.c_int), ("arch", C.c_char * 64)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "i360hip_create": (i32, [i32, C.POINTER(vp)]),
+    "i360hip_destroy": (None, [vp]),
+    "i360hip_get_device_info": (i32, [vp, C.POINTER(DeviceInfo)]),
+    "i360hip_last_error": (C.c_char_p, [vp]),
+    "i360hip_version": (C.c_char_p, []),
+    "i360hip_warp_perspective_u16": (i32, [vp, vp, i32, i32, vp, i32, i32, vp, i32, i32]),
+    "i360hip_median_stack_u16": (i32, [vp, vp, i32, vp, i32, i32, C.c_float]),
+    "i360hip_apply_gain_u16": (i32, [vp, vp, i32, C.c_float]),
+    "i360hip_demosaic_bayer_u16": (i32, [vp, vp, i32, i32, C.c_uint16, vp, vp, C.c_uint32, vp]),
+    "i360hip_blend_chunk_u16": (i32, [vp, vp, i32, i32, i32, i32, vp, vp]),
+    "i360hip_median_stack_u16_dev": (i32, [vp, vp, i32, vp, sz, C.c_float, vp]),
+    "i360hip_demosaic_bayer_u16_dev": (i32, [vp, vp, i32, i32, C.c_uint16, vp, vp, C.c_uint32, vp, vp]),
+    "i360hip_demosaic_bayer_f32_dev": (i32, [vp, vp, i32, i32, C.c_uint16, vp, vp, C.c_uint32, vp, vp]),
+    "i360hip_warp_perspective_u16_dev": (i32, [vp, vp, i32, i32, vp, i32, i32, vp, i32, i32, vp]),
+    "i360hip_apply_gain_u16_dev": (i32, [vp, vp, sz, C.c_float, vp]),
+    "i360hip_blend_chunk_u16_dev": (i32, [vp, vp, i32, i32, i32, i32, vp, vp, vp]),
+}
+
+
+def header_symbols():
+    """Every function name declared in include/*.h."""
+    names = set()
+    for f in sorted(os.listdir(HEADER_DIR)):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(HEADER_DIR, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(i360hip_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+class HipLibraryMissing(RuntimeError):
+    pass
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, where, msg):
+        super().__init__(f"{where}: {RESULT_NAMES.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipLibraryMissing(f"{LIB_PATH} is not built; run `python -m image360_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB

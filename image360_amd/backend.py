@@ -1,0 +1,182 @@
+"""Python mirror of ``WindowsApp::Compute::IComputeBackend`` over the C ABI.
+
+Method names, argument meaning, result codes and error behaviour follow the
+reference interface (WindowsApp.Compute/HeaderFiles/IComputeBackend.h:16-169):
+every operation returns a ComputeResult code, writes into caller-allocated
+output arrays and leaves the error text in ``GetLastError()``.  Device-array
+variants (``*_dev``) take raw device pointers (ints, e.g. from
+``torch.Tensor.data_ptr()``) and an optional stream handle.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (DEVICE_ERROR, INVALID_PARAM, NOT_SUPPORTED, NO_GPU, SUCCESS, HipError, lib)
+
+DEFAULT_BANDS = 5
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _c_contig(a, dtype):
+    if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.flags.c_contiguous):
+        raise TypeError(f"expected a C-contiguous {np.dtype(dtype).name} array")
+    return a
+
+
+class HipComputeBackend:
+    """IComputeBackend over the HIP library (one device context)."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        self._ctx = None
+        self._error = ""
+
+    # ---- lifetime (IComputeBackend.h:21-25)
+    def Initialize(self) -> int:
+        if self._ctx is not None:
+            return SUCCESS
+        L = lib()
+        h = C.c_void_p()
+        rc = L.i360hip_create(self.device, C.byref(h))
+        if rc != SUCCESS:
+            self._error = (L.i360hip_last_error(None) or b"").decode()
+            return rc
+        self._ctx = h
+        return SUCCESS
+
+    def Shutdown(self) -> None:
+        if self._ctx is not None:
+            lib().i360hip_destroy(self._ctx)
+            self._ctx = None
+
+    def IsInitialized(self) -> bool:
+        return self._ctx is not None
+
+    def GetGpuInfo(self) -> dict:
+        info = _abi.DeviceInfo()
+        if self._ctx is None:
+            return {"deviceId": -1}
+        lib().i360hip_get_device_info(self._ctx, C.byref(info))
+        return {"deviceId": info.device_id, "name": info.name.decode(), "totalMemory": info.total_memory,
+                "freeMemory": info.free_memory, "computeMajor": info.compute_major,
+                "computeMinor": info.compute_minor, "maxThreadsPerBlock": info.max_threads_per_block,
+                "multiProcessorCount": info.multiprocessor_count, "hasTensorCores": bool(info.has_matrix_cores),
+                "arch": info.arch.decode()}
+
+    def GetLastError(self) -> str:
+        if self._ctx is None:
+            return self._error or "Not initialized."
+        return (lib().i360hip_last_error(self._ctx) or b"").decode()
+
+    def _ready(self) -> int:
+        if self._ctx is None:
+            self._error = "Not initialized."
+            return DEVICE_ERROR
+        return SUCCESS
+
+    # ---- host-array operations (synchronous)
+    def WarpPerspective(self, src, srcW, srcH, dst, dstW, dstH, homography, offsetX, offsetY) -> int:
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        src = _c_contig(src, np.uint16)
+        dst = _c_contig(dst, np.uint16)
+        m = np.ascontiguousarray(homography, dtype=np.float32).reshape(9)
+        return lib().i360hip_warp_perspective_u16(self._ctx, _ptr(src), srcW, srcH, _ptr(dst), dstW, dstH, _ptr(m),
+                                                 offsetX, offsetY)
+
+    def MedianStack(self, inputs, numInputs, output, width, height, sigmaThreshold=2.0) -> int:
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        planes = [_c_contig(p, np.uint16) for p in inputs[:numInputs]]
+        arr = (C.c_void_p * max(1, len(planes)))(*[_ptr(p) for p in planes])
+        output = _c_contig(output, np.uint16)
+        return lib().i360hip_median_stack_u16(self._ctx, arr, numInputs, _ptr(output), width, height,
+                                              float(sigmaThreshold))
+
+    def ApplyGain(self, data, numPixels, gain) -> int:
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        data = _c_contig(data, np.uint16)
+        return lib().i360hip_apply_gain_u16(self._ctx, _ptr(data), numPixels, float(gain))
+
+    def DemosaicBayer(self, cfaData, width, height, blackLevel, camMul, rgbCam, filters, rgbOut) -> int:
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        cfa = _c_contig(cfaData, np.uint16)
+        cm = np.ascontiguousarray(camMul, dtype=np.float32).reshape(4)
+        rc_ = np.ascontiguousarray(rgbCam, dtype=np.float32).reshape(12)
+        out = _c_contig(rgbOut, np.uint16)
+        return lib().i360hip_demosaic_bayer_u16(self._ctx, _ptr(cfa), width, height, blackLevel, _ptr(cm), _ptr(rc_),
+                                                filters, _ptr(out))
+
+    def BlendChunkContributors(self, warpedBuffers, width, height, outResult, numBands=DEFAULT_BANDS,
+                               ownerOut=None) -> int:
+        """Kernels::SeamBlend::BlendChunkContributors (SeamBlendKernels.h:91-92)."""
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        bufs = [_c_contig(b, np.uint16) for b in warpedBuffers]
+        arr = (C.c_void_p * max(1, len(bufs)))(*[_ptr(b) for b in bufs])
+        out = _c_contig(outResult, np.uint16)
+        own = None if ownerOut is None else _ptr(_c_contig(ownerOut, np.int32))
+        return lib().i360hip_blend_chunk_u16(self._ctx, arr, len(bufs), width, height, numBands, _ptr(out), own)
+
+    # Align / Optimize / burst operations are not on this backend's path: the
+    # reference GPU backends answer NOT_SUPPORTED for the burst family
+    # (CudaPipeline.cpp:794-839); hosts delegate the rest to their CPU backend.
+    def BlockMatchAlign(self, *a, **k) -> int:
+        self._error = "BlockMatchAlign is not supported by the HIP backend."
+        return NOT_SUPPORTED
+
+    RobustMergeAccumulate = TileFftMerge = StructureTensorKernelRegression = BlockMatchAlign
+
+    # ---- device-pointer operations (asynchronous on `stream`)
+    def median_stack_dev(self, input_ptrs, output_ptr, num_scalars, sigma=2.0, stream=None) -> int:
+        arr = (C.c_void_p * max(1, len(input_ptrs)))(*input_ptrs)
+        return lib().i360hip_median_stack_u16_dev(self._ctx, arr, len(input_ptrs), output_ptr, num_scalars,
+                                                  float(sigma), stream)
+
+    def demosaic_dev(self, cfa_ptr, width, height, black, cam_mul, rgb_cam, filters, out_ptr, f32=False,
+                     stream=None) -> int:
+        cm = np.ascontiguousarray(cam_mul, dtype=np.float32).reshape(4)
+        rc_ = np.ascontiguousarray(rgb_cam, dtype=np.float32).reshape(12)
+        fn = lib().i360hip_demosaic_bayer_f32_dev if f32 else lib().i360hip_demosaic_bayer_u16_dev
+        return fn(self._ctx, cfa_ptr, width, height, black, _ptr(cm), _ptr(rc_), filters, out_ptr, stream)
+
+    def warp_dev(self, src_ptr, srcW, srcH, dst_ptr, dstW, dstH, inv_h, offX, offY, stream=None) -> int:
+        m = np.ascontiguousarray(inv_h, dtype=np.float32).reshape(9)
+        return lib().i360hip_warp_perspective_u16_dev(self._ctx, src_ptr, srcW, srcH, dst_ptr, dstW, dstH, _ptr(m),
+                                                      offX, offY, stream)
+
+    def gain_dev(self, ptr, n, gain, stream=None) -> int:
+        return lib().i360hip_apply_gain_u16_dev(self._ctx, ptr, n, float(gain), stream)
+
+    def blend_dev(self, warped_ptrs, width, height, out_ptr, num_bands=DEFAULT_BANDS, owner_ptr=None,
+                  stream=None) -> int:
+        arr = (C.c_void_p * max(1, len(warped_ptrs)))(*warped_ptrs)
+        return lib().i360hip_blend_chunk_u16_dev(self._ctx, arr, len(warped_ptrs), width, height, num_bands,
+                                                 out_ptr, owner_ptr, stream)
+
+    def check(self, rc: int, where: str) -> None:
+        if rc != SUCCESS:
+            raise HipError(rc, where, self.GetLastError())
+
+    def __enter__(self):
+        rc = self.Initialize()
+        if rc != SUCCESS:
+            raise HipError(rc, "Initialize", self.GetLastError())
+        return self
+
+    def __exit__(self, *exc):
+        self.Shutdown()
+
+    def __del__(self):
+        try:
+            self.Shutdown()
+        except Exception:
+            pass

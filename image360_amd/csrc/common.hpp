@@ -1,0 +1,82 @@
+// Internal shared definitions of the HIP backend (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "i360hip.h"
+
+namespace i360 {
+
+// A device buffer owned by a context, grown on demand and reused.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Launch-time status of one operation: the first failure wins.
+struct Status {
+    int code = I360HIP_SUCCESS;
+    char msg[512] = {};
+    bool ok() const { return code == I360HIP_SUCCESS; }
+    void set(int c, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+};
+
+inline const char* HipName(hipError_t e) { return hipGetErrorString(e); }
+
+// Maps a HIP error onto the reference's result codes.
+inline int CodeOf(hipError_t e) {
+    switch (e) {
+        case hipSuccess: return I360HIP_SUCCESS;
+        case hipErrorOutOfMemory: return I360HIP_OUT_OF_MEMORY;
+        case hipErrorInvalidValue:
+        case hipErrorInvalidDevicePointer: return I360HIP_INVALID_PARAM;
+        case hipErrorLaunchFailure:
+        case hipErrorInvalidConfiguration:
+        case hipErrorInvalidDeviceFunction: return I360HIP_KERNEL_LAUNCH_FAILED;
+        case hipErrorNoDevice:
+        case hipErrorInvalidDevice: return I360HIP_NO_GPU;
+        default: return I360HIP_DEVICE_ERROR;
+    }
+}
+
+#define I360_TRY(st, expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            (st).set(::i360::CodeOf(e_), "%s failed: %s (%s:%d)", #expr, ::i360::HipName(e_), \
+                     __FILE__, __LINE__);                                                    \
+            return (st).code;                                                                \
+        }                                                                                    \
+    } while (0)
+
+// Grid size for `n` items at `per_block` items per block, capped by the
+// 2^31-1 x-dimension limit.
+inline unsigned GridFor(size_t n, size_t per_block) {
+    size_t g = (n + per_block - 1) / per_block;
+    return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+}  // namespace i360
+
+struct i360hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    char last_error[512] = {};
+    hipDeviceProp_t props{};
+    std::vector<i360::DevBuf> bufs;  // indexed scratch slots
+    std::vector<i360::DevBuf> pinned;
+};
+
+namespace i360 {
+// Context-owned device buffer `slot` of at least `bytes` bytes (grown on
+// demand, kept until the context is destroyed).
+void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st);
+}  // namespace i360

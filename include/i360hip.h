@@ -1,0 +1,151 @@
+/*
+ * i360hip.h -- C ABI of the MI355X (gfx950) compute backend for Image360.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int result codes.
+ * Every entry point that takes host pointers is synchronous (results are
+ * valid on return), mirroring the contract of
+ * WindowsApp::Compute::IComputeBackend (WindowsApp.Compute/HeaderFiles/
+ * IComputeBackend.h:16-169).  Entry points ending in `_dev` take device
+ * pointers and a hipStream_t (passed as void*, NULL = the context's own
+ * stream) and are asynchronous on that stream.
+ *
+ * Thread safety: a context may be shared by several host threads; calls on
+ * one context are serialised by the context (the reference's Vulkan backend
+ * does the same, VulkanPipeline.cpp:71-82).  `_dev` calls on distinct
+ * streams of one context are serialised at the host-side enqueue only.
+ *
+ * The error text of the last failing call is kept per context
+ * (i360hip_last_error), like IComputeBackend::GetLastError.
+ */
+#ifndef I360HIP_H
+#define I360HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
+#define I360HIP_ABI_VERSION 1
+
+/* Result codes: the values of WindowsApp::Compute::ComputeResult
+ * (WindowsApp.Compute/HeaderFiles/ComputeTypes.h:202-220). */
+enum {
+    I360HIP_SUCCESS = 0,
+    I360HIP_NO_GPU = 1,
+    I360HIP_INVALID_PARAM = 2,
+    I360HIP_OUT_OF_MEMORY = 3,
+    I360HIP_KERNEL_LAUNCH_FAILED = 4,
+    I360HIP_DEVICE_ERROR = 5, /* ComputeResult::CUDA_ERROR, also "not initialised" */
+    I360HIP_NOT_SUPPORTED = 6
+};
+
+/* Limits shared with the reference. */
+#define I360HIP_MAX_STACK_INPUTS 32   /* MedianStack_Scalar.cpp:11, CpuComputeBackend.cpp:143 */
+#define I360HIP_MAX_CONTRIBUTORS 32   /* RenderExecutor.cpp:15 */
+#define I360HIP_DEFAULT_BANDS 5       /* SeamBlendKernels.h:10 */
+#define I360HIP_MAX_BANDS 12
+
+typedef struct i360hip_ctx i360hip_ctx;
+
+/* Device identity, the fields of WindowsApp::Compute::GpuInfo
+ * (ComputeTypes.h:188-199) plus the gfx architecture name. */
+typedef struct i360hip_device_info {
+    int device_id;
+    char name[256];
+    size_t total_memory;
+    size_t free_memory;
+    int compute_major;
+    int compute_minor;
+    int max_threads_per_block;
+    int multiprocessor_count;
+    int has_matrix_cores;
+    char arch[64];
+} i360hip_device_info;
+
+/* ---- lifetime (IComputeBackend::Initialize/Shutdown/GetGpuInfo/GetLastError,
+ *      IComputeBackend.h:21-25) ---------------------------------------- */
+
+/* Creates a context on `device`.  Returns I360HIP_NO_GPU when no gfx950
+ * device is present or the device index is out of range. */
+int i360hip_create(int device, i360hip_ctx** out_ctx);
+void i360hip_destroy(i360hip_ctx* ctx);
+int i360hip_get_device_info(i360hip_ctx* ctx, i360hip_device_info* out);
+/* Last error text of `ctx`; with ctx == NULL, the last i360hip_create error. */
+const char* i360hip_last_error(const i360hip_ctx* ctx);
+/* Library build identification (gfx target, ABI version). */
+const char* i360hip_version(void);
+
+/* ---- IComputeBackend operations, host pointers, synchronous ------------ */
+
+/* IComputeBackend::WarpPerspective (IComputeBackend.h:29-32); semantics of
+ * WarpPerspective_Scalar.cpp:8-45: `inv_h` maps destination -> source
+ * (row-major, 9 floats); the destination pixel (x,y) samples source point
+ * inv_h * (x+offset_x, y+offset_y, 1); bilinear RGB48, (0,0,0) where the
+ * sample is out of bounds or |w| < 1e-10.  Bit-exact with the Scalar tier. */
+int i360hip_warp_perspective_u16(i360hip_ctx* ctx, const uint16_t* src, int src_w, int src_h, uint16_t* dst,
+                                 int dst_w, int dst_h, const float* inv_h, int offset_x, int offset_y);
+
+/* IComputeBackend::MedianStack (IComputeBackend.h:35-38): sigma-clipped
+ * upper median of width*height*3 independent scalars across 1..32 planes
+ * (MedianStackKernels.h:19-55).  Bit-exact. */
+int i360hip_median_stack_u16(i360hip_ctx* ctx, const uint16_t* const* inputs, int num_inputs, uint16_t* output,
+                             int width, int height, float sigma_threshold);
+
+/* IComputeBackend::ApplyGain (IComputeBackend.h:41-42): in place,
+ * u16(clamp(v*gain)); no-op when |gain-1| < 1e-6 (GainColorOps.cpp:16-26).
+ * `num_values` counts u16 values (the reference's numPixels argument). */
+int i360hip_apply_gain_u16(i360hip_ctx* ctx, uint16_t* data, int num_values, float gain);
+
+/* IComputeBackend::DemosaicBayer (IComputeBackend.h:46-49): black level,
+ * white balance, same-colour 3x3 bilinear, 3x4 colour matrix
+ * (BayerDemosaic_Scalar.cpp:9-50).  cam_mul: 4 floats; rgb_cam: 12 floats
+ * (row-major 3x4).  Bit-exact. */
+int i360hip_demosaic_bayer_u16(i360hip_ctx* ctx, const uint16_t* cfa, int width, int height, uint16_t black_level,
+                               const float* cam_mul, const float* rgb_cam, uint32_t filters, uint16_t* rgb_out);
+
+/* Kernels::SeamBlend::BlendChunkContributors (SeamBlendKernels.h:91-92,
+ * SeamBlend.cpp:549-587) on RGB48 warped contributors: coverage, chamfer
+ * distance, ownership, pairwise DP seams, multi-band compose.  The
+ * reference fixes num_bands to 5 (pass I360HIP_DEFAULT_BANDS for parity).
+ * owner_out (optional, width*height int32) receives the final owner map.
+ * Bit-exact with the reference CPU implementation. */
+int i360hip_blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int num_contributors, int width,
+                            int height, int num_bands, uint16_t* out, int32_t* owner_out);
+
+/* ---- device-pointer forms (asynchronous on `stream`) ------------------- */
+
+/* `inputs` is a HOST array of num_inputs DEVICE pointers; `num_scalars`
+ * counts u16 values per plane. */
+int i360hip_median_stack_u16_dev(i360hip_ctx* ctx, const uint16_t* const* inputs, int num_inputs, uint16_t* output,
+                                 size_t num_scalars, float sigma_threshold, void* stream);
+int i360hip_demosaic_bayer_u16_dev(i360hip_ctx* ctx, const uint16_t* cfa, int width, int height,
+                                   uint16_t black_level, const float* cam_mul, const float* rgb_cam,
+                                   uint32_t filters, uint16_t* rgb_out, void* stream);
+/* FP32 RGB output: the colour-matrix value clamped to [0,65535] before the
+ * u16 truncation (truncating it gives the u16 entry point's output). */
+int i360hip_demosaic_bayer_f32_dev(i360hip_ctx* ctx, const uint16_t* cfa, int width, int height,
+                                   uint16_t black_level, const float* cam_mul, const float* rgb_cam,
+                                   uint32_t filters, float* rgb_out, void* stream);
+int i360hip_warp_perspective_u16_dev(i360hip_ctx* ctx, const uint16_t* src, int src_w, int src_h, uint16_t* dst,
+                                     int dst_w, int dst_h, const float* inv_h, int offset_x, int offset_y,
+                                     void* stream);
+int i360hip_apply_gain_u16_dev(i360hip_ctx* ctx, uint16_t* data, size_t num_values, float gain, void* stream);
+/* `warped` is a HOST array of DEVICE pointers. */
+int i360hip_blend_chunk_u16_dev(i360hip_ctx* ctx, const uint16_t* const* warped, int num_contributors, int width,
+                                int height, int num_bands, uint16_t* out, int32_t* owner_out, void* stream);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* I360HIP_H */

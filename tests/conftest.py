@@ -22,3 +22,15 @@ def golden():
         return np.load(os.path.join(d, name + ".npz"), allow_pickle=False)
 
     return load
+
+
+@pytest.fixture(scope="session")
+def hip():
+    """One HIP backend context for the GPU tests (fails loudly without the library or a gfx950 GPU)."""
+    from image360_amd import HipComputeBackend
+
+    be = HipComputeBackend(0)
+    rc = be.Initialize()
+    assert rc == 0, f"HipComputeBackend.Initialize failed: {rc} {be.GetLastError()}"
+    yield be
+    be.Shutdown()

@@ -1,0 +1,74 @@
+"""GPU SeamBlend::BlendChunkContributors against the reference's own outputs
+(golden vectors) and the oracle: bit-exact owner maps and RGB48 output."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.gpu_util import note
+
+pytestmark = pytest.mark.gpu
+
+
+def _blend(hip, bufs, bands=5):
+    h, w = bufs[0].shape[:2]
+    out = np.zeros((h, w, 3), np.uint16)
+    owner = np.zeros((h, w), np.int32)
+    hip.check(hip.BlendChunkContributors([np.ascontiguousarray(b) for b in bufs], w, h, out, bands, owner), "blend")
+    return out, owner
+
+
+@pytest.mark.parametrize("scene", ["ghost", "gap", "route", "c0", "c1", "c2", "c3"])
+def test_blend_golden(hip, golden, scene):
+    g = golden("blend")
+    note(f"blend golden {scene}")
+    out, owner = _blend(hip, list(g[f"{scene}_in"]))
+    assert np.array_equal(owner, g[f"{scene}_owner"])
+    assert np.array_equal(out, g[f"{scene}_out"])
+
+
+def test_blend_seven_bands(hip, golden):
+    g = golden("compose7")
+    note("blend 7 bands")
+    out, owner = _blend(hip, list(g["inp"]), 7)
+    assert np.array_equal(owner, g["owner"])
+    assert np.array_equal(out, g["out"])
+
+
+def test_blend_empty_and_single(hip):
+    note("blend c=0 / c=1")
+    out = np.ones((8, 9, 3), np.uint16)
+    assert hip.BlendChunkContributors([], 9, 8, out) == 0
+    assert not out.any()
+    img = np.random.Generator(np.random.PCG64(2)).integers(1, 60000, size=(33, 47, 3)).astype(np.uint16)
+    o, own = _blend(hip, [img])
+    assert np.array_equal(o, po.blend_chunk([img]))
+
+
+def test_blend_render_behaviour(hip, golden):
+    # tests/render_blend/main.cpp:113-115, 145-153, 193 on the GPU output
+    g = golden("blend")
+    out, _ = _blend(hip, list(g["ghost_in"]))
+    obj = np.array([60000, 2000, 2000])
+    assert np.abs(out[20, 165].astype(int) - obj).sum() < 3000
+    assert np.abs(out[20, 235].astype(int) - obj).sum() < 3000
+    out, _ = _blend(hip, list(g["gap_in"]))
+    assert not (out == 0).all(axis=2).any()
+
+
+@pytest.mark.parametrize("seed,cw,ch,c", [(1, 257, 190, 3), (2, 190, 300, 4), (3, 400, 64, 2), (4, 128, 128, 6)])
+def test_blend_random_vs_oracle(hip, seed, cw, ch, c):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    bufs = []
+    for i in range(c):
+        sw, sh = int(rng.integers(cw // 2, cw)), int(rng.integers(ch // 2, ch))
+        img = np.clip(rng.normal(30000, 6000, size=(sh, sw, 3)), 1, 65535).astype(np.uint16)
+        th = rng.uniform(-0.1, 0.1)
+        fwd = np.array([[np.cos(th), -np.sin(th), rng.uniform(-sw / 3, cw - sw / 2)],
+                        [np.sin(th), np.cos(th), rng.uniform(-sh / 3, ch - sh / 2)], [0, 0, 1]])
+        inv = np.linalg.inv(fwd).astype(np.float32)
+        bufs.append(po.warp_perspective(img, cw, ch, inv))
+    note(f"blend random seed={seed} {cw}x{ch} c={c}")
+    want, want_owner = po.blend_chunk(bufs, 5, want_owner=True)
+    out, owner = _blend(hip, bufs)
+    assert np.array_equal(owner, want_owner)
+    assert np.array_equal(out, want)
