@@ -37,6 +37,10 @@ class DeviceInfo(C.Structure):
                 ("has_matrix_cores", C.c_int), ("arch", C.c_char * 64)]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("launches", C.c_int), ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "i360hip_create": (i32, [i32, C.POINTER(vp)]),
@@ -44,6 +48,8 @@ SIGNATURES = {
     "i360hip_get_device_info": (i32, [vp, C.POINTER(DeviceInfo)]),
     "i360hip_last_error": (C.c_char_p, [vp]),
     "i360hip_version": (C.c_char_p, []),
+    "i360hip_profile_enable": (i32, [vp, i32]),
+    "i360hip_profile_read": (i32, [vp, C.POINTER(KernelStat), i32, C.POINTER(i32)]),
     "i360hip_warp_perspective_u16": (i32, [vp, vp, i32, i32, vp, i32, i32, vp, i32, i32]),
     "i360hip_median_stack_u16": (i32, [vp, vp, i32, vp, i32, i32, C.c_float]),
     "i360hip_apply_gain_u16": (i32, [vp, vp, i32, C.c_float]),
@@ -84,6 +90,14 @@ class HipError(RuntimeError):
 def lib():
     global _LIB
     if _LIB is None:
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7;
+        # when torch is importable, load it first so this library binds to
+        # the same runtime (same soname) instead of a second copy, which
+        # would leave torch without devices.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise HipLibraryMissing(f"{LIB_PATH} is not built; run `python -m image360_amd.build`")
         L = C.CDLL(LIB_PATH)

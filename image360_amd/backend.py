@@ -162,6 +162,17 @@ class HipComputeBackend:
         return lib().i360hip_blend_chunk_u16_dev(self._ctx, arr, len(warped_ptrs), width, height, num_bands,
                                                  out_ptr, owner_ptr, stream)
 
+    def profile_enable(self, on=True) -> None:
+        lib().i360hip_profile_enable(self._ctx, 1 if on else 0)
+
+    def profile_read(self) -> dict:
+        """{kernel: {"launches", "total_ms", "alg_bytes"}} from event timing."""
+        arr = (_abi.KernelStat * 64)()
+        n = C.c_int(0)
+        lib().i360hip_profile_read(self._ctx, arr, 64, C.byref(n))
+        return {arr[i].name.decode(): {"launches": arr[i].launches, "total_ms": arr[i].total_ms,
+                                       "alg_bytes": arr[i].alg_bytes} for i in range(n.value)}
+
     def check(self, rc: int, where: str) -> None:
         if rc != SUCCESS:
             raise HipError(rc, where, self.GetLastError())

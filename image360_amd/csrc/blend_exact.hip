@@ -614,8 +614,9 @@ void blend_release(i360hip_ctx* ctx) {
     }
 }
 
-#define BL_LAUNCH(st, ...)                                                                     \
+#define BL_LAUNCH(st, NAME, BYTES, ...)                                                        \
     do {                                                                                       \
+        ProfScope ps_(ctx, s, NAME, static_cast<double>(BYTES));                               \
         __VA_ARGS__;                                                                           \
         hipError_t e_ = hipGetLastError();                                                     \
         if (e_ != hipSuccess) {                                                                \
@@ -628,8 +629,8 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
                      uint16_t* out, int32_t* owner_out, hipStream_t s, Status& st) {
     const size_t n = static_cast<size_t>(w) * h;
     if (c == 0) {
-        BL_LAUNCH(st, zero_u16_kernel<<<GridFor(n * 3, 256), 256, 0, s>>>(out, n * 3));
-        if (owner_out) BL_LAUNCH(st, fill_i32_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner_out, n, -1));
+        BL_LAUNCH(st, "zero_u16", n * 6, zero_u16_kernel<<<GridFor(n * 3, 256), 256, 0, s>>>(out, n * 3));
+        if (owner_out) BL_LAUNCH(st, "fill_i32", n * 4, fill_i32_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner_out, n, -1));
         return;
     }
     if (w > 16384) {
@@ -665,10 +666,10 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     for (int i = 0; i < c; ++i) imgs.p[i] = warped[i];
 
     // 1. coverage + boxes
-    BL_LAUNCH(st, box_init_kernel<<<1, 64, 0, s>>>(boxes, c));
+    BL_LAUNCH(st, "box_init", 16 * c, box_init_kernel<<<1, 64, 0, s>>>(boxes, c));
     {
         const unsigned gx = GridFor(n, 256) < 1024 ? GridFor(n, 256) : 1024;
-        BL_LAUNCH(st, coverage_u16_kernel<<<dim3(gx, c), 256, 0, s>>>(imgs, w, h, cov, boxes));
+        BL_LAUNCH(st, "coverage_u16", 7.0 * n * c, coverage_u16_kernel<<<dim3(gx, c), 256, 0, s>>>(imgs, w, h, cov, boxes));
     }
     // 2. chamfer, forward then backward (one block per contributor)
     {
@@ -676,26 +677,26 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
         const int cols = (w + S - 1) / S;
         const int nt = ((cols + 63) / 64) * 64;
         if (S == 4) {
-            BL_LAUNCH(st, (chamfer_kernel<4, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
-            BL_LAUNCH(st, (chamfer_kernel<4, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
+            BL_LAUNCH(st, "chamfer_fwd", 5.0 * n * c, (chamfer_kernel<4, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
+            BL_LAUNCH(st, "chamfer_bwd", 8.0 * n * c, (chamfer_kernel<4, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
         } else if (S == 8) {
-            BL_LAUNCH(st, (chamfer_kernel<8, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
-            BL_LAUNCH(st, (chamfer_kernel<8, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
+            BL_LAUNCH(st, "chamfer_fwd", 5.0 * n * c, (chamfer_kernel<8, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
+            BL_LAUNCH(st, "chamfer_bwd", 8.0 * n * c, (chamfer_kernel<8, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
         } else {
-            BL_LAUNCH(st, (chamfer_kernel<16, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
-            BL_LAUNCH(st, (chamfer_kernel<16, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
+            BL_LAUNCH(st, "chamfer_fwd", 5.0 * n * c, (chamfer_kernel<16, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
+            BL_LAUNCH(st, "chamfer_bwd", 8.0 * n * c, (chamfer_kernel<16, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
         }
     }
     // 3. ownership
-    BL_LAUNCH(st, ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, c, n, owner));
+    BL_LAUNCH(st, "ownership", 4.0 * n * (c + 1), ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, c, n, owner));
     // 4. pairwise seams in the reference's order
     for (int i = 0; i < c; ++i)
         for (int j = i + 1; j < c; ++j)
-            BL_LAUNCH(st, seam_dp_kernel<<<1, 1024, 0, s>>>(
+            BL_LAUNCH(st, "seam_dp", 0, seam_dp_kernel<<<1, 1024, 0, s>>>(
                               warped[i], warped[j], cov + n * i, cov + n * j, boxes, i, j, w, owner,
                               reinterpret_cast<int*>(base + oLohi), reinterpret_cast<int*>(base + oPrev),
                               reinterpret_cast<int*>(base + oChosen), base + oBack));
-    if (owner_out) BL_LAUNCH(st, copy_owner_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner, owner_out, n));
+    if (owner_out) BL_LAUNCH(st, "copy_owner", 8.0 * n, copy_owner_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner, owner_out, n));
     // 5. pyramids of every contributor (contributors without owned pixels
     // carry all-zero weights and drop out of the sums exactly)
     PyrArgs pa{};
@@ -712,8 +713,8 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
             }
         }
         const dim3 grid((lw[l] + kRedTW - 1) / kRedTW, (lh[l] + kRedTH - 1) / kRedTH, c);
-        if (l == 1) BL_LAUNCH(st, reduce_kernel<true><<<grid, 256, 0, s>>>(pa, lw[0], lh[0], lw[1], lh[1]));
-        else BL_LAUNCH(st, reduce_kernel<false><<<grid, 256, 0, s>>>(pa, lw[l - 1], lh[l - 1], lw[l], lh[l]));
+        if (l == 1) BL_LAUNCH(st, "reduce_l0", (11.0 * n + 20.0 * lw[1] * lh[1]) * c, reduce_kernel<true><<<grid, 256, 0, s>>>(pa, lw[0], lh[0], lw[1], lh[1]));
+        else BL_LAUNCH(st, "reduce", 20.0 * c * (1.0 * lw[l - 1] * lh[l - 1] + 1.0 * lw[l] * lh[l]), reduce_kernel<false><<<grid, 256, 0, s>>>(pa, lw[l - 1], lh[l - 1], lw[l], lh[l]));
     }
     // 6. compose top-down, then level 0
     for (int l = L - 1; l >= 1; --l) {
@@ -727,7 +728,7 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
         ca.recon_next = top ? nullptr : reinterpret_cast<const float4*>(base + oRec[l + 1]);
         ca.recon = reinterpret_cast<float4*>(base + oRec[l]);
         const dim3 grid((lw[l] + 63) / 64, (lh[l] + 3) / 4);
-        BL_LAUNCH(st, compose_level_kernel<<<grid, 256, 0, s>>>(ca, c, lw[l], lh[l], top ? 1 : lw[l + 1],
+        BL_LAUNCH(st, "compose_level", 1.0 * lw[l] * lh[l] * (20.0 * c + 16.0) + (top ? 0.0 : 1.0 * lw[l + 1] * lh[l + 1] * (16.0 * c + 16.0)), compose_level_kernel<<<grid, 256, 0, s>>>(ca, c, lw[l], lh[l], top ? 1 : lw[l + 1],
                                                                top ? 1 : lh[l + 1], top));
     }
     FinalArgs fa{};
@@ -740,7 +741,7 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     fa.owner = owner;
     fa.out = out;
     const dim3 grid((w + 63) / 64, (h + 3) / 4);
-    BL_LAUNCH(st, final_level0_kernel<<<grid, 256, 0, s>>>(fa, w, h, L > 1 ? lw[1] : 1, L > 1 ? lh[1] : 1, L));
+    BL_LAUNCH(st, "final_level0", 17.0 * n + (L > 1 ? 32.0 * lw[1] * lh[1] : 0.0), final_level0_kernel<<<grid, 256, 0, s>>>(fa, w, h, L > 1 ? lw[1] : 1, L > 1 ? lh[1] : 1, L));
 }
 
 }  // namespace i360

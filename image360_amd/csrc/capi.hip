@@ -180,6 +180,7 @@ int i360hip_warp_perspective_u16_dev(i360hip_ctx* ctx, const uint16_t* src, int 
     I360_TRY(st, hipSetDevice(ctx->device));
     Mat3f m;
     std::memcpy(m.v, inv_h, sizeof(m.v));
+    ProfScope ps(ctx, StreamOf(ctx, stream), "warp_u16", 12.0 * dw * dh);
     hipError_t e = launch_warp_u16(src, sw, sh, dst, dw, dh, m, ox, oy, StreamOf(ctx, stream));
     if (e != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "warp launch: %s", HipName(e));
     return Finish(ctx, st);
@@ -226,6 +227,7 @@ int i360hip_median_stack_u16_dev(i360hip_ctx* ctx, const uint16_t* const* inputs
     Status st;
     auto run = [&]() -> int {
         I360_TRY(st, hipSetDevice(ctx->device));
+        ProfScope ps(ctx, StreamOf(ctx, stream), "median_stack", 2.0 * num_scalars * (num_inputs + 1));
         I360_TRY(st, launch_median_stack(pt, num_inputs, output, num_scalars, sigma, StreamOf(ctx, stream)));
         return st.code;
     };
@@ -275,6 +277,7 @@ int i360hip_apply_gain_u16_dev(i360hip_ctx* ctx, uint16_t* data, size_t n, float
     Status st;
     auto run = [&]() -> int {
         I360_TRY(st, hipSetDevice(ctx->device));
+        ProfScope ps(ctx, StreamOf(ctx, stream), "gain_u16", 4.0 * n);
         I360_TRY(st, launch_gain_u16(data, n, gain, StreamOf(ctx, stream)));
         return st.code;
     };
@@ -318,6 +321,8 @@ static int DemosaicDev(i360hip_ctx* ctx, const uint16_t* cfa, int w, int h, uint
     Status st;
     auto run = [&]() -> int {
         I360_TRY(st, hipSetDevice(ctx->device));
+        ProfScope ps(ctx, StreamOf(ctx, stream), f32 ? "demosaic_f32" : "demosaic_u16",
+                     (f32 ? 14.0 : 8.0) * w * h);
         I360_TRY(st, launch_demosaic(cfa, w, h, black, prm, out, f32, StreamOf(ctx, stream)));
         return st.code;
     };

@@ -65,7 +65,20 @@ inline unsigned GridFor(size_t n, size_t per_block) {
 
 }  // namespace i360
 
+namespace i360 {
+// Per-launch timing records (HIP events on the launch stream), kept only
+// while profiling is enabled on the context.
+struct ProfRecord {
+    const char* name;
+    hipEvent_t start, stop;
+    double alg_bytes;
+};
+}  // namespace i360
+
 struct i360hip_ctx {
+    bool profiling = false;
+    std::vector<i360::ProfRecord> prof;
+    std::vector<hipEvent_t> event_pool;
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -79,4 +92,16 @@ namespace i360 {
 // Context-owned device buffer `slot` of at least `bytes` bytes (grown on
 // demand, kept until the context is destroyed).
 void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st);
+}  // namespace i360
+
+namespace i360 {
+// Brackets one kernel launch with events when profiling is on.
+struct ProfScope {
+    i360hip_ctx* ctx;
+    hipStream_t s;
+    ProfRecord rec{};
+    bool on;
+    ProfScope(i360hip_ctx* c, hipStream_t st, const char* name, double bytes);
+    ~ProfScope();
+};
 }  // namespace i360

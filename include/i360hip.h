@@ -81,6 +81,25 @@ const char* i360hip_last_error(const i360hip_ctx* ctx);
 /* Library build identification (gfx target, ABI version). */
 const char* i360hip_version(void);
 
+/* ---- kernel timing ------------------------------------------------------
+ * While enabled, every kernel the context launches is bracketed by HIP
+ * events on its launch stream.  i360hip_profile_read synchronises the
+ * recorded launches and returns, per kernel name, the launch count, the
+ * summed event time and the summed algorithmic bytes (inputs read once plus
+ * outputs written once at their storage precision).  No counterpart in the
+ * reference, which has no profiling hooks. */
+typedef struct i360hip_kernel_stat {
+    char name[64];
+    int launches;
+    double total_ms;
+    double alg_bytes;
+} i360hip_kernel_stat;
+
+/* enable != 0 starts recording (and drops earlier records); 0 stops. */
+int i360hip_profile_enable(i360hip_ctx* ctx, int enable);
+/* Writes up to max_stats aggregated entries, sets *num_stats. */
+int i360hip_profile_read(i360hip_ctx* ctx, i360hip_kernel_stat* stats, int max_stats, int* num_stats);
+
 /* ---- IComputeBackend operations, host pointers, synchronous ------------ */
 
 /* IComputeBackend::WarpPerspective (IComputeBackend.h:29-32); semantics of

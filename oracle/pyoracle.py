@@ -220,6 +220,14 @@ def ref_blend_chunk(warped):
     return out
 
 
+def ref_blend_chunk_bands(warped, num_bands):
+    warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
+    h, w = warped[0].shape[:2]
+    out = np.zeros((h, w, 3), np.uint16)
+    ref().ref_blend_chunk_bands(_ptr_array(warped), len(warped), w, h, num_bands, _p(out, u16p))
+    return out
+
+
 def ref_blend_owner(warped):
     warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
     h, w = warped[0].shape[:2]

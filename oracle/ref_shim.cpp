@@ -79,6 +79,26 @@ void ref_multiband_compose(const uint16_t* const* warped, const float* const* di
     for (size_t i = 0; i < res.size(); ++i) out[i] = res[i];
 }
 
+// BlendChunkContributors with a chosen band count: the same Detail calls in
+// the order of SeamBlend.cpp:549-587, ending in MultiBandCompose(numBands).
+void ref_blend_owner(const uint16_t* const* warped, int c, int w, int h, int32_t* ownerOut);
+void ref_blend_chunk_bands(const uint16_t* const* warped, int c, int w, int h, int numBands, uint16_t* out) {
+    size_t n = static_cast<size_t>(w) * h;
+    std::vector<int32_t> owner(n);
+    ref_blend_owner(warped, c, w, h, owner.data());
+    std::vector<const unsigned short*> ptrs(warped, warped + c);
+    std::vector<std::vector<float>> dist(c);
+    for (int i = 0; i < c; ++i) {
+        std::vector<unsigned char> cov(n);
+        for (size_t p = 0; p < n; ++p) cov[p] = (warped[i][p * 3] | warped[i][p * 3 + 1] | warped[i][p * 3 + 2]) ? 1 : 0;
+        K::SeamBlend::Detail::ComputeCoverageDistance(cov, w, h, dist[i]);
+    }
+    std::vector<int> own(owner.begin(), owner.end());
+    std::vector<unsigned short> res;
+    K::SeamBlend::Detail::MultiBandCompose(ptrs, dist, own, w, h, numBands, res);
+    for (size_t i = 0; i < res.size(); ++i) out[i] = res[i];
+}
+
 // Full BlendChunkContributors pipeline, but returning the owner map that
 // feeds MultiBandCompose (reassembled from the public Detail functions in
 // the same order as SeamBlend.cpp:549-587).
