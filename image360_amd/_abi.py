@@ -41,6 +41,21 @@ class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("launches", C.c_int), ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
 
 
+class Frame(C.Structure):
+    _fields_ = [("pixels", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("format", C.c_int),
+                ("gain", C.c_float), ("h", C.c_double * 9)]
+
+
+class ChunkDesc(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("width", C.c_int), ("height", C.c_int), ("first", C.c_int),
+                ("count", C.c_int), ("out", C.c_void_p), ("owner", C.c_void_p)]
+
+
+PIX_F16, PIX_F32 = 0, 1
+RENDER_FP64, RENDER_PLAIN, RENDER_PRECISION, RENDER_BATCHED = 0, 1, 2, 3
+RENDER_NAMES = {0: "fp64", 1: "plain", 2: "precision", 3: "batched"}
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "i360hip_create": (i32, [i32, C.POINTER(vp)]),
@@ -52,6 +67,10 @@ SIGNATURES = {
     "i360hip_profile_read": (i32, [vp, C.POINTER(KernelStat), i32, C.POINTER(i32)]),
     "i360hip_warp_perspective_u16": (i32, [vp, vp, i32, i32, vp, i32, i32, vp, i32, i32]),
     "i360hip_median_stack_u16": (i32, [vp, vp, i32, vp, i32, i32, C.c_float]),
+    "i360hip_render_chunks": (i32, [vp, C.POINTER(Frame), i32, C.POINTER(ChunkDesc), i32, C.POINTER(i32), i32, i32,
+                                    vp]),
+    "i360hip_render_map_error": (i32, [vp, C.POINTER(Frame), i32, C.POINTER(ChunkDesc), i32, C.POINTER(i32),
+                                       C.POINTER(C.c_double)]),
     "i360hip_apply_gain_u16": (i32, [vp, vp, i32, C.c_float]),
     "i360hip_demosaic_bayer_u16": (i32, [vp, vp, i32, i32, C.c_uint16, vp, vp, C.c_uint32, vp]),
     "i360hip_blend_chunk_u16": (i32, [vp, vp, i32, i32, i32, i32, vp, vp]),

@@ -162,6 +162,39 @@ class HipComputeBackend:
         return lib().i360hip_blend_chunk_u16_dev(self._ctx, arr, len(warped_ptrs), width, height, num_bands,
                                                  out_ptr, owner_ptr, stream)
 
+    @staticmethod
+    def _render_args(frames, chunks, contributors):
+        fa = (_abi.Frame * max(1, len(frames)))()
+        for i, f in enumerate(frames):
+            fa[i].pixels = f["ptr"]
+            fa[i].width = f["w"]
+            fa[i].height = f["h"]
+            fa[i].format = f.get("fmt", _abi.PIX_F16)
+            fa[i].gain = f.get("gain", 1.0)
+            for k, v in enumerate(np.asarray(f["H"], dtype=np.float64).reshape(9)):
+                fa[i].h[k] = float(v)
+        ca = (_abi.ChunkDesc * max(1, len(chunks)))()
+        for i, c in enumerate(chunks):
+            ca[i].x, ca[i].y, ca[i].width, ca[i].height = c["x"], c["y"], c["w"], c["h"]
+            ca[i].first, ca[i].count = c["first"], c["count"]
+            ca[i].out = c["out"]
+            ca[i].owner = c.get("owner") or None
+        cont = (C.c_int * max(1, len(contributors)))(*[int(v) for v in contributors])
+        return fa, ca, cont
+
+    def render_chunks(self, frames, chunks, contributors, num_bands, mode, stream=None) -> int:
+        """i360hip_render_chunks: frames = [{ptr, w, h, fmt, gain, H}], chunks =
+        [{x, y, w, h, first, count, out, owner}] (device pointers)."""
+        fa, ca, cont = self._render_args(frames, chunks, contributors)
+        return lib().i360hip_render_chunks(self._ctx, fa, len(frames), ca, len(chunks), cont, num_bands, mode, stream)
+
+    def render_map_error(self, frames, chunks, contributors) -> float:
+        fa, ca, cont = self._render_args(frames, chunks, contributors)
+        out = C.c_double(0)
+        self.check(lib().i360hip_render_map_error(self._ctx, fa, len(frames), ca, len(chunks), cont, C.byref(out)),
+                   "render_map_error")
+        return out.value
+
     def profile_enable(self, on=True) -> None:
         lib().i360hip_profile_enable(self._ctx, 1 if on else 0)
 

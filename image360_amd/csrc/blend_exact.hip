@@ -38,6 +38,8 @@
 #include <cstring>
 
 #include "blend.hpp"
+#include "chamfer.hpp"
+#include "seam.hpp"
 #include "ops.hpp"
 
 namespace i360 {
@@ -84,276 +86,22 @@ __global__ void box_init_kernel(int* boxes, int c) {
     if (i < c) { boxes[4 * i] = INT_MAX; boxes[4 * i + 1] = INT_MAX; boxes[4 * i + 2] = -1; boxes[4 * i + 3] = -1; }
 }
 
-// ------------------------------------------------------------- chamfer --
-// One block per contributor.  FWD: init from coverage (kInf / 0), raster
-// order.  BWD: init from the forward result, reverse raster order.
-template <int S, bool FWD>
-__global__ void __launch_bounds__(1024) chamfer_kernel(const uint8_t* __restrict__ cov_all, float* __restrict__ dist_all,
-                                                       int w, int h) {
-    __shared__ float firstv[2][1024];
-    __shared__ float lastv[2][1024];
-    const int ci = blockIdx.x;
-    const size_t n = static_cast<size_t>(w) * h;
-    const uint8_t* cov = cov_all + n * ci;
-    float* d = dist_all + n * ci;
-    const int nt = blockDim.x, t = threadIdx.x;
-    const int xs = t * S;
-    const float kInf = static_cast<float>(w + h);
-    float prev[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) prev[s] = 0.0f;
-    const int tmax = 2 * (h - 1) + (nt - 1);
-    const int lag = FWD ? t : (nt - 1 - t);
-    for (int T = 0; T <= tmax; ++T) {
-        const int k = T - lag;
-        if (k >= 0 && (k & 1) == 0 && (k >> 1) < h && xs < w) {
-            const int r = k >> 1;
-            const int y = FWD ? r : (h - 1 - r);
-            const int pb = (r - 1) & 1, cb = r & 1;
-            float cur[S];
-            float* row = d + static_cast<size_t>(y) * w;
-            if (FWD) {
-                const float carryL = (t > 0) ? lastv[cb][t - 1] : 0.0f;
-                const float prevL = (t > 0) ? lastv[pb][t - 1] : 0.0f;
-                const float prevR = (t + 1 < nt) ? firstv[pb][t + 1] : 0.0f;
-                const uint8_t* crow = cov + static_cast<size_t>(y) * w;
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int x = xs + s;
-                    if (x >= w) { cur[s] = 0.0f; continue; }
-                    float b = crow[x] ? kInf : 0.0f;
-                    if (x > 0) b = fminf(b, (s == 0 ? carryL : cur[s - 1]) + 1.0f);
-                    if (r > 0) {
-                        b = fminf(b, prev[s] + 1.0f);
-                        if (x > 0) b = fminf(b, (s == 0 ? prevL : prev[s - 1]) + kDiag);
-                        if (x < w - 1) b = fminf(b, (s == S - 1 ? prevR : prev[s + 1]) + kDiag);
-                    }
-                    cur[s] = b;
-                }
-            } else {
-                const float carryR = (t + 1 < nt) ? firstv[cb][t + 1] : 0.0f;
-                const float prevR = (t + 1 < nt) ? firstv[pb][t + 1] : 0.0f;
-                const float prevL = (t > 0) ? lastv[pb][t - 1] : 0.0f;
-#pragma unroll
-                for (int s = S - 1; s >= 0; --s) {
-                    const int x = xs + s;
-                    if (x >= w) { cur[s] = 0.0f; continue; }
-                    float b = row[x];
-                    if (x < w - 1) b = fminf(b, (s == S - 1 ? carryR : cur[s + 1]) + 1.0f);
-                    if (r > 0) {
-                        b = fminf(b, prev[s] + 1.0f);
-                        if (x < w - 1) b = fminf(b, (s == S - 1 ? prevR : prev[s + 1]) + kDiag);
-                        if (x > 0) b = fminf(b, (s == 0 ? prevL : prev[s - 1]) + kDiag);
-                    }
-                    cur[s] = b;
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (xs + s < w) row[xs + s] = cur[s];
-                prev[s] = cur[s];
-            }
-            firstv[cb][t] = cur[0];
-            lastv[cb][t] = cur[S - 1];
-        }
-        __syncthreads();
-    }
-}
-
 // ----------------------------------------------------------- ownership --
-__global__ void __launch_bounds__(256) ownership_kernel(const float* __restrict__ dist, int c, size_t n,
-                                                        int32_t* __restrict__ owner) {
+// Distances outside a contributor's coverage box are 0 (never written).
+__global__ void __launch_bounds__(256) ownership_kernel(const float* __restrict__ dist, const int* __restrict__ boxes,
+                                                        int c, int w, size_t n, int32_t* __restrict__ owner) {
     const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const int x = static_cast<int>(p % w), y = static_cast<int>(p / w);
     int best = -1;
     float bd = 0.0f;
     for (int i = 0; i < c; ++i) {
+        const int* b = boxes + 4 * i;
+        if (x < b[0] || x > b[2] || y < b[1] || y > b[3]) continue;
         const float v = dist[n * i + p];
         if (v > bd) { bd = v; best = i; }
     }
     owner[p] = best;
-}
-
-// ------------------------------------------------------------- seam DP --
-constexpr int kSeamRadius = 24;  // SeamBlend.cpp:319
-constexpr int kCostBatch = 128;
-constexpr int kBackBatch = 256;
-
-__device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w, int line, int t) {
-    return rowScan ? static_cast<size_t>(oy0 + line) * w + (ox0 + t) : static_cast<size_t>(oy0 + t) * w + (ox0 + line);
-}
-
-__global__ void __launch_bounds__(1024) seam_dp_kernel(const uint16_t* __restrict__ imgI,
-                                                       const uint16_t* __restrict__ imgJ,
-                                                       const uint8_t* __restrict__ covI,
-                                                       const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
-                                                       int ci, int cj, int w, int32_t* __restrict__ owner,
-                                                       int* __restrict__ lohi, int* __restrict__ prevl,
-                                                       int* __restrict__ chosen, uint8_t* __restrict__ back) {
-    const int* bI = boxes + 4 * ci;
-    const int* bJ = boxes + 4 * cj;
-    if (bI[2] < 0 || bJ[2] < 0) return;  // an empty coverage box
-    const int ox0 = max(bI[0], bJ[0]), oy0 = max(bI[1], bJ[1]);
-    const int ox1 = min(bI[2], bJ[2]), oy1 = min(bI[3], bJ[3]);
-    if (ox0 > ox1 || oy0 > oy1) return;
-    const int bw = ox1 - ox0 + 1, bh = oy1 - oy0 + 1;
-    const bool rowScan = bw >= bh;
-    const float cI = rowScan ? static_cast<float>(bI[0] + bI[2]) * 0.5f : static_cast<float>(bI[1] + bI[3]) * 0.5f;
-    const float cJ = rowScan ? static_cast<float>(bJ[0] + bJ[2]) * 0.5f : static_cast<float>(bJ[1] + bJ[3]) * 0.5f;
-    const bool iLow = cI <= cJ;
-    const int lines = rowScan ? bh : bw, len = rowScan ? bw : bh;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-
-    // Phase 1: crossover of the current owner map and contested band per line.
-    for (int line = wave; line < lines; line += nwaves) {
-        int first = -1, cross = -1;
-        for (int base = 0; base < len && cross < 0; base += 64) {
-            const int t = base + lane;
-            const int o = (t < len) ? owner[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
-            const bool ij = (t < len) && (o == ci || o == cj);
-            const unsigned long long m = __ballot(ij);
-            int start = 0;
-            if (first < 0) {
-                if (m == 0ull) continue;
-                const int f = __ffsll(static_cast<long long>(m)) - 1;
-                first = __shfl(o, f);
-                start = f + 1;
-            }
-            unsigned long long m2 = __ballot(ij && o != first);
-            m2 = (start >= 64) ? 0ull : (m2 & (~0ull << start));
-            if (m2) cross = base + __ffsll(static_cast<long long>(m2)) - 1;
-        }
-        int lo = 1, hi = 0;
-        if (cross >= 0) {
-            const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
-            const int t = a + lane;
-            bool both = false;
-            if (t <= b) {
-                const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
-                both = covI[p] && covJ[p];
-            }
-            const unsigned long long m = __ballot(both);
-            if (m) {
-                lo = a + __ffsll(static_cast<long long>(m)) - 1;
-                hi = a + 63 - __clzll(static_cast<long long>(m));
-            }
-        }
-        if (lane == 0) { lohi[2 * line] = lo; lohi[2 * line + 1] = hi; chosen[line] = -1; }
-    }
-    __syncthreads();
-
-    // Phase 2: DP over contested lines (SeamBlend.cpp:362-407).
-    __shared__ float costs[kCostBatch][64];
-    __shared__ int slo[kCostBatch], shi[kCostBatch];
-    __shared__ int s_last;
-    __shared__ float s_dpfinal[64];
-    __shared__ int s_lastw;
-    int last = -1, lastLo = 0, lastW = 0;
-    float dpv = 0.0f;  // this lane's dp value on the last contested line
-    for (int b0 = 0; b0 < lines; b0 += kCostBatch) {
-        const int nb = min(kCostBatch, lines - b0);
-        for (int idx = threadIdx.x; idx < nb * 64; idx += blockDim.x) {
-            const int l = idx >> 6, k = idx & 63;
-            const int lo = lohi[2 * (b0 + l)], hi = lohi[2 * (b0 + l) + 1];
-            if (k == 0) { slo[l] = lo; shi[l] = hi; }
-            if (k <= hi - lo) {
-                const size_t p = seam_at(rowScan, ox0, oy0, w, b0 + l, lo + k);
-                float cost = 0.0f;
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    cost = cost + fabsf(static_cast<float>(imgI[p * 3 + c]) - static_cast<float>(imgJ[p * 3 + c]));
-                costs[l][k] = cost;
-            }
-        }
-        __syncthreads();
-        if (wave == 0) {
-            for (int l = 0; l < nb; ++l) {
-                const int lo = slo[l], hi = shi[l];
-                if (hi < lo) continue;
-                const int lw = hi - lo + 1;
-                const int line = b0 + l;
-                float v = 0.0f;
-                int bp = -1;
-                if (last < 0) {
-                    if (lane < lw) v = costs[l][lane];
-                } else {
-                    const int t = lo + lane;
-                    float best = 3.402823466e+38f;
-                    for (int pk = 0; pk < lastW; ++pk) {
-                        const float dprev = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dpv), pk));
-                        const float cand = dprev + 2.0f * fabsf(static_cast<float>(t - (lastLo + pk)));
-                        if (cand < best) { best = cand; bp = pk; }
-                    }
-                    if (lane < lw) v = costs[l][lane] + best;
-                }
-                if (lane < lw) back[static_cast<size_t>(line) * 64 + lane] = static_cast<uint8_t>(bp & 0xFF);
-                if (lane == 0) prevl[line] = last;
-                dpv = v;
-                last = line;
-                lastLo = lo;
-                lastW = lw;
-            }
-        }
-        __syncthreads();
-    }
-    if (wave == 0) {
-        if (lane == 0) { s_last = last; s_lastw = lastW; }
-        s_dpfinal[lane] = dpv;
-    }
-    __syncthreads();
-    last = s_last;
-    if (last < 0) return;  // no contested pixel in this overlap (uniform)
-
-    // Phase 3: backtrack through back-pointers staged in LDS, from the end.
-    __shared__ uint8_t sback[kBackBatch][64];
-    __shared__ int sprev[kBackBatch], slo2[kBackBatch];
-    __shared__ int s_line, s_k;
-    if (threadIdx.x == 0) {
-        float best = 3.402823466e+38f;
-        int bk = 0;
-        for (int k = 0; k < s_lastw; ++k)
-            if (s_dpfinal[k] < best) { best = s_dpfinal[k]; bk = k; }
-        s_line = last;
-        s_k = bk;
-    }
-    __syncthreads();
-    for (int b1 = last + 1; b1 > 0; b1 -= kBackBatch) {
-        const int b0 = max(0, b1 - kBackBatch);
-        for (int idx = threadIdx.x; idx < (b1 - b0) * 64; idx += blockDim.x)
-            sback[idx >> 6][idx & 63] = back[static_cast<size_t>(b0 + (idx >> 6)) * 64 + (idx & 63)];
-        for (int idx = threadIdx.x; idx < b1 - b0; idx += blockDim.x) {
-            sprev[idx] = prevl[b0 + idx];
-            slo2[idx] = lohi[2 * (b0 + idx)];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int line = s_line, k = s_k;
-            while (line >= b0) {
-                chosen[line] = slo2[line - b0] + k;
-                const int pk = static_cast<int>(sback[line - b0][k]);
-                line = sprev[line - b0];
-                k = pk;
-            }
-            s_line = line;
-            s_k = k;
-        }
-        __syncthreads();
-        if (s_line < 0) break;
-    }
-    __syncthreads();
-
-    // Phase 4: reassign I/J-owned pixels inside each contested band.
-    for (int line = wave; line < lines; line += nwaves) {
-        const int ch = chosen[line];
-        if (ch < 0) continue;
-        const int lo = lohi[2 * line], hi = lohi[2 * line + 1];
-        for (int t = lo + lane; t <= hi; t += 64) {
-            const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
-            const int o = owner[p];
-            if (o != ci && o != cj) continue;
-            owner[p] = ((t < ch) == iLow) ? ci : cj;
-        }
-    }
 }
 
 // ------------------------------------------------------------ pyramids --
@@ -649,6 +397,7 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     const size_t oCov = take(n * c), oDist = take(n * c * 4), oOwner = take(n * 4), oBox = take(4 * 4 * c);
     const size_t oLohi = take(8 * static_cast<size_t>(maxLines)), oPrev = take(4 * static_cast<size_t>(maxLines));
     const size_t oChosen = take(4 * static_cast<size_t>(maxLines)), oBack = take(64 * static_cast<size_t>(maxLines));
+    const size_t oWork = take(sizeof(ChamferWork) * I360HIP_MAX_CONTRIBUTORS);
     size_t oNd[I360HIP_MAX_BANDS][I360HIP_MAX_CONTRIBUTORS] = {}, oWt[I360HIP_MAX_BANDS][I360HIP_MAX_CONTRIBUTORS] = {};
     size_t oRec[I360HIP_MAX_BANDS] = {};
     for (int l = 1; l < L; ++l) {
@@ -673,27 +422,28 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     }
     // 2. chamfer, forward then backward (one block per contributor)
     {
-        const int S = w <= 4096 ? 4 : (w <= 8192 ? 8 : 16);
-        const int cols = (w + S - 1) / S;
-        const int nt = ((cols + 63) / 64) * 64;
-        if (S == 4) {
-            BL_LAUNCH(st, "chamfer_fwd", 5.0 * n * c, (chamfer_kernel<4, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
-            BL_LAUNCH(st, "chamfer_bwd", 8.0 * n * c, (chamfer_kernel<4, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
-        } else if (S == 8) {
-            BL_LAUNCH(st, "chamfer_fwd", 5.0 * n * c, (chamfer_kernel<8, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
-            BL_LAUNCH(st, "chamfer_bwd", 8.0 * n * c, (chamfer_kernel<8, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
-        } else {
-            BL_LAUNCH(st, "chamfer_fwd", 5.0 * n * c, (chamfer_kernel<16, true><<<c, nt, 0, s>>>(cov, dist, w, h)));
-            BL_LAUNCH(st, "chamfer_bwd", 8.0 * n * c, (chamfer_kernel<16, false><<<c, nt, 0, s>>>(cov, dist, w, h)));
+        ChamferWork hw[I360HIP_MAX_CONTRIBUTORS];
+        for (int i = 0; i < c; ++i) hw[i] = ChamferWork{cov + n * i, dist + n * i, boxes + 4 * i, w, h};
+        auto* dwork = reinterpret_cast<ChamferWork*>(base + oWork);
+        if (hipMemcpyAsync(dwork, hw, sizeof(ChamferWork) * c, hipMemcpyHostToDevice, s) != hipSuccess) {
+            st.set(I360HIP_DEVICE_ERROR, "chamfer work table upload failed");
+            return;
+        }
+        {
+            ProfScope ps(ctx, s, "chamfer", 13.0 * n * c);
+            if (launch_chamfer(dwork, c, w, s) != hipSuccess) {
+                st.set(I360HIP_KERNEL_LAUNCH_FAILED, "chamfer launch failed");
+                return;
+            }
         }
     }
     // 3. ownership
-    BL_LAUNCH(st, "ownership", 4.0 * n * (c + 1), ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, c, n, owner));
+    BL_LAUNCH(st, "ownership", 4.0 * n * (c + 1), ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, boxes, c, w, n, owner));
     // 4. pairwise seams in the reference's order
     for (int i = 0; i < c; ++i)
         for (int j = i + 1; j < c; ++j)
-            BL_LAUNCH(st, "seam_dp", 0, seam_dp_kernel<<<1, 1024, 0, s>>>(
-                              warped[i], warped[j], cov + n * i, cov + n * j, boxes, i, j, w, owner,
+            BL_LAUNCH(st, "seam_dp", 0, seam_dp_kernel<U16Img, float><<<1, 1024, 0, s>>>(
+                              U16Img{warped[i]}, U16Img{warped[j]}, cov + n * i, cov + n * j, boxes, i, j, w, owner,
                               reinterpret_cast<int*>(base + oLohi), reinterpret_cast<int*>(base + oPrev),
                               reinterpret_cast<int*>(base + oChosen), base + oBack));
     if (owner_out) BL_LAUNCH(st, "copy_owner", 8.0 * n, copy_owner_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner, owner_out, n));

@@ -135,6 +135,7 @@ void i360hip_destroy(i360hip_ctx* ctx) {
         std::lock_guard<std::mutex> lk(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        render_release(ctx);
         for (auto& b : ctx->bufs)
             if (b.ptr) (void)hipFree(b.ptr);
         ctx->bufs.clear();

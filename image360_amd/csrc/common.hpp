@@ -76,6 +76,7 @@ struct ProfRecord {
 }  // namespace i360
 
 struct i360hip_ctx {
+    void* render_cache = nullptr;  // cached render plan / graph (render_float.hip)
     bool profiling = false;
     std::vector<i360::ProfRecord> prof;
     std::vector<hipEvent_t> event_pool;
@@ -92,6 +93,8 @@ namespace i360 {
 // Context-owned device buffer `slot` of at least `bytes` bytes (grown on
 // demand, kept until the context is destroyed).
 void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st);
+// Drops the context's cached render plan and graph.
+void render_release(i360hip_ctx* ctx);
 }  // namespace i360
 
 namespace i360 {

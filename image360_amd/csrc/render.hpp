@@ -1,0 +1,61 @@
+// Float chunk render: internal descriptors shared by the planner and the
+// kernels (render_float.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace i360 {
+
+constexpr int kMaxLevels = I360HIP_MAX_BANDS;
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+
+// Storage/math precision of one render form.
+enum RenderMode : int {
+    kModeFP64 = I360HIP_RENDER_FP64,
+    kModePlain = I360HIP_RENDER_PLAIN,
+    kModePrecision = I360HIP_RENDER_PRECISION,
+    kModeBatched = I360HIP_RENDER_BATCHED,
+};
+
+// One (chunk, contributor) pair.
+struct RItem {
+    const void* src;  // frame pixels, interleaved RGB (FP16 or FP32)
+    int fw, fh, fmt;  // fmt: I360HIP_PIX_F16 / I360HIP_PIX_F32
+    float gain;
+    double hfwd[9];   // frame -> canvas
+    double hinv[9];   // canvas -> frame (filled on the device, FP64)
+    float hinv32[9];  // binary32 inverse of the binary32 forward matrix (plain form)
+    int chunk;        // chunk index
+    int local;        // index within the chunk's contributor list
+    void* warped;     // chunk-sized RGBA (coverage in A), storage type of the form
+    uint8_t* cov;     // chunk-sized coverage bits
+    float* dist;      // chunk-sized chamfer distances (inside the coverage box)
+    int* box;         // coverage box x0,y0,x1,y1 (chunk-local, inclusive)
+    int lbox[kMaxLevels][4];  // conservative support box per level (footprint dilated)
+    void* nd[kMaxLevels];     // level l >= 1: RGBA (numerator rgb, denominator)
+    void* wt[kMaxLevels];     // level l >= 1: weight
+};
+
+struct RChunk {
+    int x, y, w, h;
+    int first, count;  // items of this chunk
+    int levels;
+    int lw[kMaxLevels], lh[kMaxLevels];
+    int* boxes;          // count * 4 (the items' box pointers point here)
+    int32_t* owner;      // w*h
+    float* out;          // FP32 RGB, w*h*3 (caller's)
+    int32_t* owner_out;  // optional (caller's)
+    void* recon[kMaxLevels];  // level l >= 1 reconstruction, RGBA in the math type
+    int* lohi;
+    int* prevl;
+    int* chosen;
+    uint8_t* back;
+};
+
+// A tile of some stage: id = item or chunk index, (tx, ty) tile coordinates.
+struct RTile {
+    int id, tx, ty, pad;
+};
+
+}  // namespace i360

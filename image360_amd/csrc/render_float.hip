@@ -1,0 +1,1214 @@
+// GPU-resident float chunk render (i360hip_render_chunks): warp, coverage,
+// chamfer ownership, DP seams, multi-band compose, for one chunk after
+// another (FP64 / PLAIN / PRECISION) or for every chunk at once (BATCHED).
+//
+// Every stage is a table-driven kernel: the host planner lists the work
+// items ((chunk, contributor) pairs), the chunks and, per stage, the tiles
+// to process.  The per-chunk forms launch each stage on one chunk's slice
+// of the tables; the batched form launches each stage once over all tiles
+// of all chunks with a persistent grid and replays the launch sequence from
+// a HIP graph.
+//
+// Support boxes: each work item's pixels outside its footprint (the
+// frame's projected bounding box, widened by 2 px) are uncovered, so every
+// level of its pyramids is exactly zero outside the footprint dilated by
+// the blur support; kernels read such samples as 0 without touching memory
+// and skip the contributor there (a zero weight adds nothing to a band).
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "blend.hpp"
+#include "chamfer.hpp"
+#include "render.hpp"
+#include "seam.hpp"
+
+namespace i360 {
+
+namespace {
+
+constexpr int kSlotRender = 12;
+constexpr int kSlotRenderTables = 13;
+
+// ------------------------------------------------------------ storage ---
+template <int MODE> struct Form;
+template <> struct Form<kModeFP64> { using M = double; using S = double; };
+template <> struct Form<kModePlain> { using M = float; using S = float; };
+template <> struct Form<kModePrecision> { using M = float; using S = _Float16; };
+
+__device__ __forceinline__ void ld4(const void* p, size_t i, float (&o)[4], float*) {
+    const float4 q = static_cast<const float4*>(p)[i];
+    o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+}
+__device__ __forceinline__ void ld4(const void* p, size_t i, float (&o)[4], _Float16*) {
+    const h4 q = static_cast<const h4*>(p)[i];
+    o[0] = static_cast<float>(q.x); o[1] = static_cast<float>(q.y);
+    o[2] = static_cast<float>(q.z); o[3] = static_cast<float>(q.w);
+}
+__device__ __forceinline__ void ld4(const void* p, size_t i, double (&o)[4], double*) {
+    const double4 q = static_cast<const double4*>(p)[i];
+    o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+}
+__device__ __forceinline__ void st4(void* p, size_t i, const float (&v)[4], float*) {
+    static_cast<float4*>(p)[i] = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st4(void* p, size_t i, const float (&v)[4], _Float16*) {
+    h4 q;
+    q.x = static_cast<_Float16>(v[0]); q.y = static_cast<_Float16>(v[1]);
+    q.z = static_cast<_Float16>(v[2]); q.w = static_cast<_Float16>(v[3]);
+    static_cast<h4*>(p)[i] = q;
+}
+__device__ __forceinline__ void st4(void* p, size_t i, const double (&v)[4], double*) {
+    static_cast<double4*>(p)[i] = make_double4(v[0], v[1], v[2], v[3]);
+}
+template <typename M, typename S>
+__device__ __forceinline__ M ld1(const void* p, size_t i) {
+    return static_cast<M>(static_cast<const S*>(p)[i]);
+}
+template <typename S, typename M>
+__device__ __forceinline__ void st1(void* p, size_t i, M v) {
+    static_cast<S*>(p)[i] = static_cast<S>(v);
+}
+
+__device__ __forceinline__ bool in_box(const int* b, int x, int y) {
+    return x >= b[0] && x <= b[2] && y >= b[1] && y <= b[3];
+}
+
+// ----------------------------------------------------------- FP64 map ---
+__global__ void rmap_kernel(RItem* items, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* m = items[i].hfwd;
+    const double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], k = m[7], j = m[8];
+    const double c0 = e * j - f * k, c1 = d * j - f * g, c2 = d * k - e * g;
+    const double det = (a * c0 - b * c1) + c * c2;
+    const double s = 1.0 / det;
+    double* o = items[i].hinv;
+    o[0] = c0 * s; o[1] = (c * k - b * j) * s; o[2] = (b * f - c * e) * s;
+    o[3] = (f * g - d * j) * s; o[4] = (a * j - c * g) * s; o[5] = (c * d - a * f) * s;
+    o[6] = c2 * s; o[7] = (b * g - a * k) * s; o[8] = (a * e - b * d) * s;
+}
+
+__global__ void rbox_init_kernel(RItem* items, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int* b = items[i].box;
+    b[0] = INT_MAX; b[1] = INT_MAX; b[2] = -1; b[3] = -1;
+}
+
+// --------------------------------------------------------------- warp ---
+constexpr int kWarpTW = 64, kWarpTH = 16;
+
+template <typename M>
+__device__ __forceinline__ void fetch_rgb(const void* src, int fmt, size_t idx, M (&o)[3]) {
+    if (fmt == I360HIP_PIX_F16) {
+        const _Float16* p = static_cast<const _Float16*>(src) + idx * 3;
+        o[0] = static_cast<M>(static_cast<float>(p[0]));
+        o[1] = static_cast<M>(static_cast<float>(p[1]));
+        o[2] = static_cast<M>(static_cast<float>(p[2]));
+    } else {
+        const float* p = static_cast<const float*>(src) + idx * 3;
+        o[0] = static_cast<M>(p[0]); o[1] = static_cast<M>(p[1]); o[2] = static_cast<M>(p[2]);
+    }
+}
+
+// Per-tile FP32 local map reduced from the FP64 inverse: for the tile-local
+// offset d = (dx, dy), source = s0 + (A d) / (1 + g . d) exactly (the
+// projective map re-centred on the tile origin), s0 split into an integer
+// and a fraction so the binary32 part stays small.
+struct TileMap {
+    int ix, iy;
+    float fx, fy, a00, a01, a10, a11, g0, g1, q0;
+    int ok;
+};
+
+__device__ __forceinline__ TileMap tile_map(const double* h, double X0, double Y0) {
+    TileMap t{};
+    const double u0 = (h[0] * X0 + h[1] * Y0) + h[2];
+    const double v0 = (h[3] * X0 + h[4] * Y0) + h[5];
+    const double q0 = (h[6] * X0 + h[7] * Y0) + h[8];
+    if (fabs(q0) < 1e-6) return t;  // ok = 0: per-pixel FP64 map for this tile
+    const double sx = u0 / q0, sy = v0 / q0;
+    if (fabs(sx) > 1e9 || fabs(sy) > 1e9) return t;
+    const double fx = floor(sx), fy = floor(sy);
+    t.ix = static_cast<int>(fx);
+    t.iy = static_cast<int>(fy);
+    t.fx = static_cast<float>(sx - fx);
+    t.fy = static_cast<float>(sy - fy);
+    t.a00 = static_cast<float>((h[0] - sx * h[6]) / q0);
+    t.a01 = static_cast<float>((h[1] - sx * h[7]) / q0);
+    t.a10 = static_cast<float>((h[3] - sy * h[6]) / q0);
+    t.a11 = static_cast<float>((h[4] - sy * h[7]) / q0);
+    t.g0 = static_cast<float>(h[6] / q0);
+    t.g1 = static_cast<float>(h[7] / q0);
+    t.q0 = static_cast<float>(q0);
+    t.ok = 1;
+    return t;
+}
+
+// Integer sample origin and fractions for one pixel; false if invalid.
+template <int MODE, typename M>
+__device__ __forceinline__ bool map_pixel(const RItem& it, const RChunk& ch, const TileMap& tm, int x, int y,
+                                          int dx, int dy, int& sx0, int& sy0, M& fx, M& fy) {
+    if (MODE == kModePrecision && tm.ok) {
+        const float den = (1.0f + tm.g0 * static_cast<float>(dx)) + tm.g1 * static_cast<float>(dy);
+        if (fabsf(tm.q0 * den) < 1e-10f) return false;
+        const float ox = (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) / den;
+        const float oy = (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) / den;
+        const float lx = tm.fx + ox, ly = tm.fy + oy;
+        if (!(fabsf(lx) < 1e8f && fabsf(ly) < 1e8f)) return false;
+        const float flx = floorf(lx), fly = floorf(ly);
+        sx0 = tm.ix + static_cast<int>(flx);
+        sy0 = tm.iy + static_cast<int>(fly);
+        fx = static_cast<M>(lx - flx);
+        fy = static_cast<M>(ly - fly);
+        return sx0 >= 0 && sy0 >= 0 && sx0 <= it.fw - 2 && sy0 <= it.fh - 2;
+    }
+    if (MODE == kModePlain) {
+        const float* m = it.hinv32;
+        const float X = static_cast<float>(ch.x + x), Y = static_cast<float>(ch.y + y);
+        float u = (m[0] * X + m[1] * Y) + m[2];
+        float v = (m[3] * X + m[4] * Y) + m[5];
+        const float q = (m[6] * X + m[7] * Y) + m[8];
+        if (fabsf(q) < 1e-10f) return false;
+        u = u / q;
+        v = v / q;
+        const float fu = floorf(u), fv = floorf(v);
+        if (!(fu >= 0.0f && fv >= 0.0f && fu <= static_cast<float>(it.fw - 2) && fv <= static_cast<float>(it.fh - 2)))
+            return false;
+        sx0 = static_cast<int>(fu);
+        sy0 = static_cast<int>(fv);
+        fx = static_cast<M>(u - fu);
+        fy = static_cast<M>(v - fv);
+        return true;
+    }
+    // FP64 (also the precision form's fallback for degenerate tiles)
+    const double* m = it.hinv;
+    const double X = static_cast<double>(ch.x + x), Y = static_cast<double>(ch.y + y);
+    double u = (m[0] * X + m[1] * Y) + m[2];
+    double v = (m[3] * X + m[4] * Y) + m[5];
+    const double q = (m[6] * X + m[7] * Y) + m[8];
+    if (fabs(q) < 1e-10) return false;
+    u = u / q;
+    v = v / q;
+    const double fu = floor(u), fv = floor(v);
+    if (!(fu >= 0.0 && fv >= 0.0 && fu <= static_cast<double>(it.fw - 2) && fv <= static_cast<double>(it.fh - 2)))
+        return false;
+    sx0 = static_cast<int>(fu);
+    sy0 = static_cast<int>(fv);
+    fx = static_cast<M>(u - fu);
+    fy = static_cast<M>(v - fv);
+    return true;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ items,
+                                                    const RChunk* __restrict__ chunks,
+                                                    const RTile* __restrict__ tiles, int ntiles) {
+    using M = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::M;
+    using S = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::S;
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    __shared__ TileMap smap;
+    __shared__ int sbox[4];
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RItem& it = items[tl.id];
+        const RChunk& ch = chunks[it.chunk];
+        const int tx0 = tl.tx * kWarpTW, ty0 = tl.ty * kWarpTH;
+        if (threadIdx.x == 0) {
+            if (MM == kModePrecision)
+                smap = tile_map(it.hinv, static_cast<double>(ch.x + tx0), static_cast<double>(ch.y + ty0));
+            sbox[0] = INT_MAX; sbox[1] = INT_MAX; sbox[2] = -1; sbox[3] = -1;
+        }
+        __syncthreads();
+        const TileMap tm = smap;
+        const int* fb = it.lbox[0];
+        const bool useGain = fabsf(it.gain - 1.0f) >= 1e-6f;
+        int bx0 = INT_MAX, by0 = INT_MAX, bx1 = -1, by1 = -1;
+        const int lx = threadIdx.x & 63;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ly = (threadIdx.x >> 6) * 4 + r;
+            const int x = tx0 + lx, y = ty0 + ly;
+            if (x >= ch.w || y >= ch.h || !in_box(fb, x, y)) continue;
+            const size_t p = static_cast<size_t>(y) * ch.w + x;
+            int sx0, sy0;
+            M fx, fy;
+            M v[4] = {M(0), M(0), M(0), M(0)};
+            const bool ok = map_pixel<MM, M>(it, ch, tm, x, y, lx, ly, sx0, sy0, fx, fy);
+            if (ok) {
+                const M omx = M(1) - fx, omy = M(1) - fy;
+                const M w00 = omx * omy, w10 = fx * omy, w01 = omx * fy, w11 = fx * fy;
+                M a[3], b[3], c[3], d[3];
+                const size_t i0 = static_cast<size_t>(sy0) * it.fw + sx0;
+                fetch_rgb<M>(it.src, it.fmt, i0, a);
+                fetch_rgb<M>(it.src, it.fmt, i0 + 1, b);
+                fetch_rgb<M>(it.src, it.fmt, i0 + it.fw, c);
+                fetch_rgb<M>(it.src, it.fmt, i0 + it.fw + 1, d);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    M val = ((w00 * a[k] + w10 * b[k]) + w01 * c[k]) + w11 * d[k];
+                    if (useGain) val = val * static_cast<M>(it.gain);
+                    v[k] = val < M(0) ? M(0) : (M(1) < val ? M(1) : val);
+                }
+                v[3] = M(1);
+                bx0 = min(bx0, x); by0 = min(by0, y); bx1 = max(bx1, x); by1 = max(by1, y);
+            }
+            st4(it.warped, p, v, static_cast<S*>(nullptr));
+            it.cov[p] = ok ? 1 : 0;
+        }
+        if (bx1 >= 0) {
+            atomicMin(&sbox[0], bx0); atomicMin(&sbox[1], by0); atomicMax(&sbox[2], bx1); atomicMax(&sbox[3], by1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && sbox[2] >= 0) {
+            atomicMin(&it.box[0], sbox[0]); atomicMin(&it.box[1], sbox[1]);
+            atomicMax(&it.box[2], sbox[2]); atomicMax(&it.box[3], sbox[3]);
+        }
+        __syncthreads();
+    }
+}
+
+// Largest |FP32 tile map - FP64 map| in source pixels (diagnostic).
+__global__ void __launch_bounds__(256) rmap_error_kernel(const RItem* __restrict__ items,
+                                                         const RChunk* __restrict__ chunks,
+                                                         const RTile* __restrict__ tiles, int ntiles,
+                                                         unsigned int* __restrict__ maxbits) {
+    __shared__ TileMap smap;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RItem& it = items[tl.id];
+        const RChunk& ch = chunks[it.chunk];
+        const int tx0 = tl.tx * kWarpTW, ty0 = tl.ty * kWarpTH;
+        if (threadIdx.x == 0)
+            smap = tile_map(it.hinv, static_cast<double>(ch.x + tx0), static_cast<double>(ch.y + ty0));
+        __syncthreads();
+        const TileMap tm = smap;
+        float worst = 0.0f;
+        const int lx = threadIdx.x & 63;
+        for (int r = 0; r < 4; ++r) {
+            const int ly = (threadIdx.x >> 6) * 4 + r;
+            const int x = tx0 + lx, y = ty0 + ly;
+            if (x >= ch.w || y >= ch.h || !tm.ok) continue;
+            const double* m = it.hinv;
+            const double X = ch.x + x, Y = ch.y + y;
+            const double q = (m[6] * X + m[7] * Y) + m[8];
+            const double u = ((m[0] * X + m[1] * Y) + m[2]) / q, v = ((m[3] * X + m[4] * Y) + m[5]) / q;
+            if (!(u >= 0.0 && v >= 0.0 && u < it.fw - 1 && v < it.fh - 1)) continue;
+            const float den = (1.0f + tm.g0 * static_cast<float>(lx)) + tm.g1 * static_cast<float>(ly);
+            const float ox = (tm.a00 * static_cast<float>(lx) + tm.a01 * static_cast<float>(ly)) / den;
+            const float oy = (tm.a10 * static_cast<float>(lx) + tm.a11 * static_cast<float>(ly)) / den;
+            const double eu = (tm.ix + static_cast<double>(tm.fx + ox)) - u;
+            const double ev = (tm.iy + static_cast<double>(tm.fy + oy)) - v;
+            worst = fmaxf(worst, static_cast<float>(fmax(fabs(eu), fabs(ev))));
+        }
+        atomicMax(maxbits, __float_as_uint(worst));
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------- ownership ---
+__global__ void __launch_bounds__(256) rowner_kernel(const RItem* __restrict__ items,
+                                                     const RChunk* __restrict__ chunks,
+                                                     const RTile* __restrict__ tiles, int ntiles) {
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RChunk& ch = chunks[tl.id];
+        const int lx = threadIdx.x & 63;
+        for (int r = 0; r < 4; ++r) {
+            const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
+            if (x >= ch.w || y >= ch.h) continue;
+            const size_t p = static_cast<size_t>(y) * ch.w + x;
+            int best = -1;
+            float bd = 0.0f;
+            for (int k = 0; k < ch.count; ++k) {
+                const RItem& it = items[ch.first + k];
+                if (!in_box(it.box, x, y)) continue;
+                const float v = it.dist[p];
+                if (v > bd) { bd = v; best = k; }
+            }
+            ch.owner[p] = best;
+        }
+    }
+}
+
+// float-valued contributor image for the seam DP: RGBA storage -> M
+template <typename S, typename M>
+struct RgbaImg {
+    const void* p;
+    __device__ __forceinline__ void rgb(size_t i, M (&o)[3]) const {
+        M v[4];
+        ld4(p, i, v, static_cast<S*>(nullptr));
+        o[0] = v[0]; o[1] = v[1]; o[2] = v[2];
+    }
+    __device__ static constexpr M smooth() { return M(2) / M(65535); }
+};
+
+// ------------------------------------------------------------ reduce ---
+constexpr int kRedTW = 64, kRedTH = 8, kRedRows = 2 * kRedTH + 3;
+
+template <typename M>
+__device__ __forceinline__ M blur5m(M a, M b, M m, M d, M e) {
+    const M k0 = M(1) / M(16), k1 = M(4) / M(16), k2 = M(6) / M(16);
+    const M ta = k0 * a, tb = k1 * b, tm = k2 * m, td = k1 * d, te = k0 * e;
+    return (((ta + tb) + tm) + td) + te;
+}
+
+// One tile of level l+1 for one item: level-l samples outside the item's
+// support read as zero.
+template <int MODE, bool L0>
+__global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ items,
+                                                      const RChunk* __restrict__ chunks,
+                                                      const RTile* __restrict__ tiles, int ntiles, int l) {
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    using M = typename Form<MM>::M;
+    using S = typename Form<MM>::S;
+    __shared__ M tmp[5][kRedRows][kRedTW];
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RItem& it = items[tl.id];
+        const RChunk& ch = chunks[it.chunk];
+        const int w = ch.lw[l], h = ch.lh[l], ow = ch.lw[l + 1], oh = ch.lh[l + 1];
+        const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
+        const int* sb = L0 ? it.box : it.lbox[l];
+        for (int idx = threadIdx.x; idx < kRedRows * kRedTW; idx += 256) {
+            const int rr = idx / kRedTW, xx = idx % kRedTW;
+            const int ox = ox0 + xx;
+            if (ox >= ow) continue;
+            const int sx = min(2 * ox, w - 1);
+            const int ry = min(max(2 * oy0 - 2 + rr, 0), h - 1);
+            M v[5][5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const int cx = min(max(sx - 2 + k, 0), w - 1);
+#pragma unroll
+                for (int c = 0; c < 5; ++c) v[k][c] = M(0);
+                if (in_box(sb, cx, ry)) {
+                    const size_t p = static_cast<size_t>(ry) * w + cx;
+                    M q[4];
+                    if (L0) {
+                        ld4(it.warped, p, q, static_cast<S*>(nullptr));
+                        v[k][0] = q[0] * q[3];
+                        v[k][1] = q[1] * q[3];
+                        v[k][2] = q[2] * q[3];
+                        v[k][3] = q[3];
+                        v[k][4] = ch.owner[p] == it.local ? M(1) : M(0);
+                    } else {
+                        ld4(it.nd[l], p, q, static_cast<S*>(nullptr));
+                        v[k][0] = q[0]; v[k][1] = q[1]; v[k][2] = q[2]; v[k][3] = q[3];
+                        v[k][4] = ld1<M, S>(it.wt[l], p);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) tmp[c][rr][xx] = blur5m<M>(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c]);
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kRedTH * kRedTW; idx += 256) {
+            const int yy = idx / kRedTW, xx = idx % kRedTW;
+            const int ox = ox0 + xx, oy = oy0 + yy;
+            if (ox >= ow || oy >= oh) continue;
+            const int r = 2 * yy + 2;
+            M o[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                o[c] = blur5m<M>(tmp[c][r - 2][xx], tmp[c][r - 1][xx], tmp[c][r][xx], tmp[c][r + 1][xx], tmp[c][r + 2][xx]);
+            const size_t q = static_cast<size_t>(oy) * ow + ox;
+            const M nd[4] = {o[0], o[1], o[2], o[3]};
+            st4(it.nd[l + 1], q, nd, static_cast<S*>(nullptr));
+            st1<S, M>(it.wt[l + 1], q, o[4]);
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------- compose ---
+template <typename M>
+struct Coord {
+    int x0, x1, y0, y1;
+    M wx, wy;
+};
+
+template <typename M>
+__device__ __forceinline__ Coord<M> expand_at(int x, int y, int sw, int sh, int dw, int dh) {
+    const M sx = (sw > 1) ? static_cast<M>(sw - 1) / static_cast<M>(max(1, dw - 1)) : M(0);
+    const M sy = (sh > 1) ? static_cast<M>(sh - 1) / static_cast<M>(max(1, dh - 1)) : M(0);
+    Coord<M> c;
+    const M fy = sy * static_cast<M>(y);
+    c.y0 = min(static_cast<int>(fy), sh - 1);
+    c.y1 = min(c.y0 + 1, sh - 1);
+    c.wy = fy - static_cast<M>(c.y0);
+    const M fx = sx * static_cast<M>(x);
+    c.x0 = min(static_cast<int>(fx), sw - 1);
+    c.x1 = min(c.x0 + 1, sw - 1);
+    c.wx = fx - static_cast<M>(c.x0);
+    return c;
+}
+
+template <typename M>
+__device__ __forceinline__ M lerp2m(const Coord<M>& c, M v00, M v10, M v01, M v11) {
+    const M omx = M(1) - c.wx, omy = M(1) - c.wy;
+    const M top = omx * v00 + c.wx * v10;
+    const M bot = omx * v01 + c.wx * v11;
+    return omy * top + c.wy * bot;
+}
+
+// Hole-filled Gaussian at one level-l sample; zero outside the support box.
+template <typename M, typename S>
+__device__ __forceinline__ void gauss_m(const void* nd, const int* box, int w, int x, int y, M (&g)[3]) {
+    if (!in_box(box, x, y)) { g[0] = g[1] = g[2] = M(0); return; }
+    M q[4];
+    ld4(nd, static_cast<size_t>(y) * w + x, q, static_cast<S*>(nullptr));
+    const M inv = (q[3] > static_cast<M>(1e-6)) ? M(1) / q[3] : M(0);
+    g[0] = q[0] * inv; g[1] = q[1] * inv; g[2] = q[2] * inv;
+}
+
+template <typename M, typename S>
+__device__ __forceinline__ void expand_gauss_m(const void* nd, const int* box, int sw, const Coord<M>& c,
+                                               M (&o)[3]) {
+    M a[3], b[3], d[3], e[3];
+    gauss_m<M, S>(nd, box, sw, c.x0, c.y0, a);
+    gauss_m<M, S>(nd, box, sw, c.x1, c.y0, b);
+    gauss_m<M, S>(nd, box, sw, c.x0, c.y1, d);
+    gauss_m<M, S>(nd, box, sw, c.x1, c.y1, e);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = lerp2m<M>(c, a[k], b[k], d[k], e[k]);
+}
+
+template <typename M>
+__device__ __forceinline__ void expand_recon_m(const void* rc, int sw, const Coord<M>& c, M (&o)[3]) {
+    M a[4], b[4], d[4], e[4];
+    ld4(rc, static_cast<size_t>(c.y0) * sw + c.x0, a, static_cast<M*>(nullptr));
+    ld4(rc, static_cast<size_t>(c.y0) * sw + c.x1, b, static_cast<M*>(nullptr));
+    ld4(rc, static_cast<size_t>(c.y1) * sw + c.x0, d, static_cast<M*>(nullptr));
+    ld4(rc, static_cast<size_t>(c.y1) * sw + c.x1, e, static_cast<M*>(nullptr));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = lerp2m<M>(c, a[k], b[k], d[k], e[k]);
+}
+
+constexpr int kCompTW = 64, kCompTH = 4;
+
+// recon_l = sum_i w_i * (G_i,l - expand(G_i,l+1)) (+ expand(recon_{l+1}) below the top)
+template <int MODE>
+__global__ void __launch_bounds__(256) rcompose_kernel(const RItem* __restrict__ items,
+                                                       const RChunk* __restrict__ chunks,
+                                                       const RTile* __restrict__ tiles, int ntiles, int l) {
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    using M = typename Form<MM>::M;
+    using S = typename Form<MM>::S;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RChunk& ch = chunks[tl.id];
+        const int x = tl.tx * kCompTW + (threadIdx.x & 63), y = tl.ty * kCompTH + (threadIdx.x >> 6);
+        const int w = ch.lw[l], h = ch.lh[l];
+        if (x >= w || y >= h) continue;
+        const bool top = (l == ch.levels - 1);
+        const int nw = top ? 1 : ch.lw[l + 1], nh = top ? 1 : ch.lh[l + 1];
+        Coord<M> c{};
+        if (!top) c = expand_at<M>(x, y, nw, nh, w, h);
+        M band[3] = {M(0), M(0), M(0)};
+        const size_t p = static_cast<size_t>(y) * w + x;
+        for (int k = 0; k < ch.count; ++k) {
+            const RItem& it = items[ch.first + k];
+            if (!in_box(it.lbox[l], x, y)) continue;
+            const M wt = ld1<M, S>(it.wt[l], p);
+            if (wt == M(0)) continue;
+            M g[3];
+            gauss_m<M, S>(it.nd[l], it.lbox[l], w, x, y, g);
+            if (!top) {
+                M up[3];
+                expand_gauss_m<M, S>(it.nd[l + 1], it.lbox[l + 1], nw, c, up);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) g[q] = g[q] - up[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) band[q] = band[q] + wt * g[q];
+        }
+        if (!top) {
+            M ur[3];
+            expand_recon_m<M>(ch.recon[l + 1], nw, c, ur);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) band[q] = band[q] + ur[q];
+        }
+        const M o4[4] = {band[0], band[1], band[2], M(0)};
+        st4(ch.recon[l], p, o4, static_cast<M*>(nullptr));
+    }
+}
+
+// Level 0: only the owner carries weight (0/1 owner mask); FP32 RGB out.
+template <int MODE>
+__global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ items,
+                                                     const RChunk* __restrict__ chunks,
+                                                     const RTile* __restrict__ tiles, int ntiles) {
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    using M = typename Form<MM>::M;
+    using S = typename Form<MM>::S;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RChunk& ch = chunks[tl.id];
+        const int lx = threadIdx.x & 63;
+        for (int r = 0; r < 4; ++r) {
+            const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
+            if (x >= ch.w || y >= ch.h) continue;
+            const size_t p = static_cast<size_t>(y) * ch.w + x;
+            float* dst = ch.out + p * 3;
+            const int o = ch.owner[p];
+            if (o < 0) { dst[0] = 0.0f; dst[1] = 0.0f; dst[2] = 0.0f; continue; }
+            const RItem& it = items[ch.first + o];
+            M q[4];
+            ld4(it.warped, p, q, static_cast<S*>(nullptr));
+            const M inv = (q[3] > static_cast<M>(1e-6)) ? M(1) / q[3] : M(0);
+            M v[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = (q[k] * q[3]) * inv;
+            if (ch.levels > 1) {
+                const Coord<M> c = expand_at<M>(x, y, ch.lw[1], ch.lh[1], ch.w, ch.h);
+                M up[3], ur[3];
+                expand_gauss_m<M, S>(it.nd[1], it.lbox[1], ch.lw[1], c, up);
+                expand_recon_m<M>(ch.recon[1], ch.lw[1], c, ur);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v[k] = (v[k] - up[k]) + ur[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                M t = v[k] < M(1) ? v[k] : M(1);
+                t = M(0) < t ? t : M(0);
+                dst[k] = static_cast<float>(t);
+            }
+        }
+        if (ch.owner_out) {
+            for (int r = 0; r < 4; ++r) {
+                const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
+                if (x >= ch.w || y >= ch.h) continue;
+                const size_t p = static_cast<size_t>(y) * ch.w + x;
+                ch.owner_out[p] = ch.owner[p];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- plan ---
+struct Plan {
+    std::vector<RItem> items;
+    std::vector<RChunk> chunks;
+    std::vector<ChamferWork> chamfer;
+    // tiles per stage, grouped by chunk: [chunk] -> (offset, count)
+    std::vector<RTile> tiles;
+    struct Range { int off = 0, n = 0; };
+    std::vector<Range> warp, own, fin;
+    std::vector<std::vector<Range>> red;   // [chunk][l] tiles of level l+1
+    std::vector<std::vector<Range>> comp;  // [chunk][l] tiles of level l
+    std::vector<std::vector<std::pair<int, int>>> pairs;  // [chunk] item-local pairs
+    size_t bytes = 0;
+    int max_w = 0;
+    int mode = 0;
+    uint64_t key = 0;
+};
+
+inline size_t A256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+void dilate(const int (&in)[4], int ow, int oh, int (&out)[4]) {
+    // support of [2x-2, 2x+2] intersecting [in0, in1]
+    auto lo = [](int a) { return a - 2 >= 0 ? (a - 2 + 1) / 2 : 0; };
+    out[0] = std::max(0, lo(in[0]));
+    out[1] = std::max(0, lo(in[1]));
+    out[2] = std::min(ow - 1, (in[2] + 2) / 2);
+    out[3] = std::min(oh - 1, (in[3] + 2) / 2);
+}
+
+bool invert_f32(const double* hd, float* inv) {
+    float hm[9];
+    for (int i = 0; i < 9; ++i) hm[i] = static_cast<float>(hd[i]);
+    const float a = hm[0], b = hm[1], c = hm[2], d = hm[3], e = hm[4], f = hm[5], g = hm[6], k = hm[7], j = hm[8];
+    const float c0 = e * j - f * k, c1 = d * j - f * g, c2 = d * k - e * g;
+    const float det = (a * c0 - b * c1) + c * c2;
+    if (std::fabs(det) < 1e-12f) return false;
+    const float s = 1.0f / det;
+    inv[0] = c0 * s; inv[1] = (c * k - b * j) * s; inv[2] = (b * f - c * e) * s;
+    inv[3] = (f * g - d * j) * s; inv[4] = (a * j - c * g) * s; inv[5] = (c * d - a * f) * s;
+    inv[6] = c2 * s; inv[7] = (b * g - a * k) * s; inv[8] = (a * e - b * d) * s;
+    return true;
+}
+
+uint64_t mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    return h;
+}
+
+}  // namespace
+
+// Builds the plan (host tables + workspace layout); returns false on error.
+static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc, const int* contrib,
+                       int bands, int mode, Plan& P, Status& st) {
+    P = Plan{};
+    P.mode = mode;
+    const bool batched = mode == kModeBatched;
+    const size_t sS = (mode == kModeFP64) ? 8 : (mode == kModePlain ? 4 : 2);
+    const size_t sM = (mode == kModeFP64) ? 8 : 4;
+    uint64_t key = mix(0, static_cast<uint64_t>(mode));
+    key = mix(key, static_cast<uint64_t>(bands));
+    // layout offsets are relative; absolute pointers are patched after allocation
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
+    struct Pending { size_t warped, cov, dist, nd[kMaxLevels], wt[kMaxLevels]; };
+    struct PendingC { size_t boxes, owner, recon[kMaxLevels], lohi, prevl, chosen, back; };
+    std::vector<Pending> pit;
+    std::vector<PendingC> pch;
+    // the per-chunk forms reuse one chunk's worth of buffers
+    size_t slot_base = 0, slot_max = 0;
+    for (int ci = 0; ci < nc; ++ci) {
+        const i360hip_chunk& cc = chunks[ci];
+        if (cc.width <= 0 || cc.height <= 0 || !cc.out || cc.count < 0 || cc.first < 0) {
+            st.set(I360HIP_INVALID_PARAM, "chunk %d: invalid rectangle, output or contributor range", ci);
+            return false;
+        }
+        if (cc.width > 16384) {
+            st.set(I360HIP_INVALID_PARAM, "chunk %d wider than 16384 px", ci);
+            return false;
+        }
+        key = mix(key, static_cast<uint64_t>(cc.x) * 1000003ull + cc.y);
+        key = mix(key, static_cast<uint64_t>(cc.width) * 1000003ull + cc.height);
+        key = mix(key, reinterpret_cast<uintptr_t>(cc.out));
+        key = mix(key, reinterpret_cast<uintptr_t>(cc.owner));
+        if (!batched) off = slot_base;
+        RChunk ch{};
+        ch.x = cc.x; ch.y = cc.y; ch.w = cc.width; ch.h = cc.height;
+        ch.out = cc.out;
+        ch.owner_out = cc.owner;
+        ch.levels = ClampBands(cc.width, cc.height, bands);
+        ch.lw[0] = cc.width; ch.lh[0] = cc.height;
+        for (int l = 1; l < ch.levels; ++l) { ch.lw[l] = (ch.lw[l - 1] + 1) / 2; ch.lh[l] = (ch.lh[l - 1] + 1) / 2; }
+        ch.first = static_cast<int>(P.items.size());
+        const size_t n = static_cast<size_t>(cc.width) * cc.height;
+        std::vector<std::pair<int, int>> used;  // (frame, local)
+        const int cap = std::min(cc.count, I360HIP_MAX_CONTRIBUTORS);
+        for (int k = 0; k < cap; ++k) {
+            const int f = contrib[cc.first + k];
+            key = mix(key, static_cast<uint64_t>(f));
+            if (f < 0 || f >= nf) {
+                st.set(I360HIP_INVALID_PARAM, "chunk %d: contributor %d out of range", ci, f);
+                return false;
+            }
+            const i360hip_frame& fr = frames[f];
+            if (!fr.pixels || fr.width < 2 || fr.height < 2 ||
+                (fr.format != I360HIP_PIX_F16 && fr.format != I360HIP_PIX_F32)) {
+                st.set(I360HIP_INVALID_PARAM, "frame %d: invalid pixels/size/format", f);
+                return false;
+            }
+            RItem it{};
+            it.src = fr.pixels; it.fw = fr.width; it.fh = fr.height; it.fmt = fr.format; it.gain = fr.gain;
+            for (int q = 0; q < 9; ++q) { it.hfwd[q] = fr.h[q]; key = mix(key, *reinterpret_cast<const uint64_t*>(&fr.h[q])); }
+            key = mix(key, reinterpret_cast<uintptr_t>(fr.pixels));
+            key = mix(key, static_cast<uint64_t>(fr.width) * 65537ull + static_cast<uint64_t>(fr.height) * 3ull + fr.format);
+            {
+                uint32_t gb;
+                std::memcpy(&gb, &fr.gain, 4);
+                key = mix(key, gb);
+            }
+            // footprint: AABB of the projected frame corners, widened by 2 px
+            double mnx = 1e300, mny = 1e300, mxx = -1e300, mxy = -1e300;
+            bool finite = true;
+            for (int cnr = 0; cnr < 4; ++cnr) {
+                const double u = (cnr & 1) ? fr.width : 0.0, v = (cnr & 2) ? fr.height : 0.0;
+                const double q = fr.h[6] * u + fr.h[7] * v + fr.h[8];
+                if (std::fabs(q) < 1e-12) { finite = false; break; }
+                const double X = (fr.h[0] * u + fr.h[1] * v + fr.h[2]) / q;
+                const double Y = (fr.h[3] * u + fr.h[4] * v + fr.h[5]) / q;
+                mnx = std::min(mnx, X); mny = std::min(mny, Y); mxx = std::max(mxx, X); mxy = std::max(mxy, Y);
+            }
+            int fb[4];
+            if (finite) {
+                fb[0] = static_cast<int>(std::max(-1.0, std::floor(mnx) - 2 - cc.x));
+                fb[1] = static_cast<int>(std::max(-1.0, std::floor(mny) - 2 - cc.y));
+                fb[2] = static_cast<int>(std::min(1e9, std::ceil(mxx) + 2 - cc.x));
+                fb[3] = static_cast<int>(std::min(1e9, std::ceil(mxy) + 2 - cc.y));
+                fb[0] = std::max(fb[0], 0); fb[1] = std::max(fb[1], 0);
+                fb[2] = std::min(fb[2], cc.width - 1); fb[3] = std::min(fb[3], cc.height - 1);
+            } else {  // a corner at infinity: the whole chunk may be covered
+                fb[0] = 0; fb[1] = 0; fb[2] = cc.width - 1; fb[3] = cc.height - 1;
+            }
+            if (fb[0] > fb[2] || fb[1] > fb[3]) continue;  // never covers this chunk
+            if (mode == kModePlain && !invert_f32(fr.h, it.hinv32)) continue;  // RenderExecutor.cpp:102
+            std::memcpy(it.lbox[0], fb, sizeof(fb));
+            for (int l = 0; l + 1 < ch.levels; ++l) dilate(it.lbox[l], ch.lw[l + 1], ch.lh[l + 1], it.lbox[l + 1]);
+            it.chunk = ci;
+            it.local = static_cast<int>(used.size());
+            Pending pd{};
+            pd.warped = take(n * 4 * sS);
+            pd.cov = take(n);
+            pd.dist = take(n * 4);
+            for (int l = 1; l < ch.levels; ++l) {
+                const size_t m = static_cast<size_t>(ch.lw[l]) * ch.lh[l];
+                pd.nd[l] = take(m * 4 * sS);
+                pd.wt[l] = take(m * sS);
+            }
+            pit.push_back(pd);
+            P.items.push_back(it);
+            used.push_back({f, it.local});
+        }
+        ch.count = static_cast<int>(used.size());
+        PendingC pc{};
+        pc.boxes = take(16 * static_cast<size_t>(std::max(1, ch.count)));
+        pc.owner = take(n * 4);
+        for (int l = 1; l < ch.levels; ++l) pc.recon[l] = take(static_cast<size_t>(ch.lw[l]) * ch.lh[l] * 4 * sM);
+        const size_t lines = static_cast<size_t>(std::max(cc.width, cc.height));
+        pc.lohi = take(8 * lines); pc.prevl = take(4 * lines); pc.chosen = take(4 * lines); pc.back = take(64 * lines);
+        pch.push_back(pc);
+        P.chunks.push_back(ch);
+        std::vector<std::pair<int, int>> pr;
+        for (int i = 0; i < ch.count; ++i)
+            for (int j = i + 1; j < ch.count; ++j) pr.push_back({i, j});
+        P.pairs.push_back(pr);
+        P.max_w = std::max(P.max_w, cc.width);
+        if (!batched) slot_max = std::max(slot_max, off);
+    }
+    P.bytes = batched ? off : slot_max;
+    P.key = key;
+    // pointer patching happens in patch_plan once the workspace exists
+    P.chamfer.resize(P.items.size());
+    // remember the relative layout inside the items/chunks (stored as offsets)
+    for (size_t i = 0; i < P.items.size(); ++i) {
+        RItem& it = P.items[i];
+        it.warped = reinterpret_cast<void*>(pit[i].warped);
+        it.cov = reinterpret_cast<uint8_t*>(pit[i].cov);
+        it.dist = reinterpret_cast<float*>(pit[i].dist);
+        for (int l = 1; l < kMaxLevels; ++l) {
+            it.nd[l] = reinterpret_cast<void*>(pit[i].nd[l]);
+            it.wt[l] = reinterpret_cast<void*>(pit[i].wt[l]);
+        }
+    }
+    for (size_t c = 0; c < P.chunks.size(); ++c) {
+        RChunk& ch = P.chunks[c];
+        ch.boxes = reinterpret_cast<int*>(pch[c].boxes);
+        ch.owner = reinterpret_cast<int32_t*>(pch[c].owner);
+        for (int l = 1; l < kMaxLevels; ++l) ch.recon[l] = reinterpret_cast<void*>(pch[c].recon[l]);
+        ch.lohi = reinterpret_cast<int*>(pch[c].lohi);
+        ch.prevl = reinterpret_cast<int*>(pch[c].prevl);
+        ch.chosen = reinterpret_cast<int*>(pch[c].chosen);
+        ch.back = reinterpret_cast<uint8_t*>(pch[c].back);
+    }
+    // tiles, one stage after another so that each stage's tiles are
+    // contiguous across chunks (batched launches) and per chunk (the
+    // per-chunk forms); pyramid stages are level-major.
+    auto add = [&](int id, int tx0, int tx1, int ty0, int ty1) {
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) P.tiles.push_back(RTile{id, tx, ty, 0});
+    };
+    const int nch = static_cast<int>(P.chunks.size());
+    P.warp.assign(nch, {});
+    P.own.assign(nch, {});
+    for (int c = 0; c < nch; ++c) {
+        const RChunk& ch = P.chunks[c];
+        P.warp[c].off = static_cast<int>(P.tiles.size());
+        for (int k = 0; k < ch.count; ++k) {
+            const int* b = P.items[ch.first + k].lbox[0];
+            add(ch.first + k, b[0] / kWarpTW, b[2] / kWarpTW, b[1] / kWarpTH, b[3] / kWarpTH);
+        }
+        P.warp[c].n = static_cast<int>(P.tiles.size()) - P.warp[c].off;
+    }
+    for (int c = 0; c < nch; ++c) {
+        const RChunk& ch = P.chunks[c];
+        P.own[c].off = static_cast<int>(P.tiles.size());
+        add(c, 0, (ch.w - 1) / kWarpTW, 0, (ch.h - 1) / kWarpTH);
+        P.own[c].n = static_cast<int>(P.tiles.size()) - P.own[c].off;
+    }
+    P.fin = P.own;
+    int maxL = 1;
+    for (const auto& ch : P.chunks) maxL = std::max(maxL, ch.levels);
+    P.red.assign(nch, std::vector<Plan::Range>(kMaxLevels));
+    P.comp.assign(nch, std::vector<Plan::Range>(kMaxLevels));
+    for (int l = 0; l + 1 < maxL; ++l)
+        for (int c = 0; c < nch; ++c) {
+            const RChunk& ch = P.chunks[c];
+            P.red[c][l].off = static_cast<int>(P.tiles.size());
+            if (l + 1 < ch.levels)
+                for (int k = 0; k < ch.count; ++k) {
+                    const int* b = P.items[ch.first + k].lbox[l + 1];
+                    add(ch.first + k, b[0] / kRedTW, b[2] / kRedTW, b[1] / kRedTH, b[3] / kRedTH);
+                }
+            P.red[c][l].n = static_cast<int>(P.tiles.size()) - P.red[c][l].off;
+        }
+    for (int l = 1; l < maxL; ++l)
+        for (int c = 0; c < nch; ++c) {
+            const RChunk& ch = P.chunks[c];
+            P.comp[c][l].off = static_cast<int>(P.tiles.size());
+            if (l < ch.levels) add(c, 0, (ch.lw[l] - 1) / kCompTW, 0, (ch.lh[l] - 1) / kCompTH);
+            P.comp[c][l].n = static_cast<int>(P.tiles.size()) - P.comp[c][l].off;
+        }
+    return true;
+}
+
+static void patch_plan(Plan& P, uint8_t* base) {
+    auto fix = [&](auto*& p) {
+        using T = std::remove_reference_t<decltype(p)>;
+        p = reinterpret_cast<T>(base + reinterpret_cast<uintptr_t>(p));
+    };
+    for (auto& it : P.items) {
+        fix(it.warped); fix(it.cov); fix(it.dist);
+        for (int l = 1; l < kMaxLevels; ++l) { fix(it.nd[l]); fix(it.wt[l]); }
+    }
+    for (auto& ch : P.chunks) {
+        fix(ch.boxes); fix(ch.owner); fix(ch.lohi); fix(ch.prevl); fix(ch.chosen); fix(ch.back);
+        for (int l = 1; l < kMaxLevels; ++l) fix(ch.recon[l]);
+        for (int k = 0; k < ch.count; ++k) P.items[ch.first + k].box = ch.boxes + 4 * k;
+    }
+    for (size_t i = 0; i < P.items.size(); ++i) {
+        const RItem& it = P.items[i];
+        const RChunk& ch = P.chunks[it.chunk];
+        P.chamfer[i] = ChamferWork{it.cov, it.dist, it.box, ch.w, ch.h};
+    }
+}
+
+// Device copies of the tables.
+struct DevTables {
+    RItem* items = nullptr;
+    RChunk* chunks = nullptr;
+    RTile* tiles = nullptr;
+    ChamferWork* chamfer = nullptr;
+    SeamWork<RgbaImg<double, double>>* seam64 = nullptr;
+    SeamWork<RgbaImg<float, float>>* seam32 = nullptr;
+    SeamWork<RgbaImg<_Float16, float>>* seam16 = nullptr;
+    std::vector<std::vector<int>> round_chunks;  // seam round k -> chunk ids
+    std::vector<int> round_off;                  // offset of round k in the seam table
+};
+
+template <typename Img>
+static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables& T) {
+    size_t rounds = 0;
+    for (auto& pr : P.pairs) rounds = std::max(rounds, pr.size());
+    T.round_chunks.assign(rounds, {});
+    T.round_off.assign(rounds, 0);
+    // round-major for the batched form; the per-chunk forms index by (chunk, pair)
+    for (size_t k = 0; k < rounds; ++k) {
+        T.round_off[k] = static_cast<int>(out.size());
+        for (size_t c = 0; c < P.chunks.size(); ++c) {
+            if (k >= P.pairs[c].size()) continue;
+            const RChunk& ch = P.chunks[c];
+            const auto [i, j] = P.pairs[c][k];
+            const RItem& a = P.items[ch.first + i];
+            const RItem& b = P.items[ch.first + j];
+            SeamWork<Img> s{};
+            s.imgI = Img{a.warped};
+            s.imgJ = Img{b.warped};
+            s.covI = a.cov; s.covJ = b.cov; s.boxes = ch.boxes; s.ci = i; s.cj = j; s.w = ch.w;
+            s.owner = ch.owner; s.lohi = ch.lohi; s.prevl = ch.prevl; s.chosen = ch.chosen; s.back = ch.back;
+            out.push_back(s);
+            T.round_chunks[k].push_back(static_cast<int>(c));
+        }
+    }
+}
+
+#define RL(st, NAME, BYTES, ...)                                                                    \
+    do {                                                                                            \
+        ProfScope ps_(ctx, s, NAME, static_cast<double>(BYTES));                                    \
+        __VA_ARGS__;                                                                                \
+        hipError_t e_ = hipGetLastError();                                                          \
+        if (e_ != hipSuccess) {                                                                     \
+            (st).set(I360HIP_KERNEL_LAUNCH_FAILED, "%s launch: %s", NAME, HipName(e_));             \
+            return;                                                                                 \
+        }                                                                                           \
+    } while (0)
+
+static unsigned persistent_grid(i360hip_ctx* ctx, int n) {
+    const int cap = ctx->props.multiProcessorCount * 8;
+    return static_cast<unsigned>(std::max(1, std::min(n, cap)));
+}
+
+// Enqueues the stages of chunk `c` (or of every chunk when c < 0).
+template <int MODE>
+static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, hipStream_t s, Status& st) {
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    using M = typename Form<MM>::M;
+    using S = typename Form<MM>::S;
+    const size_t sS = sizeof(S), sM = sizeof(M);
+    const int c0 = c < 0 ? 0 : c, c1 = c < 0 ? static_cast<int>(P.chunks.size()) : c + 1;
+    auto span = [&](const std::vector<Plan::Range>& v, int& off, int& n) {
+        off = v[c0].off;
+        n = 0;
+        for (int k = c0; k < c1; ++k) n += v[k].n;
+    };
+    int off, n;
+    {
+        const int i0 = P.chunks[c0].first;
+        const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
+        if (i1 > i0) RL(st, "box_init", 0.0, rbox_init_kernel<<<(i1 - i0 + 63) / 64, 64, 0, s>>>(T.items + i0, i1 - i0));
+    }
+    // warp
+    span(P.warp, off, n);
+    double area0 = 0;
+    for (int k = c0; k < c1; ++k) area0 += static_cast<double>(P.chunks[k].w) * P.chunks[k].h;
+    RL(st, MODE == kModeFP64 ? "warp_fp64" : (MODE == kModePlain ? "warp_f32" : "warp_f16"),
+       static_cast<double>(n) * kWarpTW * kWarpTH * (6.0 + 4 * sS + 1),
+       rwarp_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+    // chamfer over the items of the chunk range
+    {
+        const int i0 = P.chunks[c0].first;
+        const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
+        ProfScope ps_(ctx, s, "chamfer", 0.0);
+        if (i1 > i0 && launch_chamfer(T.chamfer + i0, i1 - i0, P.max_w, s) != hipSuccess) {
+            st.set(I360HIP_KERNEL_LAUNCH_FAILED, "chamfer launch failed");
+            return;
+        }
+    }
+    span(P.own, off, n);
+    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+    // seams: the k-th pair of every chunk in range, in pair order
+    {
+        ProfScope ps_(ctx, s, "seam_dp", 0.0);
+        for (size_t k = 0; k < T.round_chunks.size(); ++k) {
+            const auto& rc = T.round_chunks[k];
+            int first = -1, cnt = 0;
+            for (size_t q = 0; q < rc.size(); ++q) {
+                if (rc[q] < c0 || rc[q] >= c1) continue;
+                if (first < 0) first = T.round_off[k] + static_cast<int>(q);
+                ++cnt;
+            }
+            if (cnt == 0) continue;
+            if (MM == kModeFP64)
+                seam_dp_table_kernel<RgbaImg<double, double>, double><<<cnt, 1024, 0, s>>>(T.seam64 + first);
+            else if (MM == kModePlain)
+                seam_dp_table_kernel<RgbaImg<float, float>, float><<<cnt, 1024, 0, s>>>(T.seam32 + first);
+            else
+                seam_dp_table_kernel<RgbaImg<_Float16, float>, float><<<cnt, 1024, 0, s>>>(T.seam16 + first);
+            if (hipGetLastError() != hipSuccess) {
+                st.set(I360HIP_KERNEL_LAUNCH_FAILED, "seam launch failed");
+                return;
+            }
+        }
+    }
+    // pyramids
+    int maxL = 1;
+    for (int k = c0; k < c1; ++k) maxL = std::max(maxL, P.chunks[k].levels);
+    for (int l = 0; l + 1 < maxL; ++l) {
+        off = P.red[c0][l].off;
+        n = 0;
+        for (int k = c0; k < c1; ++k) n += P.red[k][l].n;
+        if (n == 0) continue;
+        const double bytes = static_cast<double>(n) * kRedTW * kRedTH * (4.0 * (4 * sS + sS) + (4 * sS + sS));
+        if (l == 0)
+            RL(st, "reduce_l0", bytes, rreduce_kernel<MODE, true><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l));
+        else
+            RL(st, "reduce", bytes, rreduce_kernel<MODE, false><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l));
+    }
+    for (int l = maxL - 1; l >= 1; --l) {
+        off = P.comp[c0][l].off;
+        n = 0;
+        for (int k = c0; k < c1; ++k) n += P.comp[k][l].n;
+        if (n == 0) continue;
+        const double bytes = static_cast<double>(n) * kCompTW * kCompTH * (4 * sM + 2.0 * (4 * sS + sS));
+        RL(st, "compose_level", bytes,
+           rcompose_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l));
+    }
+    span(P.fin, off, n);
+    RL(st, "final_level0", area0 * (4.0 + 4 * sS + 12.0),
+       rfinal_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+}
+
+struct RenderCache {
+    uint64_t key = 0;
+    int mode = -1;
+    Plan plan;
+    DevTables tables;
+    hipGraphExec_t graph = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+static RenderCache& cache_of(i360hip_ctx* ctx) {
+    if (!ctx->render_cache) ctx->render_cache = new RenderCache();
+    return *static_cast<RenderCache*>(ctx->render_cache);
+}
+
+void render_release(i360hip_ctx* ctx) {
+    if (!ctx->render_cache) return;
+    auto* rc = static_cast<RenderCache*>(ctx->render_cache);
+    if (rc->graph) (void)hipGraphExecDestroy(rc->graph);
+    delete rc;
+    ctx->render_cache = nullptr;
+}
+
+// Uploads the tables of `P` into the table slot; fills T.
+static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s, Status& st, int slot) {
+    std::vector<SeamWork<RgbaImg<double, double>>> s64;
+    std::vector<SeamWork<RgbaImg<float, float>>> s32;
+    std::vector<SeamWork<RgbaImg<_Float16, float>>> s16;
+    T = DevTables{};
+    if (P.mode == kModeFP64) seam_table(P, s64, T);
+    else if (P.mode == kModePlain) seam_table(P, s32, T);
+    else seam_table(P, s16, T);
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
+    const size_t oI = take(sizeof(RItem) * std::max<size_t>(1, P.items.size()));
+    const size_t oC = take(sizeof(RChunk) * std::max<size_t>(1, P.chunks.size()));
+    const size_t oT = take(sizeof(RTile) * std::max<size_t>(1, P.tiles.size()));
+    const size_t oW = take(sizeof(ChamferWork) * std::max<size_t>(1, P.chamfer.size()));
+    const size_t oS = take(std::max({sizeof(s64[0]) * s64.size(), sizeof(s32[0]) * s32.size(),
+                                     sizeof(s16[0]) * s16.size(), size_t(256)}));
+    std::vector<uint8_t> host(off);
+    std::memcpy(host.data() + oI, P.items.data(), sizeof(RItem) * P.items.size());
+    std::memcpy(host.data() + oC, P.chunks.data(), sizeof(RChunk) * P.chunks.size());
+    std::memcpy(host.data() + oT, P.tiles.data(), sizeof(RTile) * P.tiles.size());
+    std::memcpy(host.data() + oW, P.chamfer.data(), sizeof(ChamferWork) * P.chamfer.size());
+    if (!s64.empty()) std::memcpy(host.data() + oS, s64.data(), sizeof(s64[0]) * s64.size());
+    if (!s32.empty()) std::memcpy(host.data() + oS, s32.data(), sizeof(s32[0]) * s32.size());
+    if (!s16.empty()) std::memcpy(host.data() + oS, s16.data(), sizeof(s16[0]) * s16.size());
+    auto* dev = static_cast<uint8_t*>(ctx_scratch(ctx, slot, off, st));
+    if (!st.ok()) return;
+    if (hipMemcpyAsync(dev, host.data(), off, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        st.set(I360HIP_DEVICE_ERROR, "render table upload failed");
+        return;
+    }
+    T.items = reinterpret_cast<RItem*>(dev + oI);
+    T.chunks = reinterpret_cast<RChunk*>(dev + oC);
+    T.tiles = reinterpret_cast<RTile*>(dev + oT);
+    T.chamfer = reinterpret_cast<ChamferWork*>(dev + oW);
+    T.seam64 = reinterpret_cast<decltype(T.seam64)>(dev + oS);
+    T.seam32 = reinterpret_cast<decltype(T.seam32)>(dev + oS);
+    T.seam16 = reinterpret_cast<decltype(T.seam16)>(dev + oS);
+}
+
+static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc,
+                       const int* contrib, int bands, int mode, hipStream_t s, Status& st) {
+    RenderCache& rc = cache_of(ctx);
+    Plan P;
+    if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st)) return;
+    const bool reuse = rc.graph && rc.key == P.key && rc.mode == mode && mode == kModeBatched;
+    if (reuse) {
+        if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
+        return;
+    }
+    if (rc.graph) {
+        (void)hipGraphExecDestroy(rc.graph);
+        rc.graph = nullptr;
+    }
+    auto* base = static_cast<uint8_t*>(ctx_scratch(ctx, kSlotRender, std::max<size_t>(P.bytes, 256), st));
+    if (!st.ok()) return;
+    patch_plan(P, base);
+    upload_tables(ctx, P, rc.tables, s, st, kSlotRenderTables);
+    if (!st.ok()) return;
+    rc.plan = std::move(P);
+    rc.key = rc.plan.key;
+    rc.mode = mode;
+    const Plan& Q = rc.plan;
+    const DevTables& T = rc.tables;
+    const int ni = static_cast<int>(Q.items.size());
+    auto body = [&](hipStream_t ss) {
+        if (ni > 0) rmap_kernel<<<(ni + 63) / 64, 64, 0, ss>>>(T.items, ni);
+        switch (mode) {
+            case kModeFP64:
+                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModeFP64>(ctx, Q, T, c, ss, st);
+                break;
+            case kModePlain:
+                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModePlain>(ctx, Q, T, c, ss, st);
+                break;
+            case kModePrecision:
+                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModePrecision>(ctx, Q, T, c, ss, st);
+                break;
+            default:
+                if (!Q.chunks.empty()) enqueue<kModeBatched>(ctx, Q, T, -1, ss, st);
+                break;
+        }
+    };
+    if (mode != kModeBatched || ctx->profiling) {
+        body(s);
+        return;
+    }
+    // batched: capture once, replay on later identical calls
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        body(s);
+        return;
+    }
+    body(s);
+    if (hipStreamEndCapture(s, &g) != hipSuccess || !st.ok()) {
+        if (g) (void)hipGraphDestroy(g);
+        if (st.ok()) st.set(I360HIP_DEVICE_ERROR, "graph capture failed");
+        return;
+    }
+    hipGraphExec_t ge = nullptr;
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        st.set(I360HIP_DEVICE_ERROR, "graph instantiate failed");
+        return;
+    }
+    (void)hipGraphDestroy(g);
+    rc.graph = ge;
+    if (hipGraphLaunch(ge, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
+}
+
+}  // namespace i360
+
+using namespace i360;
+
+extern "C" {
+
+int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                          const i360hip_chunk* chunks, int num_chunks, const int* contributors, int num_bands,
+                          int mode, void* stream) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if ((!frames && num_frames > 0) || (!chunks && num_chunks > 0) || (!contributors && num_chunks > 0)) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument.");
+        return I360HIP_INVALID_PARAM;
+    }
+    if (num_bands < 1 || num_bands > I360HIP_MAX_BANDS || mode < I360HIP_RENDER_FP64 || mode > I360HIP_RENDER_BATCHED) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "numBands must be 1-12 and mode a known form.");
+        return I360HIP_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Status st;
+    if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    run_render(ctx, frames, num_frames, chunks, num_chunks, contributors, num_bands, mode, s, st);
+    if (!st.ok()) std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+    return st.code;
+}
+
+int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                             const i360hip_chunk* chunks, int num_chunks, const int* contributors,
+                             double* max_error_px) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!max_error_px || (!frames && num_frames > 0) || (!chunks && num_chunks > 0)) return I360HIP_INVALID_PARAM;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Status st;
+    if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
+    Plan P;
+    if (!build_plan(frames, num_frames, chunks, num_chunks, contributors, 1, kModePrecision, P, st)) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        return st.code;
+    }
+    // only the item table and warp tiles are used; no workspace needed
+    for (auto& it : P.items) { it.warped = nullptr; it.cov = nullptr; it.dist = nullptr; }
+    DevTables T;
+    upload_tables(ctx, P, T, ctx->stream, st, kSlotRender + 3);
+    auto* bits = static_cast<unsigned int*>(ctx_scratch(ctx, kSlotRender + 2, 256, st));
+    if (!st.ok()) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        return st.code;
+    }
+    (void)hipMemsetAsync(bits, 0, sizeof(unsigned int), ctx->stream);
+    const int ni = static_cast<int>(P.items.size());
+    if (ni > 0) rmap_kernel<<<(ni + 63) / 64, 64, 0, ctx->stream>>>(T.items, ni);
+    const int n = static_cast<int>(P.tiles.size());
+    // warp tiles come first per chunk; all warp ranges together
+    int total = 0;
+    for (auto& r : P.warp) total += r.n;
+    (void)total;
+    for (auto& r : P.warp)
+        if (r.n > 0)
+            rmap_error_kernel<<<persistent_grid(ctx, r.n), 256, 0, ctx->stream>>>(T.items, T.chunks, T.tiles + r.off, r.n, bits);
+    (void)n;
+    unsigned int hb = 0;
+    if (hipMemcpyAsync(&hb, bits, sizeof(hb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return I360HIP_DEVICE_ERROR;
+    float f;
+    std::memcpy(&f, &hb, sizeof(f));
+    *max_error_px = static_cast<double>(f);
+    return I360HIP_SUCCESS;
+}
+
+}  // extern "C"

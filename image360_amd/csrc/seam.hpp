@@ -1,0 +1,261 @@
+// Pairwise DP seam refinement (WindowsApp.Core/SourceFiles/
+// SeamBlend.cpp:293-445), shared by the RGB48 and float renders.
+//
+// One block handles one contributor pair of one chunk: (1) the crossover
+// of the current owner map and the contested band of every line, one
+// wavefront per line with ballots; (2) the line-to-line DP on one
+// wavefront, lane k = candidate position lo+k, the previous line's values
+// broadcast lane by lane (readlane), candidates scanned in increasing order
+// with a strict < so the first minimum wins as in the reference; (3) the
+// backtrack through back-pointers staged in LDS from the last line down;
+// (4) the reassignment of I/J-owned pixels inside each contested band.
+// Img supplies rgb(p, M[3]) and the smoothness weight per pixel of seam
+// movement (2 in RGB48 units, SeamBlend.cpp:309; 2/65535 for [0,1] images).
+#pragma once
+
+#include "common.hpp"
+
+namespace i360 {
+
+template <typename M> struct SeamMax;
+template <> struct SeamMax<float> { static constexpr float value = 3.402823466e+38f; };
+template <> struct SeamMax<double> { static constexpr double value = 1.7976931348623157e+308; };
+
+__device__ __forceinline__ float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// RGB48 contributor (reference storage).
+struct U16Img {
+    const uint16_t* p;
+    __device__ __forceinline__ void rgb(size_t i, float (&o)[3]) const {
+        o[0] = static_cast<float>(p[i * 3]);
+        o[1] = static_cast<float>(p[i * 3 + 1]);
+        o[2] = static_cast<float>(p[i * 3 + 2]);
+    }
+    __device__ static constexpr float smooth() { return 2.0f; }
+};
+
+constexpr int kSeamRadius = 24;  // SeamBlend.cpp:319
+constexpr int kCostBatch = 64;
+constexpr int kBackBatch = 256;
+
+__device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w, int line, int t) {
+    return rowScan ? static_cast<size_t>(oy0 + line) * w + (ox0 + t) : static_cast<size_t>(oy0 + t) * w + (ox0 + line);
+}
+
+template <class Img, typename M>
+__device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
+                                                       const uint8_t* __restrict__ covI,
+                                                       const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
+                                                       int ci, int cj, int w, int32_t* __restrict__ owner,
+                                                       int* __restrict__ lohi, int* __restrict__ prevl,
+                                                       int* __restrict__ chosen, uint8_t* __restrict__ back) {
+    const int* bI = boxes + 4 * ci;
+    const int* bJ = boxes + 4 * cj;
+    if (bI[2] < 0 || bJ[2] < 0) return;  // an empty coverage box
+    const int ox0 = max(bI[0], bJ[0]), oy0 = max(bI[1], bJ[1]);
+    const int ox1 = min(bI[2], bJ[2]), oy1 = min(bI[3], bJ[3]);
+    if (ox0 > ox1 || oy0 > oy1) return;
+    const int bw = ox1 - ox0 + 1, bh = oy1 - oy0 + 1;
+    const bool rowScan = bw >= bh;
+    const float cI = rowScan ? static_cast<float>(bI[0] + bI[2]) * 0.5f : static_cast<float>(bI[1] + bI[3]) * 0.5f;
+    const float cJ = rowScan ? static_cast<float>(bJ[0] + bJ[2]) * 0.5f : static_cast<float>(bJ[1] + bJ[3]) * 0.5f;
+    const bool iLow = cI <= cJ;
+    const int lines = rowScan ? bh : bw, len = rowScan ? bw : bh;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+
+    // Phase 1: crossover of the current owner map and contested band per line.
+    for (int line = wave; line < lines; line += nwaves) {
+        int first = -1, cross = -1;
+        for (int base = 0; base < len && cross < 0; base += 64) {
+            const int t = base + lane;
+            const int o = (t < len) ? owner[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
+            const bool ij = (t < len) && (o == ci || o == cj);
+            const unsigned long long m = __ballot(ij);
+            int start = 0;
+            if (first < 0) {
+                if (m == 0ull) continue;
+                const int f = __ffsll(static_cast<long long>(m)) - 1;
+                first = __shfl(o, f);
+                start = f + 1;
+            }
+            unsigned long long m2 = __ballot(ij && o != first);
+            m2 = (start >= 64) ? 0ull : (m2 & (~0ull << start));
+            if (m2) cross = base + __ffsll(static_cast<long long>(m2)) - 1;
+        }
+        int lo = 1, hi = 0;
+        if (cross >= 0) {
+            const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
+            const int t = a + lane;
+            bool both = false;
+            if (t <= b) {
+                const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
+                both = covI[p] && covJ[p];
+            }
+            const unsigned long long m = __ballot(both);
+            if (m) {
+                lo = a + __ffsll(static_cast<long long>(m)) - 1;
+                hi = a + 63 - __clzll(static_cast<long long>(m));
+            }
+        }
+        if (lane == 0) { lohi[2 * line] = lo; lohi[2 * line + 1] = hi; chosen[line] = -1; }
+    }
+    __syncthreads();
+
+    // Phase 2: DP over contested lines (SeamBlend.cpp:362-407).
+    __shared__ M costs[kCostBatch][64];
+    __shared__ int slo[kCostBatch], shi[kCostBatch];
+    __shared__ int s_last;
+    __shared__ M s_dpfinal[64];
+    __shared__ int s_lastw;
+    int last = -1, lastLo = 0, lastW = 0;
+    M dpv = M(0);  // this lane's dp value on the last contested line
+    for (int b0 = 0; b0 < lines; b0 += kCostBatch) {
+        const int nb = min(kCostBatch, lines - b0);
+        for (int idx = threadIdx.x; idx < nb * 64; idx += blockDim.x) {
+            const int l = idx >> 6, k = idx & 63;
+            const int lo = lohi[2 * (b0 + l)], hi = lohi[2 * (b0 + l) + 1];
+            if (k == 0) { slo[l] = lo; shi[l] = hi; }
+            if (k <= hi - lo) {
+                const size_t p = seam_at(rowScan, ox0, oy0, w, b0 + l, lo + k);
+                M a[3], b[3];
+                imgI.rgb(p, a);
+                imgJ.rgb(p, b);
+                M cost = M(0);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) cost = cost + fabs(a[c] - b[c]);
+                costs[l][k] = cost;
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (int l = 0; l < nb; ++l) {
+                const int lo = slo[l], hi = shi[l];
+                if (hi < lo) continue;
+                const int lw = hi - lo + 1;
+                const int line = b0 + l;
+                M v = M(0);
+                int bp = -1;
+                if (last < 0) {
+                    if (lane < lw) v = costs[l][lane];
+                } else {
+                    const int t = lo + lane;
+                    M best = SeamMax<M>::value;
+                    for (int pk = 0; pk < lastW; ++pk) {
+                        const M dprev = lane_value(dpv, pk);
+                        const M cand = dprev + Img::smooth() * fabs(static_cast<M>(t - (lastLo + pk)));
+                        if (cand < best) { best = cand; bp = pk; }
+                    }
+                    if (lane < lw) v = costs[l][lane] + best;
+                }
+                if (lane < lw) back[static_cast<size_t>(line) * 64 + lane] = static_cast<uint8_t>(bp & 0xFF);
+                if (lane == 0) prevl[line] = last;
+                dpv = v;
+                last = line;
+                lastLo = lo;
+                lastW = lw;
+            }
+        }
+        __syncthreads();
+    }
+    if (wave == 0) {
+        if (lane == 0) { s_last = last; s_lastw = lastW; }
+        s_dpfinal[lane] = dpv;
+    }
+    __syncthreads();
+    last = s_last;
+    if (last < 0) return;  // no contested pixel in this overlap (uniform)
+
+    // Phase 3: backtrack through back-pointers staged in LDS, from the end.
+    __shared__ uint8_t sback[kBackBatch][64];
+    __shared__ int sprev[kBackBatch], slo2[kBackBatch];
+    __shared__ int s_line, s_k;
+    if (threadIdx.x == 0) {
+        M best = SeamMax<M>::value;
+        int bk = 0;
+        for (int k = 0; k < s_lastw; ++k)
+            if (s_dpfinal[k] < best) { best = s_dpfinal[k]; bk = k; }
+        s_line = last;
+        s_k = bk;
+    }
+    __syncthreads();
+    for (int b1 = last + 1; b1 > 0; b1 -= kBackBatch) {
+        const int b0 = max(0, b1 - kBackBatch);
+        for (int idx = threadIdx.x; idx < (b1 - b0) * 64; idx += blockDim.x)
+            sback[idx >> 6][idx & 63] = back[static_cast<size_t>(b0 + (idx >> 6)) * 64 + (idx & 63)];
+        for (int idx = threadIdx.x; idx < b1 - b0; idx += blockDim.x) {
+            sprev[idx] = prevl[b0 + idx];
+            slo2[idx] = lohi[2 * (b0 + idx)];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int line = s_line, k = s_k;
+            while (line >= b0) {
+                chosen[line] = slo2[line - b0] + k;
+                const int pk = static_cast<int>(sback[line - b0][k]);
+                line = sprev[line - b0];
+                k = pk;
+            }
+            s_line = line;
+            s_k = k;
+        }
+        __syncthreads();
+        if (s_line < 0) break;
+    }
+    __syncthreads();
+
+    // Phase 4: reassign I/J-owned pixels inside each contested band.
+    for (int line = wave; line < lines; line += nwaves) {
+        const int ch = chosen[line];
+        if (ch < 0) continue;
+        const int lo = lohi[2 * line], hi = lohi[2 * line + 1];
+        for (int t = lo + lane; t <= hi; t += 64) {
+            const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
+            const int o = owner[p];
+            if (o != ci && o != cj) continue;
+            owner[p] = ((t < ch) == iLow) ? ci : cj;
+        }
+    }
+}
+
+
+template <class Img, typename M>
+__global__ void __launch_bounds__(1024) seam_dp_kernel(Img imgI, Img imgJ, const uint8_t* __restrict__ covI,
+                                                       const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
+                                                       int ci, int cj, int w, int32_t* __restrict__ owner,
+                                                       int* __restrict__ lohi, int* __restrict__ prevl,
+                                                       int* __restrict__ chosen, uint8_t* __restrict__ back) {
+    seam_dp_body<Img, M>(imgI, imgJ, covI, covJ, boxes, ci, cj, w, owner, lohi, prevl, chosen, back);
+}
+
+// One pair of one chunk per table entry (batched form: every chunk's k-th
+// pair in one launch).
+template <class Img>
+struct SeamWork {
+    Img imgI, imgJ;
+    const uint8_t* covI;
+    const uint8_t* covJ;
+    const int* boxes;  // the chunk's contributor boxes
+    int ci, cj, w;
+    int32_t* owner;
+    int* lohi;
+    int* prevl;
+    int* chosen;
+    uint8_t* back;
+};
+
+template <class Img, typename M>
+__global__ void __launch_bounds__(1024) seam_dp_table_kernel(const SeamWork<Img>* __restrict__ work) {
+    const SeamWork<Img> wk = work[blockIdx.x];
+    seam_dp_body<Img, M>(wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi,
+                         wk.prevl, wk.chosen, wk.back);
+}
+
+}  // namespace i360

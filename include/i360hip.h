@@ -137,6 +137,71 @@ int i360hip_demosaic_bayer_u16(i360hip_ctx* ctx, const uint16_t* cfa, int width,
 int i360hip_blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int num_contributors, int width,
                             int height, int num_bands, uint16_t* out, int32_t* owner_out);
 
+/* ---- GPU-resident float chunk render ---------------------------------------
+ * RenderExecutor::Execute's compute (RenderExecutor.cpp:59-140: warp every
+ * contributor, apply its gain, BlendChunkContributors) for many chunks at
+ * once, on frames resident in device memory, in float RGB in [0,1]:
+ *   - a sample is valid where WarpPerspective_Scalar.cpp:117-123 says so;
+ *     coverage is carried explicitly (no "(0,0,0) means no data");
+ *   - ownership = chamfer distance of the coverage + argmax + pairwise DP
+ *     seams (SeamBlend.cpp:206-445), colour costs in [0,1] units and the
+ *     smoothness weight 2/65535 per pixel;
+ *   - multi-band compose (SeamBlend.cpp:447-546), output clamped to [0,1],
+ *     0 where no contributor covers.
+ * Forms (`mode`):
+ *   FP64      binary64 map, storage and arithmetic (ground truth);
+ *   PLAIN     binary32 inverse of the binary32 forward matrix and binary32
+ *             canvas coordinates (the reference's arithmetic), FP32
+ *             storage, one chunk after another;
+ *   PRECISION FP64 inverse on the GPU, reduced per 64x16 tile to an FP32
+ *             local offset + Jacobian + perspective row (exact projective
+ *             form in tile-local coordinates), FP16 storage of warped
+ *             frames and pyramid levels, FP32 accumulation, one chunk after
+ *             another;
+ *   BATCHED   PRECISION arithmetic, every chunk of the call in one
+ *             persistent launch per stage, replayed from a HIP graph.
+ * Frames: interleaved RGB (3 values per pixel), FP16 or FP32, device
+ * memory.  Chunks name their contributors by index into `frames` through
+ * contributors[first .. first+count-1] (at most 32 are used, in order).
+ * No counterpart in IComputeBackend (the reference blends on the host). */
+enum {
+    I360HIP_PIX_F16 = 0,
+    I360HIP_PIX_F32 = 1
+};
+enum {
+    I360HIP_RENDER_FP64 = 0,
+    I360HIP_RENDER_PLAIN = 1,
+    I360HIP_RENDER_PRECISION = 2,
+    I360HIP_RENDER_BATCHED = 3
+};
+
+typedef struct i360hip_frame {
+    const void* pixels; /* device, width*height*3 values */
+    int width;
+    int height;
+    int format;         /* I360HIP_PIX_F16 or I360HIP_PIX_F32 */
+    float gain;         /* applied when |gain-1| > 1e-6 (RenderExecutor.cpp:111) */
+    double h[9];        /* forward homography frame -> canvas, row-major */
+} i360hip_frame;
+
+typedef struct i360hip_chunk {
+    int x, y, width, height; /* canvas rectangle */
+    int first, count;        /* contributor indices: contributors[first .. first+count-1] */
+    float* out;              /* device FP32 RGB, width*height*3 */
+    int32_t* owner;          /* optional device owner map (index into the chunk's list), or NULL */
+} i360hip_chunk;
+
+int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                          const i360hip_chunk* chunks, int num_chunks, const int* contributors,
+                          int num_bands, int mode, void* stream);
+
+/* Largest |source coordinate| difference (pixels) between the PRECISION
+ * form's per-tile FP32 map and the FP64 map over every output pixel of the
+ * given chunks and contributors; synchronous. */
+int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                             const i360hip_chunk* chunks, int num_chunks, const int* contributors,
+                             double* max_error_px);
+
 /* ---- device-pointer forms (asynchronous on `stream`) ------------------- */
 
 /* `inputs` is a HOST array of num_inputs DEVICE pointers; `num_scalars`

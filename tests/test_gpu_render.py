@@ -1,0 +1,145 @@
+"""GPU float chunk render (i360hip_render_chunks) against the oracle:
+FP64 form vs the FP64 oracle, plain FP32 form vs the binary32 oracle (same
+owner maps, outputs within 2e-6), precision form within 1e-3 max abs and
+>= 50 dB PSNR of the FP64 oracle, batched form identical to the precision
+form, graph replays identical to the first run."""
+import numpy as np
+import pytest
+
+from image360_amd import _abi
+from oracle import pyoracle as po
+from tests.gpu_util import note
+from tests.render_cases import psnr, small_case
+
+pytestmark = pytest.mark.gpu
+
+BANDS = 5
+
+
+def _upload(frames):
+    import torch
+
+    return [torch.from_numpy(f.copy()).cuda() for f in frames]
+
+
+def _gpu_render(hip, g, dev_frames, mode, bands=BANDS, want_owner=True):
+    import torch
+
+    frames = [{"ptr": t.data_ptr(), "w": int(g.frame_w[i]), "h": int(g.frame_h[i]), "fmt": _abi.PIX_F16,
+               "gain": float(g.gains[i]), "H": g.H[i]} for i, t in enumerate(dev_frames)]
+    outs, owners, descs, contrib = [], [], [], []
+    for c in g.chunks:
+        o = torch.zeros((c.h, c.w, 3), dtype=torch.float32, device="cuda")
+        ow = torch.full((c.h, c.w), -7, dtype=torch.int32, device="cuda")
+        descs.append({"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": len(contrib), "count": len(c.contributors),
+                      "out": o.data_ptr(), "owner": ow.data_ptr() if want_owner else None})
+        contrib += c.contributors
+        outs.append(o)
+        owners.append(ow)
+    hip.check(hip.render_chunks(frames, descs, contrib, bands, mode), f"render_chunks mode {mode}")
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in outs], [w.cpu().numpy() for w in owners]
+
+
+def _oracle(g, frames, chunk, fp64, bands=BANDS):
+    srcs = [frames[i].astype(np.float32) for i in chunk.contributors]
+    if fp64:
+        hinv = np.stack([po.invert_homography_f64(g.H[i]) for i in chunk.contributors])
+    else:
+        hinv = np.stack([po.invert_homography(g.H[i].astype(np.float32)) for i in chunk.contributors])
+    gains = g.gains[chunk.contributors]
+    return po.render_chunk(srcs, hinv, gains, chunk.x, chunk.y, chunk.w, chunk.h, bands, fp64=fp64, want_owner=True)
+
+
+@pytest.fixture(scope="module")
+def case():
+    g, frames = small_case(seed=3, chunk=256)
+    return g, frames, _upload(frames)
+
+
+def test_fp64_form_matches_fp64_oracle(hip, case):
+    g, frames, dev = case
+    note("render fp64")
+    outs, owners = _gpu_render(hip, g, dev, _abi.RENDER_FP64)
+    for k, c in enumerate(g.chunks):
+        want, want_owner = _oracle(g, frames, c, fp64=True)
+        assert np.array_equal(owners[k], want_owner), k
+        assert np.max(np.abs(outs[k] - want)) <= 1e-9, k
+
+
+def test_plain_form_matches_binary32_oracle(hip, case):
+    g, frames, dev = case
+    note("render plain")
+    outs, owners = _gpu_render(hip, g, dev, _abi.RENDER_PLAIN)
+    for k, c in enumerate(g.chunks):
+        want, want_owner = _oracle(g, frames, c, fp64=False)
+        assert np.array_equal(owners[k], want_owner), k
+        assert np.max(np.abs(outs[k] - want)) <= 2e-6, (k, np.max(np.abs(outs[k] - want)))
+
+
+def test_precision_form_within_tolerance_of_fp64(hip, case):
+    g, frames, dev = case
+    note("render precision")
+    outs, _ = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION)
+    full_o = []
+    full_w = []
+    for k, c in enumerate(g.chunks):
+        want, _ = _oracle(g, frames, c, fp64=True)
+        full_o.append(outs[k].ravel())
+        full_w.append(want.ravel())
+        assert np.max(np.abs(outs[k] - want)) <= 1e-3, (k, np.max(np.abs(outs[k] - want)))
+    assert psnr(np.concatenate(full_o), np.concatenate(full_w)) >= 50.0
+
+
+def test_batched_equals_precision_and_replays(hip, case):
+    g, frames, dev = case
+    note("render batched x3")
+    ref, ref_own = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION)
+    for rep in range(3):  # first call captures the graph, later calls replay it
+        outs, owners = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED)
+        for k in range(len(g.chunks)):
+            assert np.array_equal(owners[k], ref_own[k]), (rep, k)
+            assert np.array_equal(outs[k], ref[k]), (rep, k)
+
+
+def test_precision_map_error_is_tiny(hip, case):
+    g, frames, dev = case
+    frames_d = [{"ptr": t.data_ptr(), "w": int(g.frame_w[i]), "h": int(g.frame_h[i]), "fmt": _abi.PIX_F16,
+                 "gain": 1.0, "H": g.H[i]} for i, t in enumerate(dev)]
+    descs, contrib = [], []
+    for c in g.chunks:
+        descs.append({"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": len(contrib), "count": len(c.contributors),
+                      "out": 1, "owner": None})
+        contrib += c.contributors
+    err = hip.render_map_error(frames_d, descs, contrib)
+    note(f"map error {err}")
+    assert err < 1e-3
+
+
+def test_gains_and_one_chunk_noise(hip):
+    g, frames = small_case(seed=11, n=3, gains=[1.0, 1.2, 0.85], noise=0.01)
+    dev = _upload(frames)
+    note("render gains/noise plain")
+    outs, owners = _gpu_render(hip, g, dev, _abi.RENDER_PLAIN)
+    want, want_owner = _oracle(g, frames, g.chunks[0], fp64=False)
+    assert np.array_equal(owners[0], want_owner)
+    assert np.max(np.abs(outs[0] - want)) <= 2e-6
+    outs64, own64 = _gpu_render(hip, g, dev, _abi.RENDER_FP64)
+    want64, wo64 = _oracle(g, frames, g.chunks[0], fp64=True)
+    assert np.array_equal(own64[0], wo64)
+    assert np.max(np.abs(outs64[0] - want64)) <= 1e-9
+
+
+def test_seven_bands_and_uneven_chunks(hip):
+    g, frames = small_case(seed=5, n=4, chunk=173)
+    dev = _upload(frames)
+    note("render 7 bands uneven chunks")
+    outs, owners = _gpu_render(hip, g, dev, _abi.RENDER_PLAIN, bands=7)
+    for k, c in enumerate(g.chunks):
+        want, want_owner = _oracle(g, frames, c, fp64=False, bands=7)
+        assert np.array_equal(owners[k], want_owner), k
+        assert np.max(np.abs(outs[k] - want)) <= 2e-6, k
+    b, bo = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED, bands=7)
+    p, po_ = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION, bands=7)
+    for k in range(len(g.chunks)):
+        assert np.array_equal(b[k], p[k])
