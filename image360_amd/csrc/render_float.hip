@@ -759,9 +759,17 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         pc.lohi = take(8 * lines); pc.prevl = take(4 * lines); pc.chosen = take(4 * lines); pc.back = take(64 * lines);
         pch.push_back(pc);
         P.chunks.push_back(ch);
+        // pairs in the reference's order (SeamBlend.cpp:573-584); pairs whose
+        // footprints are disjoint have disjoint coverage boxes, which the
+        // reference skips, so they are left out here
         std::vector<std::pair<int, int>> pr;
         for (int i = 0; i < ch.count; ++i)
-            for (int j = i + 1; j < ch.count; ++j) pr.push_back({i, j});
+            for (int j = i + 1; j < ch.count; ++j) {
+                const int* a = P.items[ch.first + i].lbox[0];
+                const int* b = P.items[ch.first + j].lbox[0];
+                if (std::max(a[0], b[0]) <= std::min(a[2], b[2]) && std::max(a[1], b[1]) <= std::min(a[3], b[3]))
+                    pr.push_back({i, j});
+            }
         P.pairs.push_back(pr);
         P.max_w = std::max(P.max_w, cc.width);
         if (!batched) slot_max = std::max(slot_max, off);
