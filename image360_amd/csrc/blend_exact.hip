@@ -81,6 +81,14 @@ __global__ void __launch_bounds__(256) coverage_u16_kernel(PlaneTable imgs, int 
     }
 }
 
+// Work table of the chamfer launch, written on the device (no host staging).
+__global__ void chamfer_table_kernel(ChamferWork* work, const uint8_t* cov, float* dist, const int* boxes, int c,
+                                     int w, int h) {
+    const int i = threadIdx.x;
+    const size_t n = static_cast<size_t>(w) * h;
+    if (i < c) work[i] = ChamferWork{cov + n * i, dist + n * i, boxes + 4 * i, w, h};
+}
+
 __global__ void box_init_kernel(int* boxes, int c) {
     const int i = threadIdx.x;
     if (i < c) { boxes[4 * i] = INT_MAX; boxes[4 * i + 1] = INT_MAX; boxes[4 * i + 2] = -1; boxes[4 * i + 3] = -1; }
@@ -422,13 +430,8 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     }
     // 2. chamfer, forward then backward (one block per contributor)
     {
-        ChamferWork hw[I360HIP_MAX_CONTRIBUTORS];
-        for (int i = 0; i < c; ++i) hw[i] = ChamferWork{cov + n * i, dist + n * i, boxes + 4 * i, w, h};
         auto* dwork = reinterpret_cast<ChamferWork*>(base + oWork);
-        if (hipMemcpyAsync(dwork, hw, sizeof(ChamferWork) * c, hipMemcpyHostToDevice, s) != hipSuccess) {
-            st.set(I360HIP_DEVICE_ERROR, "chamfer work table upload failed");
-            return;
-        }
+        BL_LAUNCH(st, "chamfer_table", 0, chamfer_table_kernel<<<1, 64, 0, s>>>(dwork, cov, dist, boxes, c, w, h));
         {
             ProfScope ps(ctx, s, "chamfer", 13.0 * n * c);
             if (launch_chamfer(dwork, c, w, s) != hipSuccess) {

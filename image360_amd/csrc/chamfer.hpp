@@ -70,10 +70,12 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
         }
     };
 
+    // slot k serves rows k - base + j*P; fill it with its first row >= 0
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const int r = k - base;
-        if (active && r >= 0 && r < rows) load_row(r, ring[k]);
+        int r = k - base;
+        if (r < 0) r += ((-r + P - 1) / P) * P;
+        if (active && r < rows) load_row(r, ring[k]);
     }
 
     const int tmax = 2 * (rows - 1) + nact - 1;
