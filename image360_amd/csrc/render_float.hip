@@ -150,23 +150,46 @@ __device__ __forceinline__ TileMap tile_map(const double* h, double X0, double Y
     return t;
 }
 
+// Validity of canvas pixel (x, y) of the chunk under the FP64 map
+// (WarpPerspective_Scalar.cpp:108-122 in binary64).
+__device__ __forceinline__ bool fp64_valid(const RItem& it, const RChunk& ch, int x, int y) {
+    const double* m = it.hinv;
+    const double X = static_cast<double>(ch.x + x), Y = static_cast<double>(ch.y + y);
+    const double q = (m[6] * X + m[7] * Y) + m[8];
+    if (fabs(q) < 1e-10) return false;
+    const double u = ((m[0] * X + m[1] * Y) + m[2]) / q;
+    const double v = ((m[3] * X + m[4] * Y) + m[5]) / q;
+    const double fu = floor(u), fv = floor(v);
+    return fu >= 0.0 && fv >= 0.0 && fu <= static_cast<double>(it.fw - 2) && fv <= static_cast<double>(it.fh - 2);
+}
+
 // Integer sample origin and fractions for one pixel; false if invalid.
 template <int MODE, typename M>
 __device__ __forceinline__ bool map_pixel(const RItem& it, const RChunk& ch, const TileMap& tm, int x, int y,
                                           int dx, int dy, int& sx0, int& sy0, M& fx, M& fy) {
     if (MODE == kModePrecision && tm.ok) {
+        // The sample/no-sample decision is a discontinuity: take it from the
+        // FP64 map, exactly as the FP64 form does, so both forms agree on
+        // coverage (and hence on ownership).  The sample position comes from
+        // the FP32 tile map; bilinear sampling is continuous across integer
+        // boundaries, so a position that lands a hair past the last valid
+        // cell is resolved to that cell's edge.
+        if (!fp64_valid(it, ch, x, y)) return false;
         const float den = (1.0f + tm.g0 * static_cast<float>(dx)) + tm.g1 * static_cast<float>(dy);
-        if (fabsf(tm.q0 * den) < 1e-10f) return false;
         const float ox = (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) / den;
         const float oy = (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) / den;
         const float lx = tm.fx + ox, ly = tm.fy + oy;
-        if (!(fabsf(lx) < 1e8f && fabsf(ly) < 1e8f)) return false;
         const float flx = floorf(lx), fly = floorf(ly);
         sx0 = tm.ix + static_cast<int>(flx);
         sy0 = tm.iy + static_cast<int>(fly);
-        fx = static_cast<M>(lx - flx);
-        fy = static_cast<M>(ly - fly);
-        return sx0 >= 0 && sy0 >= 0 && sx0 <= it.fw - 2 && sy0 <= it.fh - 2;
+        float ffx = lx - flx, ffy = ly - fly;
+        if (sx0 > it.fw - 2) { sx0 = it.fw - 2; ffx = 1.0f; }
+        if (sx0 < 0) { sx0 = 0; ffx = 0.0f; }
+        if (sy0 > it.fh - 2) { sy0 = it.fh - 2; ffy = 1.0f; }
+        if (sy0 < 0) { sy0 = 0; ffy = 0.0f; }
+        fx = static_cast<M>(ffx);
+        fy = static_cast<M>(ffy);
+        return true;
     }
     if (MODE == kModePlain) {
         const float* m = it.hinv32;
@@ -346,6 +369,49 @@ struct RgbaImg {
         o[0] = v[0]; o[1] = v[1]; o[2] = v[2];
     }
     __device__ static constexpr M smooth() { return M(2) / M(65535); }
+};
+
+// Contributor image for the seam DP of the precision forms: each sample is
+// recomputed from the source frame with the FP64 map and binary64
+// bilinear weights (the FP64 form's warp, value for value), so the DP
+// costs, and therefore the seams, equal the FP64 form's.
+struct MapImg {
+    const void* src;
+    int fw, fh, fmt;
+    float gain;
+    double h[9];
+    int cx, cy, w;
+    __device__ __forceinline__ void rgb(size_t i, double (&o)[3]) const {
+        const int x = static_cast<int>(i % static_cast<size_t>(w)), y = static_cast<int>(i / static_cast<size_t>(w));
+        const double X = static_cast<double>(cx + x), Y = static_cast<double>(cy + y);
+        double u = (h[0] * X + h[1] * Y) + h[2];
+        double v = (h[3] * X + h[4] * Y) + h[5];
+        const double q = (h[6] * X + h[7] * Y) + h[8];
+        o[0] = o[1] = o[2] = 0.0;
+        if (fabs(q) < 1e-10) return;
+        u = u / q;
+        v = v / q;
+        const double fu = floor(u), fv = floor(v);
+        if (!(fu >= 0.0 && fv >= 0.0 && fu <= static_cast<double>(fw - 2) && fv <= static_cast<double>(fh - 2))) return;
+        const int sx0 = static_cast<int>(fu), sy0 = static_cast<int>(fv);
+        const double fx = u - fu, fy = v - fv;
+        const double omx = 1.0 - fx, omy = 1.0 - fy;
+        const double w00 = omx * omy, w10 = fx * omy, w01 = omx * fy, w11 = fx * fy;
+        double a[3], b[3], c[3], d[3];
+        const size_t i0 = static_cast<size_t>(sy0) * fw + sx0;
+        fetch_rgb<double>(src, fmt, i0, a);
+        fetch_rgb<double>(src, fmt, i0 + 1, b);
+        fetch_rgb<double>(src, fmt, i0 + fw, c);
+        fetch_rgb<double>(src, fmt, i0 + fw + 1, d);
+        const bool useGain = fabsf(gain - 1.0f) >= 1e-6f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double val = ((w00 * a[k] + w10 * b[k]) + w01 * c[k]) + w11 * d[k];
+            if (useGain) val = val * static_cast<double>(gain);
+            o[k] = val < 0.0 ? 0.0 : (1.0 < val ? 1.0 : val);
+        }
+    }
+    __device__ static constexpr double smooth() { return 2.0 / 65535.0; }
 };
 
 // ------------------------------------------------------------ reduce ---
@@ -879,10 +945,31 @@ struct DevTables {
     ChamferWork* chamfer = nullptr;
     SeamWork<RgbaImg<double, double>>* seam64 = nullptr;
     SeamWork<RgbaImg<float, float>>* seam32 = nullptr;
-    SeamWork<RgbaImg<_Float16, float>>* seam16 = nullptr;
+    SeamWork<MapImg>* seamMap = nullptr;
     std::vector<std::vector<int>> round_chunks;  // seam round k -> chunk ids
     std::vector<int> round_off;                  // offset of round k in the seam table
 };
+
+template <typename Img>
+static Img make_img(const RItem& it, const RChunk&) {
+    return Img{it.warped};
+}
+template <>
+MapImg make_img<MapImg>(const RItem& it, const RChunk& ch) {
+    MapImg m{};
+    m.src = it.src; m.fw = it.fw; m.fh = it.fh; m.fmt = it.fmt; m.gain = it.gain;
+    // the inverse is filled on the device; build it here with the same formula
+    const double* f = it.hfwd;
+    const double a = f[0], b = f[1], c = f[2], d = f[3], e = f[4], g5 = f[5], g = f[6], k = f[7], j = f[8];
+    const double c0 = e * j - g5 * k, c1 = d * j - g5 * g, c2 = d * k - e * g;
+    const double det = (a * c0 - b * c1) + c * c2;
+    const double s = 1.0 / det;
+    m.h[0] = c0 * s; m.h[1] = (c * k - b * j) * s; m.h[2] = (b * g5 - c * e) * s;
+    m.h[3] = (g5 * g - d * j) * s; m.h[4] = (a * j - c * g) * s; m.h[5] = (c * d - a * g5) * s;
+    m.h[6] = c2 * s; m.h[7] = (b * g - a * k) * s; m.h[8] = (a * e - b * d) * s;
+    m.cx = ch.x; m.cy = ch.y; m.w = ch.w;
+    return m;
+}
 
 template <typename Img>
 static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables& T) {
@@ -900,8 +987,8 @@ static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables
             const RItem& a = P.items[ch.first + i];
             const RItem& b = P.items[ch.first + j];
             SeamWork<Img> s{};
-            s.imgI = Img{a.warped};
-            s.imgJ = Img{b.warped};
+            s.imgI = make_img<Img>(a, ch);
+            s.imgJ = make_img<Img>(b, ch);
             s.covI = a.cov; s.covJ = b.cov; s.boxes = ch.boxes; s.ci = i; s.cj = j; s.w = ch.w;
             s.owner = ch.owner; s.lohi = ch.lohi; s.prevl = ch.prevl; s.chosen = ch.chosen; s.back = ch.back;
             out.push_back(s);
@@ -981,7 +1068,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
             else if (MM == kModePlain)
                 seam_dp_table_kernel<RgbaImg<float, float>, float><<<cnt, 1024, 0, s>>>(T.seam32 + first);
             else
-                seam_dp_table_kernel<RgbaImg<_Float16, float>, float><<<cnt, 1024, 0, s>>>(T.seam16 + first);
+                seam_dp_table_kernel<MapImg, double><<<cnt, 1024, 0, s>>>(T.seamMap + first);
             if (hipGetLastError() != hipSuccess) {
                 st.set(I360HIP_KERNEL_LAUNCH_FAILED, "seam launch failed");
                 return;
@@ -1042,7 +1129,7 @@ void render_release(i360hip_ctx* ctx) {
 static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s, Status& st, int slot) {
     std::vector<SeamWork<RgbaImg<double, double>>> s64;
     std::vector<SeamWork<RgbaImg<float, float>>> s32;
-    std::vector<SeamWork<RgbaImg<_Float16, float>>> s16;
+    std::vector<SeamWork<MapImg>> s16;
     T = DevTables{};
     if (P.mode == kModeFP64) seam_table(P, s64, T);
     else if (P.mode == kModePlain) seam_table(P, s32, T);
@@ -1076,7 +1163,7 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     T.chamfer = reinterpret_cast<ChamferWork*>(dev + oW);
     T.seam64 = reinterpret_cast<decltype(T.seam64)>(dev + oS);
     T.seam32 = reinterpret_cast<decltype(T.seam32)>(dev + oS);
-    T.seam16 = reinterpret_cast<decltype(T.seam16)>(dev + oS);
+    T.seamMap = reinterpret_cast<decltype(T.seamMap)>(dev + oS);
 }
 
 static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc,

@@ -42,6 +42,8 @@ def _gpu_render(hip, g, dev_frames, mode, bands=BANDS, want_owner=True):
 
 
 def _oracle(g, frames, chunk, fp64, bands=BANDS):
+    if not chunk.contributors:  # nothing covers the chunk: black, unowned
+        return np.zeros((chunk.h, chunk.w, 3), np.float32), np.full((chunk.h, chunk.w), -1, np.int32)
     srcs = [frames[i].astype(np.float32) for i in chunk.contributors]
     if fp64:
         hinv = np.stack([po.invert_homography_f64(g.H[i]) for i in chunk.contributors])
@@ -80,11 +82,13 @@ def test_plain_form_matches_binary32_oracle(hip, case):
 def test_precision_form_within_tolerance_of_fp64(hip, case):
     g, frames, dev = case
     note("render precision")
-    outs, _ = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION)
+    outs, owners = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION)
     full_o = []
     full_w = []
     for k, c in enumerate(g.chunks):
-        want, _ = _oracle(g, frames, c, fp64=True)
+        want, want_owner = _oracle(g, frames, c, fp64=True)
+        # every discrete decision (coverage, seams) comes from the FP64 map
+        assert np.array_equal(owners[k], want_owner), k
         full_o.append(outs[k].ravel())
         full_w.append(want.ravel())
         assert np.max(np.abs(outs[k] - want)) <= 1e-3, (k, np.max(np.abs(outs[k] - want)))
