@@ -163,7 +163,7 @@ def main():
         return ev0.elapsed_time(ev1)
 
     forms = {}
-    for name, mode, steps in (("plain", _abi.RENDER_PLAIN, 1), ("precision", _abi.RENDER_PRECISION, args.steps),
+    for name, mode, steps in (("plain", _abi.RENDER_PLAIN, 1), ("precision", _abi.RENDER_PRECISION, 1),
                               ("batched", _abi.RENDER_BATCHED, args.steps)):
         ms = timed(mode, steps, args.warmup)
         forms[name] = {"mp_s": round(out_px * steps / (ms / 1e3) / 1e6, 1), "ms_per_step": round(ms / steps, 2),

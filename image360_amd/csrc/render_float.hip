@@ -1171,7 +1171,7 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     RenderCache& rc = cache_of(ctx);
     Plan P;
     if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st)) return;
-    const bool reuse = rc.graph && rc.key == P.key && rc.mode == mode && mode == kModeBatched;
+    const bool reuse = !ctx->profiling && rc.graph && rc.key == P.key && rc.mode == mode && mode == kModeBatched;
     if (reuse) {
         if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
         return;
