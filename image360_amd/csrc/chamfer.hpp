@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
                     if (r + P < rows) load_row(r + P, ring[k]);
                 }
             }
-            __syncthreads();
+            lds_barrier();  // neighbours exchange through LDS only
         }
     }
 }

@@ -90,6 +90,16 @@ struct i360hip_ctx {
 };
 
 namespace i360 {
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// operations (lgkmcnt), not for its outstanding global loads and stores,
+// so prefetches and fire-and-forget stores stay in flight across it.  Use
+// only where the data exchanged between threads lives in LDS.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Context-owned device buffer `slot` of at least `bytes` bytes (grown on
 // demand, kept until the context is destroyed).
 void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st);

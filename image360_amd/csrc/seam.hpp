@@ -72,40 +72,80 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
 
     // Phase 1: crossover of the current owner map and contested band per line.
-    for (int line = wave; line < lines; line += nwaves) {
-        int first = -1, cross = -1;
-        for (int base = 0; base < len && cross < 0; base += 64) {
-            const int t = base + lane;
-            const int o = (t < len) ? owner[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
-            const bool ij = (t < len) && (o == ci || o == cj);
-            const unsigned long long m = __ballot(ij);
-            int start = 0;
-            if (first < 0) {
-                if (m == 0ull) continue;
-                const int f = __ffsll(static_cast<long long>(m)) - 1;
-                first = __shfl(o, f);
-                start = f + 1;
+    if (rowScan) {
+        // lines are rows: one wavefront per line, lanes along the line
+        for (int line = wave; line < lines; line += nwaves) {
+            int first = -1, cross = -1;
+            for (int base = 0; base < len && cross < 0; base += 64) {
+                const int t = base + lane;
+                const int o = (t < len) ? owner[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
+                const bool ij = (t < len) && (o == ci || o == cj);
+                const unsigned long long m = __ballot(ij);
+                int start = 0;
+                if (first < 0) {
+                    if (m == 0ull) continue;
+                    const int f = __ffsll(static_cast<long long>(m)) - 1;
+                    first = __shfl(o, f);
+                    start = f + 1;
+                }
+                unsigned long long m2 = __ballot(ij && o != first);
+                m2 = (start >= 64) ? 0ull : (m2 & (~0ull << start));
+                if (m2) cross = base + __ffsll(static_cast<long long>(m2)) - 1;
             }
-            unsigned long long m2 = __ballot(ij && o != first);
-            m2 = (start >= 64) ? 0ull : (m2 & (~0ull << start));
-            if (m2) cross = base + __ffsll(static_cast<long long>(m2)) - 1;
+            int lo = 1, hi = 0;
+            if (cross >= 0) {
+                const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
+                const int t = a + lane;
+                bool both = false;
+                if (t <= b) {
+                    const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
+                    both = covI[p] && covJ[p];
+                }
+                const unsigned long long m = __ballot(both);
+                if (m) {
+                    lo = a + __ffsll(static_cast<long long>(m)) - 1;
+                    hi = a + 63 - __clzll(static_cast<long long>(m));
+                }
+            }
+            if (lane == 0) { lohi[2 * line] = lo; lohi[2 * line + 1] = hi; chosen[line] = -1; }
         }
-        int lo = 1, hi = 0;
-        if (cross >= 0) {
-            const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
-            const int t = a + lane;
-            bool both = false;
-            if (t <= b) {
-                const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
-                both = covI[p] && covJ[p];
+    } else {
+        // lines are columns: lanes take adjacent columns and walk down them
+        // together, so every owner load of the wavefront is one row segment
+        constexpr int U = 16;
+        for (int l0 = wave * 64; l0 < lines; l0 += nwaves * 64) {
+            const int line = l0 + lane;
+            const bool valid = line < lines;
+            int first = -1, cross = -1;
+            for (int t0 = 0; t0 < len; t0 += U) {
+                if (!__any(valid && cross < 0)) break;
+                int ob[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = t0 + u;
+                    ob[u] = (valid && cross < 0 && t < len) ? owner[seam_at(false, ox0, oy0, w, line, t)] : -2;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int o = ob[u];
+                    if (cross >= 0 || !(o == ci || o == cj)) continue;
+                    if (first < 0) first = o;
+                    else if (o != first) cross = t0 + u;
+                }
             }
-            const unsigned long long m = __ballot(both);
-            if (m) {
-                lo = a + __ffsll(static_cast<long long>(m)) - 1;
-                hi = a + 63 - __clzll(static_cast<long long>(m));
+            int lo = 1, hi = 0;
+            if (valid && cross >= 0) {
+                const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
+                for (int t = a; t <= b; ++t) {
+                    const size_t p = seam_at(false, ox0, oy0, w, line, t);
+                    if (covI[p] && covJ[p]) {
+                        if (hi < lo) { lo = t; hi = t; }
+                        else { hi = t; }
+                    }
+                }
             }
+            if (valid) { lohi[2 * line] = lo; lohi[2 * line + 1] = hi; chosen[line] = -1; }
         }
-        if (lane == 0) { lohi[2 * line] = lo; lohi[2 * line + 1] = hi; chosen[line] = -1; }
     }
     __syncthreads();
 
