@@ -34,6 +34,43 @@ struct ChamferWork {
 
 constexpr float kChamferDiag = 1.41421356f;  // SeamBlend.cpp:209
 
+template <int S>
+struct StripIO;  // vector access to S consecutive values of one row
+template <>
+struct StripIO<4> {
+    static __device__ __forceinline__ void cov(const uint8_t* p, float k, float (&d)[4]) {
+        const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) d[s] = ((q >> (8 * s)) & 0xffu) ? k : 0.0f;
+    }
+    static __device__ __forceinline__ void ld(const float* p, float (&d)[4]) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.w;
+    }
+    static __device__ __forceinline__ void st(float* p, const float (&d)[4]) {
+        *reinterpret_cast<float4*>(p) = make_float4(d[0], d[1], d[2], d[3]);
+    }
+};
+template <int S>
+struct StripIO {
+    static __device__ __forceinline__ void cov(const uint8_t* p, float k, float (&d)[S]) {
+#pragma unroll
+        for (int q = 0; q < S / 4; ++q) StripIO<4>::cov(p + 4 * q, k, *reinterpret_cast<float(*)[4]>(d + 4 * q));
+    }
+    static __device__ __forceinline__ void ld(const float* p, float (&d)[S]) {
+#pragma unroll
+        for (int q = 0; q < S / 4; ++q) StripIO<4>::ld(p + 4 * q, *reinterpret_cast<float(*)[4]>(d + 4 * q));
+    }
+    static __device__ __forceinline__ void st(float* p, const float (&d)[S]) {
+#pragma unroll
+        for (int q = 0; q < S / 4; ++q) StripIO<4>::st(p + 4 * q, *reinterpret_cast<const float(*)[4]>(d + 4 * q));
+    }
+};
+
+// Block layout: the first half of the wavefronts owns the even strips, the
+// second half the odd ones.  Strips of one parity are busy on the same
+// steps (row r of strip j runs at step 2r + j), so every wavefront is
+// either fully busy or idle at each step.
 template <int S, int P, bool FWD>
 __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __restrict__ work) {
     __shared__ float firstv[2][1024];
@@ -45,9 +82,11 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
     const int xa = (x0 / S) * S;
     const int nact = (x1 + 1 - xa + S - 1) / S;
     const int rows = y1 - y0 + 1;
-    const int t = threadIdx.x;
+    const int half = blockDim.x >> 1;
+    const int t = (threadIdx.x < half) ? 2 * threadIdx.x : 2 * (threadIdx.x - half) + 1;  // strip index
     const bool active = t < nact;
     const int xs = xa + t * S;
+    const bool vec = (w & 3) == 0 && xs + S <= w;  // 16-byte aligned, whole strip inside the row
     const float kInf = static_cast<float>(w + h);
     const int lag = FWD ? t : (nact - 1 - t);
     const int base = lag >> 1, par = lag & 1;
@@ -61,12 +100,17 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
 
     auto load_row = [&](int r, float (&dst)[S]) {
         const int y = FWD ? (y0 + r) : (y1 - r);
-        const size_t rb = static_cast<size_t>(y) * w;
+        const size_t rb = static_cast<size_t>(y) * w + xs;
+        if (vec) {
+            if (FWD) StripIO<S>::cov(wk.cov + rb, kInf, dst);
+            else StripIO<S>::ld(wk.dist + rb, dst);
+            return;
+        }
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const int x = xs + s;
-            if (FWD) dst[s] = (x < w && wk.cov[rb + x]) ? kInf : 0.0f;
-            else dst[s] = (x < w) ? wk.dist[rb + x] : 0.0f;
+            const bool in = xs + s < w;
+            if (FWD) dst[s] = (in && wk.cov[rb + s]) ? kInf : 0.0f;
+            else dst[s] = in ? wk.dist[rb + s] : 0.0f;
         }
     };
 
@@ -124,12 +168,16 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
                             cur[s] = (x < w) ? b : 0.0f;
                         }
                     }
-                    float* row = wk.dist + static_cast<size_t>(y) * w;
+                    float* row = wk.dist + static_cast<size_t>(y) * w + xs;
+                    if (vec) {
+                        StripIO<S>::st(row, cur);
+                    } else {
 #pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        if (xs + s < w) row[xs + s] = cur[s];
-                        prev[s] = cur[s];
+                        for (int s = 0; s < S; ++s)
+                            if (xs + s < w) row[s] = cur[s];
                     }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) prev[s] = cur[s];
                     firstv[cb][t] = cur[0];
                     lastv[cb][t] = cur[S - 1];
                     if (r + P < rows) load_row(r + P, ring[k]);
@@ -143,7 +191,7 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
 // Threads per block for chunks of width w at strip width S.
 inline int ChamferThreads(int w, int S) {
     const int cols = (w + S - 1) / S;
-    return ((cols + 63) / 64) * 64;
+    return ((cols + 127) / 128) * 128;  // even and odd strips in separate wavefronts
 }
 
 inline int ChamferStrip(int w) { return w <= 4096 ? 4 : (w <= 8192 ? 8 : 16); }
