@@ -86,7 +86,8 @@ __global__ void chamfer_table_kernel(ChamferWork* work, const uint8_t* cov, floa
                                      int w, int h) {
     const int i = threadIdx.x;
     const size_t n = static_cast<size_t>(w) * h;
-    if (i < c) work[i] = ChamferWork{cov + n * i, dist + n * i, boxes + 4 * i, w, h};
+    const size_t np = static_cast<size_t>(DistPitch(w)) * h;
+    if (i < c) work[i] = ChamferWork{cov + n * i, dist + np * i, boxes + 4 * i, w, h};
 }
 
 __global__ void box_init_kernel(int* boxes, int c) {
@@ -101,12 +102,14 @@ __global__ void __launch_bounds__(256) ownership_kernel(const float* __restrict_
     const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int x = static_cast<int>(p % w), y = static_cast<int>(p / w);
+    const size_t np = static_cast<size_t>(DistPitch(w)) * (n / w);
+    const size_t q = static_cast<size_t>(y) * DistPitch(w) + x;
     int best = -1;
     float bd = 0.0f;
     for (int i = 0; i < c; ++i) {
         const int* b = boxes + 4 * i;
         if (x < b[0] || x > b[2] || y < b[1] || y > b[3]) continue;
-        const float v = dist[n * i + p];
+        const float v = dist[np * i + q];
         if (v > bd) { bd = v; best = i; }
     }
     owner[p] = best;
@@ -402,7 +405,8 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     // workspace layout
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
-    const size_t oCov = take(n * c), oDist = take(n * c * 4), oOwner = take(n * 4), oBox = take(4 * 4 * c);
+    const size_t oCov = take(n * c), oDist = take(static_cast<size_t>(DistPitch(w)) * h * c * 4);
+    const size_t oOwner = take(n * 4), oBox = take(4 * 4 * c);
     const size_t oLohi = take(8 * static_cast<size_t>(maxLines)), oPrev = take(4 * static_cast<size_t>(maxLines));
     const size_t oChosen = take(4 * static_cast<size_t>(maxLines)), oBack = take(64 * static_cast<size_t>(maxLines));
     const size_t oWork = take(sizeof(ChamferWork) * I360HIP_MAX_CONTRIBUTORS);

@@ -26,46 +26,24 @@
 namespace i360 {
 
 struct ChamferWork {
-    const uint8_t* cov;  // w*h coverage (0/1)
-    float* dist;         // w*h distances, written inside the box only
+    const uint8_t* cov;  // w*h coverage (0/1), row pitch w
+    float* dist;         // distances, row pitch DistPitch(w), written inside the box only
     const int* box;      // x0, y0, x1, y1 (x1 < 0: nothing covered)
     int w, h;
 };
 
+// Row pitch (floats) of a distance plane: a multiple of 4 so every strip
+// is one 16-byte access.
+__host__ __device__ inline int DistPitch(int w) { return (w + 3) & ~3; }
+
 constexpr float kChamferDiag = 1.41421356f;  // SeamBlend.cpp:209
 
-template <int S>
-struct StripIO;  // vector access to S consecutive values of one row
-template <>
-struct StripIO<4> {
-    static __device__ __forceinline__ void cov(const uint8_t* p, float k, float (&d)[4]) {
-        const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) d[s] = ((q >> (8 * s)) & 0xffu) ? k : 0.0f;
-    }
-    static __device__ __forceinline__ void ld(const float* p, float (&d)[4]) {
-        const float4 q = *reinterpret_cast<const float4*>(p);
-        d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.w;
-    }
-    static __device__ __forceinline__ void st(float* p, const float (&d)[4]) {
-        *reinterpret_cast<float4*>(p) = make_float4(d[0], d[1], d[2], d[3]);
-    }
-};
-template <int S>
-struct StripIO {
-    static __device__ __forceinline__ void cov(const uint8_t* p, float k, float (&d)[S]) {
-#pragma unroll
-        for (int q = 0; q < S / 4; ++q) StripIO<4>::cov(p + 4 * q, k, *reinterpret_cast<float(*)[4]>(d + 4 * q));
-    }
-    static __device__ __forceinline__ void ld(const float* p, float (&d)[S]) {
-#pragma unroll
-        for (int q = 0; q < S / 4; ++q) StripIO<4>::ld(p + 4 * q, *reinterpret_cast<float(*)[4]>(d + 4 * q));
-    }
-    static __device__ __forceinline__ void st(float* p, const float (&d)[S]) {
-#pragma unroll
-        for (int q = 0; q < S / 4; ++q) StripIO<4>::st(p + 4 * q, *reinterpret_cast<const float(*)[4]>(d + 4 * q));
-    }
-};
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t ld_u32(const void* p) { return *gptr(static_cast<const uint32_t*>(p)); }
+__device__ __forceinline__ uint32_t ld_u8(const uint8_t* p) { return *gptr(p); }
+__device__ __forceinline__ f4v ld_f4(const float* p) { return *gptr(reinterpret_cast<const f4v*>(p)); }
+__device__ __forceinline__ void st_f4(float* p, f4v v) { *gptr(reinterpret_cast<f4v*>(p)) = v; }
 
 // Block layout: the first half of the wavefronts owns the even strips, the
 // second half the odd ones.  Strips of one parity are busy on the same
@@ -86,47 +64,68 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
     const int t = (threadIdx.x < half) ? 2 * threadIdx.x : 2 * (threadIdx.x - half) + 1;  // strip index
     const bool active = t < nact;
     const int xs = xa + t * S;
-    const bool vec = (w & 3) == 0 && xs + S <= w;  // 16-byte aligned, whole strip inside the row
+    const int dp = DistPitch(w);
+    const int xc = min(xs, dp - S);     // an in-bounds strip for inactive threads' loads
+    const int xb = min(xs, w - 1);
     const float kInf = static_cast<float>(w + h);
     const int lag = FWD ? t : (nact - 1 - t);
     const int base = lag >> 1, par = lag & 1;
     const bool lastT = (t == nact - 1);
     const bool edgePrev = FWD ? (y0 > 0) : (y1 < h - 1);  // a row exists beyond the box (all zeros)
 
-    float ring[P][S];
+    // raw prefetched row data; converted only when the row is processed so
+    // that no instruction depends on a load until its row's step
+    //   FWD, vector path: packed coverage bytes in ring[k][0 .. S/4)
+    //   FWD, scalar path: one coverage byte per entry
+    //   BWD: the forward distance bits
+    uint32_t ring[P][S];  // rows of the current superstep
+    uint32_t next[P][S];  // rows of the next superstep, in flight
     float prev[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) prev[s] = 0.0f;
 
-    auto load_row = [&](int r, float (&dst)[S]) {
+    // loads are unconditional (rows and strips clamped into the box) so
+    // that no register is merged across a branch while its load is in flight
+    auto load_row = [&](int r, uint32_t (&dst)[S]) {
         const int y = FWD ? (y0 + r) : (y1 - r);
-        const size_t rb = static_cast<size_t>(y) * w + xs;
-        if (vec) {
-            if (FWD) StripIO<S>::cov(wk.cov + rb, kInf, dst);
-            else StripIO<S>::ld(wk.dist + rb, dst);
-            return;
-        }
+        if (FWD) {
+            const uint8_t* c = wk.cov + static_cast<size_t>(y) * w;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const bool in = xs + s < w;
-            if (FWD) dst[s] = (in && wk.cov[rb + s]) ? kInf : 0.0f;
-            else dst[s] = in ? wk.dist[rb + s] : 0.0f;
+            for (int s = 0; s < S; ++s) dst[s] = ld_u8(c + min(xb + s, w - 1));
+        } else {
+            const float* d = wk.dist + static_cast<size_t>(y) * dp + xc;
+#pragma unroll
+            for (int g = 0; g < S / 4; ++g) {
+                const f4v q = ld_f4(d + 4 * g);
+                dst[4 * g + 0] = __float_as_uint(q.x);
+                dst[4 * g + 1] = __float_as_uint(q.y);
+                dst[4 * g + 2] = __float_as_uint(q.z);
+                dst[4 * g + 3] = __float_as_uint(q.w);
+            }
         }
     };
-
-    // slot k serves rows k - base + j*P; fill it with its first row >= 0
+    auto init_of = [&](const uint32_t (&src)[S], int s) -> float {
+        if (xs + s >= w) return 0.0f;
+        if (!FWD) return __uint_as_float(src[s]);
+        return src[s] ? kInf : 0.0f;
+    };
+    // superstep j (steps 2Pj .. 2Pj+2P-1) processes rows Pj - base + k, k < P,
+    // in ring slot k; the rows of superstep j+1 are loaded at the start of
+    // superstep j and rotated in at its end, so each load has a whole
+    // superstep to arrive and no step waits on memory
+    auto fetch = [&](int j, uint32_t (&dst)[P][S]) {
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-        int r = k - base;
-        if (r < 0) r += ((-r + P - 1) / P) * P;
-        if (active && r < rows) load_row(r, ring[k]);
-    }
+        for (int k = 0; k < P; ++k) load_row(min(max(P * j + k - base, 0), rows - 1), dst[k]);
+    };
+    fetch(0, ring);
 
     const int tmax = 2 * (rows - 1) + nact - 1;
     for (int T0 = 0; T0 <= tmax; T0 += 2 * P) {
+        const int nsteps = min(2 * P, tmax - T0 + 1);  // uniform
+        fetch(T0 / (2 * P) + 1, next);
 #pragma unroll
         for (int s2 = 0; s2 < 2 * P; ++s2) {
-            if (T0 + s2 > tmax) break;  // uniform
+            if (s2 >= nsteps) continue;  // uniform: all threads skip the barrier together
             if (active && (s2 & 1) == par) {
                 const int k = s2 >> 1;
                 const int r = (T0 >> 1) + k - base;
@@ -142,7 +141,7 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
 #pragma unroll
                         for (int s = 0; s < S; ++s) {
                             const int x = xs + s;
-                            float b = ring[k][s];
+                            float b = init_of(ring[k], s);
                             if (x > 0) b = fminf(b, (s == 0 ? carryL : cur[s - 1]) + 1.0f);
                             if (hasPrev) {
                                 b = fminf(b, prev[s] + 1.0f);
@@ -158,7 +157,7 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
 #pragma unroll
                         for (int s = S - 1; s >= 0; --s) {
                             const int x = xs + s;
-                            float b = ring[k][s];
+                            float b = init_of(ring[k], s);
                             if (x < w - 1) b = fminf(b, (s == S - 1 ? carryR : cur[s + 1]) + 1.0f);
                             if (hasPrev) {
                                 b = fminf(b, prev[s] + 1.0f);
@@ -168,23 +167,25 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
                             cur[s] = (x < w) ? b : 0.0f;
                         }
                     }
-                    float* row = wk.dist + static_cast<size_t>(y) * w + xs;
-                    if (vec) {
-                        StripIO<S>::st(row, cur);
-                    } else {
+                    float* row = wk.dist + static_cast<size_t>(y) * dp + xs;
 #pragma unroll
-                        for (int s = 0; s < S; ++s)
-                            if (xs + s < w) row[s] = cur[s];
+                    for (int g = 0; g < S / 4; ++g) {
+                        f4v v;
+                        v.x = cur[4 * g]; v.y = cur[4 * g + 1]; v.z = cur[4 * g + 2]; v.w = cur[4 * g + 3];
+                        st_f4(row + 4 * g, v);
                     }
 #pragma unroll
                     for (int s = 0; s < S; ++s) prev[s] = cur[s];
                     firstv[cb][t] = cur[0];
                     lastv[cb][t] = cur[S - 1];
-                    if (r + P < rows) load_row(r + P, ring[k]);
                 }
             }
             lds_barrier();  // neighbours exchange through LDS only
         }
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) ring[k][s] = next[k][s];
     }
 }
 
@@ -208,8 +209,8 @@ inline hipError_t launch_chamfer(const ChamferWork* d_work, int n, int max_w, hi
         chamfer_kernel<8, 4, true><<<n, nt, 0, s>>>(d_work);
         chamfer_kernel<8, 4, false><<<n, nt, 0, s>>>(d_work);
     } else {
-        chamfer_kernel<16, 4, true><<<n, nt, 0, s>>>(d_work);
-        chamfer_kernel<16, 4, false><<<n, nt, 0, s>>>(d_work);
+        chamfer_kernel<16, 2, true><<<n, nt, 0, s>>>(d_work);
+        chamfer_kernel<16, 2, false><<<n, nt, 0, s>>>(d_work);
     }
     return hipGetLastError();
 }

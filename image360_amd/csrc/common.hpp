@@ -90,6 +90,20 @@ struct i360hip_ctx {
 };
 
 namespace i360 {
+// Global-address-space views of device pointers.  Pointers loaded from
+// tables are generic; accessing through them emits flat instructions,
+// which count against the LDS wait counter too and so turn every LDS wait
+// into a wait for all outstanding memory traffic.
+#define I360_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ I360_GLOBAL T* gptr(T* p) {
+    return (I360_GLOBAL T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ const I360_GLOBAL T* gptr(const T* p) {
+    return (const I360_GLOBAL T*)(p);
+}
+
 // Workgroup barrier that orders LDS only: waits for this wave's LDS
 // operations (lgkmcnt), not for its outstanding global loads and stores,
 // so prefetches and fire-and-forget stores stay in flight across it.  Use

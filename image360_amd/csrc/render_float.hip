@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(256) rowner_kernel(const RItem* __restrict__ i
             for (int k = 0; k < ch.count; ++k) {
                 const RItem& it = items[ch.first + k];
                 if (!in_box(it.box, x, y)) continue;
-                const float v = it.dist[p];
+                const float v = gptr(it.dist)[static_cast<size_t>(y) * DistPitch(ch.w) + x];
                 if (v > bd) { bd = v; best = k; }
             }
             ch.owner[p] = best;
@@ -806,7 +806,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             Pending pd{};
             pd.warped = take(n * 4 * sS);
             pd.cov = take(n);
-            pd.dist = take(n * 4);
+            pd.dist = take(static_cast<size_t>(DistPitch(cc.width)) * cc.height * 4);
             for (int l = 1; l < ch.levels; ++l) {
                 const size_t m = static_cast<size_t>(ch.lw[l]) * ch.lh[l];
                 pd.nd[l] = take(m * 4 * sS);

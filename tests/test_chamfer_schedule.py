@@ -34,14 +34,18 @@ def kernel(cov, dist, box, w, h, S, P, FWD, nt):
             if FWD: out[s] = kInf if (x < w and cov[y, x]) else f32(0)
             else: out[s] = dist[y, x] if x < w else f32(0)
         return out
+    def fetch(T, j):
+        # rows of superstep j, clamped into the box (unused slots hold clamped rows)
+        return np.stack([load_row(T, min(max(P * j + k - T['base'], 0), rows - 1)) for k in range(P)])
     for T in th:
-        for k in range(P):
-            r = k - T['base']
-            if r < 0: r += ((-r + P - 1) // P) * P
-            if T['active'] and r < rows: T['ring'][k] = load_row(T, r)
+        if T['active']:
+            T['ring'] = fetch(T, 0)
     tmax = 2 * (rows - 1) + nact - 1
     T0 = 0
     while T0 <= tmax:
+        for T in th:
+            if T['active']:
+                T['next'] = fetch(T, T0 // (2 * P) + 1)
         for s2 in range(2 * P):
             if T0 + s2 > tmax: break
             # all threads "in parallel": reads of LDS happen before writes of this step
@@ -87,7 +91,9 @@ def kernel(cov, dist, box, w, h, S, P, FWD, nt):
                     if xs + s < w: dist[y, xs + s] = cur[s]
                 T['prev'] = cur.copy()
                 firstv[cb][t] = cur[0]; lastv[cb][t] = cur[S - 1]
-                if r + P < rows: ring[k] = load_row(T, r + P)
+        for T in th:
+            if T['active']:
+                T['ring'] = T['next']
         T0 += 2 * P
 
 
