@@ -41,41 +41,50 @@ template <> struct Form<kModePlain> { using M = float; using S = float; };
 template <> struct Form<kModePrecision> { using M = float; using S = _Float16; };
 
 __device__ __forceinline__ void ld4(const void* p, size_t i, float (&o)[4], float*) {
-    const float4 q = static_cast<const float4*>(p)[i];
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v q = gptr(static_cast<const f4v*>(p))[i];
     o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
 }
 __device__ __forceinline__ void ld4(const void* p, size_t i, float (&o)[4], _Float16*) {
-    const h4 q = static_cast<const h4*>(p)[i];
+    const h4 q = gptr(static_cast<const h4*>(p))[i];
     o[0] = static_cast<float>(q.x); o[1] = static_cast<float>(q.y);
     o[2] = static_cast<float>(q.z); o[3] = static_cast<float>(q.w);
 }
 __device__ __forceinline__ void ld4(const void* p, size_t i, double (&o)[4], double*) {
-    const double4 q = static_cast<const double4*>(p)[i];
+    typedef double d4v __attribute__((ext_vector_type(4)));
+    const d4v q = gptr(static_cast<const d4v*>(p))[i];
     o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
 }
 __device__ __forceinline__ void st4(void* p, size_t i, const float (&v)[4], float*) {
-    static_cast<float4*>(p)[i] = make_float4(v[0], v[1], v[2], v[3]);
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    f4v q;
+    q.x = v[0]; q.y = v[1]; q.z = v[2]; q.w = v[3];
+    gptr(static_cast<f4v*>(p))[i] = q;
 }
 __device__ __forceinline__ void st4(void* p, size_t i, const float (&v)[4], _Float16*) {
     h4 q;
     q.x = static_cast<_Float16>(v[0]); q.y = static_cast<_Float16>(v[1]);
     q.z = static_cast<_Float16>(v[2]); q.w = static_cast<_Float16>(v[3]);
-    static_cast<h4*>(p)[i] = q;
+    gptr(static_cast<h4*>(p))[i] = q;
 }
 __device__ __forceinline__ void st4(void* p, size_t i, const double (&v)[4], double*) {
-    static_cast<double4*>(p)[i] = make_double4(v[0], v[1], v[2], v[3]);
+    typedef double d4v __attribute__((ext_vector_type(4)));
+    d4v q;
+    q.x = v[0]; q.y = v[1]; q.z = v[2]; q.w = v[3];
+    gptr(static_cast<d4v*>(p))[i] = q;
 }
 template <typename M, typename S>
 __device__ __forceinline__ M ld1(const void* p, size_t i) {
-    return static_cast<M>(static_cast<const S*>(p)[i]);
+    return static_cast<M>(gptr(static_cast<const S*>(p))[i]);
 }
 template <typename S, typename M>
 __device__ __forceinline__ void st1(void* p, size_t i, M v) {
-    static_cast<S*>(p)[i] = static_cast<S>(v);
+    gptr(static_cast<S*>(p))[i] = static_cast<S>(v);
 }
 
 __device__ __forceinline__ bool in_box(const int* b, int x, int y) {
-    return x >= b[0] && x <= b[2] && y >= b[1] && y <= b[3];
+    const auto* g = gptr(b);
+    return x >= g[0] && x <= g[2] && y >= g[1] && y <= g[3];
 }
 
 // ----------------------------------------------------------- FP64 map ---
@@ -106,12 +115,12 @@ constexpr int kWarpTW = 64, kWarpTH = 16;
 template <typename M>
 __device__ __forceinline__ void fetch_rgb(const void* src, int fmt, size_t idx, M (&o)[3]) {
     if (fmt == I360HIP_PIX_F16) {
-        const _Float16* p = static_cast<const _Float16*>(src) + idx * 3;
+        const auto* p = gptr(static_cast<const _Float16*>(src) + idx * 3);
         o[0] = static_cast<M>(static_cast<float>(p[0]));
         o[1] = static_cast<M>(static_cast<float>(p[1]));
         o[2] = static_cast<M>(static_cast<float>(p[2]));
     } else {
-        const float* p = static_cast<const float*>(src) + idx * 3;
+        const auto* p = gptr(static_cast<const float*>(src) + idx * 3);
         o[0] = static_cast<M>(p[0]); o[1] = static_cast<M>(p[1]); o[2] = static_cast<M>(p[2]);
     }
 }
@@ -282,7 +291,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                 bx0 = min(bx0, x); by0 = min(by0, y); bx1 = max(bx1, x); by1 = max(by1, y);
             }
             st4(it.warped, p, v, static_cast<S*>(nullptr));
-            it.cov[p] = ok ? 1 : 0;
+            gptr(it.cov)[p] = ok ? 1 : 0;
         }
         if (bx1 >= 0) {
             atomicMin(&sbox[0], bx0); atomicMin(&sbox[1], by0); atomicMax(&sbox[2], bx1); atomicMax(&sbox[3], by1);
@@ -354,7 +363,7 @@ __global__ void __launch_bounds__(256) rowner_kernel(const RItem* __restrict__ i
                 const float v = gptr(it.dist)[static_cast<size_t>(y) * DistPitch(ch.w) + x];
                 if (v > bd) { bd = v; best = k; }
             }
-            ch.owner[p] = best;
+            gptr(ch.owner)[p] = best;
         }
     }
 }
@@ -462,7 +471,7 @@ __global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ 
                         v[k][1] = q[1] * q[3];
                         v[k][2] = q[2] * q[3];
                         v[k][3] = q[3];
-                        v[k][4] = ch.owner[p] == it.local ? M(1) : M(0);
+                        v[k][4] = gptr(ch.owner)[p] == it.local ? M(1) : M(0);
                     } else {
                         ld4(it.nd[l], p, q, static_cast<S*>(nullptr));
                         v[k][0] = q[0]; v[k][1] = q[1]; v[k][2] = q[2]; v[k][3] = q[3];
@@ -621,8 +630,8 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
             const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
             if (x >= ch.w || y >= ch.h) continue;
             const size_t p = static_cast<size_t>(y) * ch.w + x;
-            float* dst = ch.out + p * 3;
-            const int o = ch.owner[p];
+            auto* dst = gptr(ch.out + p * 3);
+            const int o = gptr(ch.owner)[p];
             if (o < 0) { dst[0] = 0.0f; dst[1] = 0.0f; dst[2] = 0.0f; continue; }
             const RItem& it = items[ch.first + o];
             M q[4];
@@ -651,7 +660,7 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                 const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
                 if (x >= ch.w || y >= ch.h) continue;
                 const size_t p = static_cast<size_t>(y) * ch.w + x;
-                ch.owner_out[p] = ch.owner[p];
+                gptr(ch.owner_out)[p] = gptr(ch.owner)[p];
             }
         }
     }

@@ -35,9 +35,10 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 struct U16Img {
     const uint16_t* p;
     __device__ __forceinline__ void rgb(size_t i, float (&o)[3]) const {
-        o[0] = static_cast<float>(p[i * 3]);
-        o[1] = static_cast<float>(p[i * 3 + 1]);
-        o[2] = static_cast<float>(p[i * 3 + 2]);
+        const auto* g = gptr(p);
+        o[0] = static_cast<float>(g[i * 3]);
+        o[1] = static_cast<float>(g[i * 3 + 1]);
+        o[2] = static_cast<float>(g[i * 3 + 2]);
     }
     __device__ static constexpr float smooth() { return 2.0f; }
 };
@@ -78,7 +79,7 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
             int first = -1, cross = -1;
             for (int base = 0; base < len && cross < 0; base += 64) {
                 const int t = base + lane;
-                const int o = (t < len) ? owner[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
+                const int o = (t < len) ? gptr(owner)[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
                 const bool ij = (t < len) && (o == ci || o == cj);
                 const unsigned long long m = __ballot(ij);
                 int start = 0;
@@ -99,7 +100,7 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
                 bool both = false;
                 if (t <= b) {
                     const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
-                    both = covI[p] && covJ[p];
+                    both = gptr(covI)[p] && gptr(covJ)[p];
                 }
                 const unsigned long long m = __ballot(both);
                 if (m) {
@@ -107,7 +108,7 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
                     hi = a + 63 - __clzll(static_cast<long long>(m));
                 }
             }
-            if (lane == 0) { lohi[2 * line] = lo; lohi[2 * line + 1] = hi; chosen[line] = -1; }
+            if (lane == 0) { gptr(lohi)[2 * line] = lo; gptr(lohi)[2 * line + 1] = hi; gptr(chosen)[line] = -1; }
         }
     } else {
         // lines are columns: lanes take adjacent columns and walk down them
@@ -123,7 +124,7 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int t = t0 + u;
-                    ob[u] = (valid && cross < 0 && t < len) ? owner[seam_at(false, ox0, oy0, w, line, t)] : -2;
+                    ob[u] = (valid && cross < 0 && t < len) ? gptr(owner)[seam_at(false, ox0, oy0, w, line, t)] : -2;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -138,13 +139,13 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
                 const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
                 for (int t = a; t <= b; ++t) {
                     const size_t p = seam_at(false, ox0, oy0, w, line, t);
-                    if (covI[p] && covJ[p]) {
+                    if (gptr(covI)[p] && gptr(covJ)[p]) {
                         if (hi < lo) { lo = t; hi = t; }
                         else { hi = t; }
                     }
                 }
             }
-            if (valid) { lohi[2 * line] = lo; lohi[2 * line + 1] = hi; chosen[line] = -1; }
+            if (valid) { gptr(lohi)[2 * line] = lo; gptr(lohi)[2 * line + 1] = hi; gptr(chosen)[line] = -1; }
         }
     }
     __syncthreads();
@@ -161,7 +162,7 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
         const int nb = min(kCostBatch, lines - b0);
         for (int idx = threadIdx.x; idx < nb * 64; idx += blockDim.x) {
             const int l = idx >> 6, k = idx & 63;
-            const int lo = lohi[2 * (b0 + l)], hi = lohi[2 * (b0 + l) + 1];
+            const int lo = gptr(lohi)[2 * (b0 + l)], hi = gptr(lohi)[2 * (b0 + l) + 1];
             if (k == 0) { slo[l] = lo; shi[l] = hi; }
             if (k <= hi - lo) {
                 const size_t p = seam_at(rowScan, ox0, oy0, w, b0 + l, lo + k);
@@ -195,8 +196,8 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
                     }
                     if (lane < lw) v = costs[l][lane] + best;
                 }
-                if (lane < lw) back[static_cast<size_t>(line) * 64 + lane] = static_cast<uint8_t>(bp & 0xFF);
-                if (lane == 0) prevl[line] = last;
+                if (lane < lw) gptr(back)[static_cast<size_t>(line) * 64 + lane] = static_cast<uint8_t>(bp & 0xFF);
+                if (lane == 0) gptr(prevl)[line] = last;
                 dpv = v;
                 last = line;
                 lastLo = lo;
@@ -229,16 +230,16 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
     for (int b1 = last + 1; b1 > 0; b1 -= kBackBatch) {
         const int b0 = max(0, b1 - kBackBatch);
         for (int idx = threadIdx.x; idx < (b1 - b0) * 64; idx += blockDim.x)
-            sback[idx >> 6][idx & 63] = back[static_cast<size_t>(b0 + (idx >> 6)) * 64 + (idx & 63)];
+            sback[idx >> 6][idx & 63] = gptr(back)[static_cast<size_t>(b0 + (idx >> 6)) * 64 + (idx & 63)];
         for (int idx = threadIdx.x; idx < b1 - b0; idx += blockDim.x) {
-            sprev[idx] = prevl[b0 + idx];
-            slo2[idx] = lohi[2 * (b0 + idx)];
+            sprev[idx] = gptr(prevl)[b0 + idx];
+            slo2[idx] = gptr(lohi)[2 * (b0 + idx)];
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             int line = s_line, k = s_k;
             while (line >= b0) {
-                chosen[line] = slo2[line - b0] + k;
+                gptr(chosen)[line] = slo2[line - b0] + k;
                 const int pk = static_cast<int>(sback[line - b0][k]);
                 line = sprev[line - b0];
                 k = pk;
@@ -253,14 +254,14 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
 
     // Phase 4: reassign I/J-owned pixels inside each contested band.
     for (int line = wave; line < lines; line += nwaves) {
-        const int ch = chosen[line];
+        const int ch = gptr(chosen)[line];
         if (ch < 0) continue;
-        const int lo = lohi[2 * line], hi = lohi[2 * line + 1];
+        const int lo = gptr(lohi)[2 * line], hi = gptr(lohi)[2 * line + 1];
         for (int t = lo + lane; t <= hi; t += 64) {
             const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
-            const int o = owner[p];
+            const int o = gptr(owner)[p];
             if (o != ci && o != cj) continue;
-            owner[p] = ((t < ch) == iLow) ? ci : cj;
+            gptr(owner)[p] = ((t < ch) == iLow) ? ci : cj;
         }
     }
 }
