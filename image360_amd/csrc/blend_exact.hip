@@ -448,11 +448,16 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     BL_LAUNCH(st, "ownership", 4.0 * n * (c + 1), ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, boxes, c, w, n, owner));
     // 4. pairwise seams in the reference's order
     for (int i = 0; i < c; ++i)
-        for (int j = i + 1; j < c; ++j)
+        for (int j = i + 1; j < c; ++j) {
+            int* lohi = reinterpret_cast<int*>(base + oLohi);
+            int* chosen = reinterpret_cast<int*>(base + oChosen);
+            BL_LAUNCH(st, "seam_scan", 0,
+                      seam_scan_kernel<<<(maxLines + kScanLinesPerBlock - 1) / kScanLinesPerBlock, 256, 0, s>>>(
+                          cov + n * i, cov + n * j, boxes, i, j, w, owner, lohi, chosen));
             BL_LAUNCH(st, "seam_dp", 0, seam_dp_kernel<U16Img, float><<<1, 1024, 0, s>>>(
                               U16Img{warped[i]}, U16Img{warped[j]}, cov + n * i, cov + n * j, boxes, i, j, w, owner,
-                              reinterpret_cast<int*>(base + oLohi), reinterpret_cast<int*>(base + oPrev),
-                              reinterpret_cast<int*>(base + oChosen), base + oBack));
+                              lohi, reinterpret_cast<int*>(base + oPrev), chosen, base + oBack));
+        }
     if (owner_out) BL_LAUNCH(st, "copy_owner", 8.0 * n, copy_owner_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner, owner_out, n));
     // 5. pyramids of every contributor (contributors without owned pixels
     // carry all-zero weights and drop out of the sums exactly)

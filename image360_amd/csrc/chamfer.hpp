@@ -21,6 +21,8 @@
 // per-step critical path holds no global-memory latency.
 #pragma once
 
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace i360 {
@@ -195,7 +197,16 @@ inline int ChamferThreads(int w, int S) {
     return ((cols + 127) / 128) * 128;  // even and odd strips in separate wavefronts
 }
 
-inline int ChamferStrip(int w) { return w <= 4096 ? 4 : (w <= 8192 ? 8 : 16); }
+// Strip width: the narrowest of 4 / 8 / 16 columns that fits the row in
+// one block, unless I360HIP_CHAMFER_STRIP (4, 8 or 16) asks for a wider one.
+inline int ChamferStrip(int w) {
+    int s = w <= 4096 ? 4 : (w <= 8192 ? 8 : 16);
+    if (const char* e = std::getenv("I360HIP_CHAMFER_STRIP")) {
+        const int v = std::atoi(e);
+        if ((v == 4 || v == 8 || v == 16) && v >= s) s = v;
+    }
+    return s;
+}
 
 // Launches forward then backward passes over `n` work items (device table).
 inline hipError_t launch_chamfer(const ChamferWork* d_work, int n, int max_w, hipStream_t s) {

@@ -1072,12 +1072,19 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
                 ++cnt;
             }
             if (cnt == 0) continue;
-            if (MM == kModeFP64)
+            int maxLines = 1;
+            for (int k2 = c0; k2 < c1; ++k2) maxLines = std::max({maxLines, P.chunks[k2].w, P.chunks[k2].h});
+            const dim3 sg((maxLines + kScanLinesPerBlock - 1) / kScanLinesPerBlock, cnt);
+            if (MM == kModeFP64) {
+                seam_scan_table_kernel<RgbaImg<double, double>><<<sg, 256, 0, s>>>(T.seam64 + first);
                 seam_dp_table_kernel<RgbaImg<double, double>, double><<<cnt, 1024, 0, s>>>(T.seam64 + first);
-            else if (MM == kModePlain)
+            } else if (MM == kModePlain) {
+                seam_scan_table_kernel<RgbaImg<float, float>><<<sg, 256, 0, s>>>(T.seam32 + first);
                 seam_dp_table_kernel<RgbaImg<float, float>, float><<<cnt, 1024, 0, s>>>(T.seam32 + first);
-            else
+            } else {
+                seam_scan_table_kernel<MapImg><<<sg, 256, 0, s>>>(T.seamMap + first);
                 seam_dp_table_kernel<MapImg, double><<<cnt, 1024, 0, s>>>(T.seamMap + first);
+            }
             if (hipGetLastError() != hipSuccess) {
                 st.set(I360HIP_KERNEL_LAUNCH_FAILED, "seam launch failed");
                 return;

@@ -51,6 +51,93 @@ __device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w,
     return rowScan ? static_cast<size_t>(oy0 + line) * w + (ox0 + t) : static_cast<size_t>(oy0 + t) * w + (ox0 + line);
 }
 
+
+// Phase 1 of a pair: for every line, the crossover of the current owner map
+// between the pair and the contested band around it (SeamBlend.cpp:325-360).
+// One wavefront per line, lanes along the line, four 64-pixel segments in
+// flight per lane; blocks cover lines, so all lines of a pair (and all
+// pairs of a round) scan in parallel.
+template <int U = 4>
+__device__ __forceinline__ void seam_scan_body(const uint8_t* __restrict__ covI, const uint8_t* __restrict__ covJ,
+                                               const int* __restrict__ boxes, int ci, int cj, int w,
+                                               const int32_t* __restrict__ owner, int* __restrict__ lohi,
+                                               int* __restrict__ chosen, int line) {
+    const int* bI = boxes + 4 * ci;
+    const int* bJ = boxes + 4 * cj;
+    const auto* gbI = gptr(bI);
+    const auto* gbJ = gptr(bJ);
+    if (gbI[2] < 0 || gbJ[2] < 0) return;
+    const int ox0 = max(gbI[0], gbJ[0]), oy0 = max(gbI[1], gbJ[1]);
+    const int ox1 = min(gbI[2], gbJ[2]), oy1 = min(gbI[3], gbJ[3]);
+    if (ox0 > ox1 || oy0 > oy1) return;
+    const int bw = ox1 - ox0 + 1, bh = oy1 - oy0 + 1;
+    const bool rowScan = bw >= bh;
+    const int lines = rowScan ? bh : bw, len = rowScan ? bw : bh;
+    if (line >= lines) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const auto* gown = gptr(owner);
+    int first = -1, cross = -1;
+    for (int b0 = 0; b0 < len && cross < 0; b0 += 64 * U) {
+        int ob[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = b0 + 64 * u + lane;
+            ob[u] = (t < len) ? gown[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (cross >= 0) break;
+            const int base = b0 + 64 * u;
+            const int o = ob[u];
+            const bool ij = (o == ci || o == cj);
+            const unsigned long long m = __ballot(ij);
+            int start = 0;
+            if (first < 0) {
+                if (m == 0ull) continue;
+                const int f = __ffsll(static_cast<long long>(m)) - 1;
+                first = __shfl(o, f);
+                start = f + 1;
+            }
+            unsigned long long m2 = __ballot(ij && o != first);
+            m2 = (start >= 64) ? 0ull : (m2 & (~0ull << start));
+            if (m2) cross = base + __ffsll(static_cast<long long>(m2)) - 1;
+        }
+    }
+    int lo = 1, hi = 0;
+    if (cross >= 0) {
+        const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
+        const int t = a + lane;
+        bool both = false;
+        if (t <= b) {
+            const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
+            both = gptr(covI)[p] && gptr(covJ)[p];
+        }
+        const unsigned long long m = __ballot(both);
+        if (m) {
+            lo = a + __ffsll(static_cast<long long>(m)) - 1;
+            hi = a + 63 - __clzll(static_cast<long long>(m));
+        }
+    }
+    if (lane == 0) {
+        gptr(lohi)[2 * line] = lo;
+        gptr(lohi)[2 * line + 1] = hi;
+        gptr(chosen)[line] = -1;
+    }
+}
+
+// Lines per block of the scan kernel (one wavefront each).
+constexpr int kScanLinesPerBlock = 4;
+
+template <int Dummy = 0>
+__global__ void __launch_bounds__(256) seam_scan_kernel(const uint8_t* __restrict__ covI,
+                                                        const uint8_t* __restrict__ covJ,
+                                                        const int* __restrict__ boxes, int ci, int cj, int w,
+                                                        const int32_t* __restrict__ owner, int* __restrict__ lohi,
+                                                        int* __restrict__ chosen) {
+    const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
+    seam_scan_body(covI, covJ, boxes, ci, cj, w, owner, lohi, chosen, line);
+}
+
 template <class Img, typename M>
 __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
                                                        const uint8_t* __restrict__ covI,
@@ -72,82 +159,7 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
     const int lines = rowScan ? bh : bw, len = rowScan ? bw : bh;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
 
-    // Phase 1: crossover of the current owner map and contested band per line.
-    if (rowScan) {
-        // lines are rows: one wavefront per line, lanes along the line
-        for (int line = wave; line < lines; line += nwaves) {
-            int first = -1, cross = -1;
-            for (int base = 0; base < len && cross < 0; base += 64) {
-                const int t = base + lane;
-                const int o = (t < len) ? gptr(owner)[seam_at(rowScan, ox0, oy0, w, line, t)] : -2;
-                const bool ij = (t < len) && (o == ci || o == cj);
-                const unsigned long long m = __ballot(ij);
-                int start = 0;
-                if (first < 0) {
-                    if (m == 0ull) continue;
-                    const int f = __ffsll(static_cast<long long>(m)) - 1;
-                    first = __shfl(o, f);
-                    start = f + 1;
-                }
-                unsigned long long m2 = __ballot(ij && o != first);
-                m2 = (start >= 64) ? 0ull : (m2 & (~0ull << start));
-                if (m2) cross = base + __ffsll(static_cast<long long>(m2)) - 1;
-            }
-            int lo = 1, hi = 0;
-            if (cross >= 0) {
-                const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
-                const int t = a + lane;
-                bool both = false;
-                if (t <= b) {
-                    const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
-                    both = gptr(covI)[p] && gptr(covJ)[p];
-                }
-                const unsigned long long m = __ballot(both);
-                if (m) {
-                    lo = a + __ffsll(static_cast<long long>(m)) - 1;
-                    hi = a + 63 - __clzll(static_cast<long long>(m));
-                }
-            }
-            if (lane == 0) { gptr(lohi)[2 * line] = lo; gptr(lohi)[2 * line + 1] = hi; gptr(chosen)[line] = -1; }
-        }
-    } else {
-        // lines are columns: lanes take adjacent columns and walk down them
-        // together, so every owner load of the wavefront is one row segment
-        constexpr int U = 16;
-        for (int l0 = wave * 64; l0 < lines; l0 += nwaves * 64) {
-            const int line = l0 + lane;
-            const bool valid = line < lines;
-            int first = -1, cross = -1;
-            for (int t0 = 0; t0 < len; t0 += U) {
-                if (!__any(valid && cross < 0)) break;
-                int ob[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int t = t0 + u;
-                    ob[u] = (valid && cross < 0 && t < len) ? gptr(owner)[seam_at(false, ox0, oy0, w, line, t)] : -2;
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int o = ob[u];
-                    if (cross >= 0 || !(o == ci || o == cj)) continue;
-                    if (first < 0) first = o;
-                    else if (o != first) cross = t0 + u;
-                }
-            }
-            int lo = 1, hi = 0;
-            if (valid && cross >= 0) {
-                const int a = max(0, cross - kSeamRadius), b = min(len - 1, cross + kSeamRadius);
-                for (int t = a; t <= b; ++t) {
-                    const size_t p = seam_at(false, ox0, oy0, w, line, t);
-                    if (gptr(covI)[p] && gptr(covJ)[p]) {
-                        if (hi < lo) { lo = t; hi = t; }
-                        else { hi = t; }
-                    }
-                }
-            }
-            if (valid) { gptr(lohi)[2 * line] = lo; gptr(lohi)[2 * line + 1] = hi; gptr(chosen)[line] = -1; }
-        }
-    }
+    // Phase 1 (crossover and band of every line) ran in seam_scan_kernel.
     __syncthreads();
 
     // Phase 2: DP over contested lines (SeamBlend.cpp:362-407).
@@ -291,6 +303,13 @@ struct SeamWork {
     int* chosen;
     uint8_t* back;
 };
+
+template <class Img>
+__global__ void __launch_bounds__(256) seam_scan_table_kernel(const SeamWork<Img>* __restrict__ work) {
+    const SeamWork<Img> wk = work[blockIdx.y];
+    const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
+    seam_scan_body(wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi, wk.chosen, line);
+}
 
 template <class Img, typename M>
 __global__ void __launch_bounds__(1024) seam_dp_table_kernel(const SeamWork<Img>* __restrict__ work) {
