@@ -410,6 +410,7 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     const size_t oLohi = take(8 * static_cast<size_t>(maxLines)), oPrev = take(4 * static_cast<size_t>(maxLines));
     const size_t oChosen = take(4 * static_cast<size_t>(maxLines)), oBack = take(64 * static_cast<size_t>(maxLines));
     const size_t oWork = take(sizeof(ChamferWork) * I360HIP_MAX_CONTRIBUTORS);
+    const size_t oCnt = take(sizeof(int) * I360HIP_MAX_CONTRIBUTORS * I360HIP_MAX_CONTRIBUTORS);
     size_t oNd[I360HIP_MAX_BANDS][I360HIP_MAX_CONTRIBUTORS] = {}, oWt[I360HIP_MAX_BANDS][I360HIP_MAX_CONTRIBUTORS] = {};
     size_t oRec[I360HIP_MAX_BANDS] = {};
     for (int l = 1; l < L; ++l) {
@@ -447,16 +448,19 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     // 3. ownership
     BL_LAUNCH(st, "ownership", 4.0 * n * (c + 1), ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, boxes, c, w, n, owner));
     // 4. pairwise seams in the reference's order
+    int* counters = reinterpret_cast<int*>(base + oCnt);
+    BL_LAUNCH(st, "fill_i32", 0, fill_i32_kernel<<<4, 256, 0, s>>>(counters, size_t(c) * c, 0));
     for (int i = 0; i < c; ++i)
         for (int j = i + 1; j < c; ++j) {
             int* lohi = reinterpret_cast<int*>(base + oLohi);
             int* chosen = reinterpret_cast<int*>(base + oChosen);
+            int* cnt = counters + i * c + j;
             BL_LAUNCH(st, "seam_scan", 0,
                       seam_scan_kernel<<<(maxLines + kScanLinesPerBlock - 1) / kScanLinesPerBlock, 256, 0, s>>>(
-                          cov + n * i, cov + n * j, boxes, i, j, w, owner, lohi, chosen));
+                          cov + n * i, cov + n * j, boxes, i, j, w, owner, lohi, chosen, cnt));
             BL_LAUNCH(st, "seam_dp", 0, seam_dp_kernel<U16Img, float><<<1, 1024, 0, s>>>(
                               U16Img{warped[i]}, U16Img{warped[j]}, cov + n * i, cov + n * j, boxes, i, j, w, owner,
-                              lohi, reinterpret_cast<int*>(base + oPrev), chosen, base + oBack));
+                              lohi, reinterpret_cast<int*>(base + oPrev), chosen, base + oBack, cnt));
         }
     if (owner_out) BL_LAUNCH(st, "copy_owner", 8.0 * n, copy_owner_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner, owner_out, n));
     // 5. pyramids of every contributor (contributors without owned pixels

@@ -197,13 +197,15 @@ inline int ChamferThreads(int w, int S) {
     return ((cols + 127) / 128) * 128;  // even and odd strips in separate wavefronts
 }
 
-// Strip width: the narrowest of 4 / 8 / 16 columns that fits the row in
-// one block, unless I360HIP_CHAMFER_STRIP (4, 8 or 16) asks for a wider one.
+// Strip width: 8 columns (measured fastest on 4096-wide chunks: fewer
+// steps than 4 columns, shorter per-step chains than 16), 16 where a row
+// would not fit one 1024-thread block.  I360HIP_CHAMFER_STRIP (4, 8 or 16)
+// overrides it with any width that still fits.
 inline int ChamferStrip(int w) {
-    int s = w <= 4096 ? 4 : (w <= 8192 ? 8 : 16);
+    int s = w <= 8192 ? 8 : 16;
     if (const char* e = std::getenv("I360HIP_CHAMFER_STRIP")) {
         const int v = std::atoi(e);
-        if ((v == 4 || v == 8 || v == 16) && v >= s) s = v;
+        if ((v == 4 || v == 8 || v == 16) && w <= 1024 * v) s = v;
     }
     return s;
 }

@@ -51,6 +51,7 @@ struct RChunk {
     int* prevl;
     int* chosen;
     uint8_t* back;
+    int* counters;  // per-pair contested-line counts (count*count ints)
 };
 
 // A tile of some stage: id = item or chunk index, (tx, ty) tile coordinates.

@@ -102,11 +102,15 @@ __global__ void rmap_kernel(RItem* items, int n) {
     o[6] = c2 * s; o[7] = (b * g - a * k) * s; o[8] = (a * e - b * d) * s;
 }
 
-__global__ void rbox_init_kernel(RItem* items, int n) {
+// Empty coverage boxes, and cleared contested-line counters of the pairs
+// this item opens (row `local` of its chunk's counter matrix).
+__global__ void rbox_init_kernel(RItem* items, const RChunk* chunks, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int* b = items[i].box;
     b[0] = INT_MAX; b[1] = INT_MAX; b[2] = -1; b[3] = -1;
+    const RChunk& ch = chunks[items[i].chunk];
+    for (int j = 0; j < ch.count; ++j) ch.counters[items[i].local * ch.count + j] = 0;
 }
 
 // --------------------------------------------------------------- warp ---
@@ -667,6 +671,12 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
 }
 
 // --------------------------------------------------------------- plan ---
+// One seam pair of a chunk: contributors i < j (chunk-local), its round
+// and its scratch slot within that round.
+struct SeamPair {
+    int i, j, round, slot;
+};
+
 struct Plan {
     std::vector<RItem> items;
     std::vector<RChunk> chunks;
@@ -677,7 +687,7 @@ struct Plan {
     std::vector<Range> warp, own, fin;
     std::vector<std::vector<Range>> red;   // [chunk][l] tiles of level l+1
     std::vector<std::vector<Range>> comp;  // [chunk][l] tiles of level l
-    std::vector<std::vector<std::pair<int, int>>> pairs;  // [chunk] item-local pairs
+    std::vector<std::vector<SeamPair>> pairs;  // [chunk] item-local pairs, reference order
     size_t bytes = 0;
     int max_w = 0;
     int mode = 0;
@@ -685,6 +695,30 @@ struct Plan {
 };
 
 inline size_t A256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// Groups a chunk's seam pairs into rounds.  A pair (i, j) reads only which
+// of i / j owns a pixel (owners outside {i, j} count as "neither") and
+// rewrites only pixels owned by i or j, to i or j (SeamBlend.cpp:336-343,
+// 437-442); its bands, costs and sides come from coverage, images and
+// boxes, which no pair changes.  Two pairs with no contributor in common
+// therefore commute, and any order that keeps the reference order among
+// pairs sharing a contributor gives the reference's owner map.  A pair's
+// round is one past the latest round of an earlier pair it shares a
+// contributor with; pairs of one round run side by side, each in its own
+// scratch slot.  Returns the number of slots (largest round).
+int schedule_pairs(std::vector<SeamPair>& pr) {
+    std::vector<int> last(I360HIP_MAX_CONTRIBUTORS, -1);  // latest round touching each contributor
+    std::vector<int> fill;
+    int slots = 0;
+    for (SeamPair& p : pr) {
+        p.round = std::max(last[p.i], last[p.j]) + 1;
+        last[p.i] = last[p.j] = p.round;
+        if (static_cast<int>(fill.size()) <= p.round) fill.resize(p.round + 1, 0);
+        p.slot = fill[p.round]++;
+        slots = std::max(slots, fill[p.round]);
+    }
+    return slots;
+}
 
 void dilate(const int (&in)[4], int ow, int oh, int (&out)[4]) {
     // support of [2x-2, 2x+2] intersecting [in0, in1]
@@ -730,7 +764,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
     struct Pending { size_t warped, cov, dist, nd[kMaxLevels], wt[kMaxLevels]; };
-    struct PendingC { size_t boxes, owner, recon[kMaxLevels], lohi, prevl, chosen, back; };
+    struct PendingC { size_t boxes, owner, recon[kMaxLevels], lohi, prevl, chosen, back, counters; };
     std::vector<Pending> pit;
     std::vector<PendingC> pch;
     // the per-chunk forms reuse one chunk's worth of buffers
@@ -830,21 +864,23 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         pc.boxes = take(16 * static_cast<size_t>(std::max(1, ch.count)));
         pc.owner = take(n * 4);
         for (int l = 1; l < ch.levels; ++l) pc.recon[l] = take(static_cast<size_t>(ch.lw[l]) * ch.lh[l] * 4 * sM);
-        const size_t lines = static_cast<size_t>(std::max(cc.width, cc.height));
-        pc.lohi = take(8 * lines); pc.prevl = take(4 * lines); pc.chosen = take(4 * lines); pc.back = take(64 * lines);
-        pch.push_back(pc);
-        P.chunks.push_back(ch);
         // pairs in the reference's order (SeamBlend.cpp:573-584); pairs whose
         // footprints are disjoint have disjoint coverage boxes, which the
         // reference skips, so they are left out here
-        std::vector<std::pair<int, int>> pr;
+        std::vector<SeamPair> pr;
         for (int i = 0; i < ch.count; ++i)
             for (int j = i + 1; j < ch.count; ++j) {
                 const int* a = P.items[ch.first + i].lbox[0];
                 const int* b = P.items[ch.first + j].lbox[0];
                 if (std::max(a[0], b[0]) <= std::min(a[2], b[2]) && std::max(a[1], b[1]) <= std::min(a[3], b[3]))
-                    pr.push_back({i, j});
+                    pr.push_back({i, j, 0, 0});
             }
+        const int nslots = schedule_pairs(pr);
+        const size_t lines = static_cast<size_t>(std::max(cc.width, cc.height)) * std::max(1, nslots);
+        pc.lohi = take(8 * lines); pc.prevl = take(4 * lines); pc.chosen = take(4 * lines); pc.back = take(64 * lines);
+        pc.counters = take(4 * static_cast<size_t>(std::max(1, ch.count)) * std::max(1, ch.count));
+        pch.push_back(pc);
+        P.chunks.push_back(ch);
         P.pairs.push_back(pr);
         P.max_w = std::max(P.max_w, cc.width);
         if (!batched) slot_max = std::max(slot_max, off);
@@ -873,6 +909,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         ch.prevl = reinterpret_cast<int*>(pch[c].prevl);
         ch.chosen = reinterpret_cast<int*>(pch[c].chosen);
         ch.back = reinterpret_cast<uint8_t*>(pch[c].back);
+        ch.counters = reinterpret_cast<int*>(pch[c].counters);
     }
     // tiles, one stage after another so that each stage's tiles are
     // contiguous across chunks (batched launches) and per chunk (the
@@ -935,7 +972,7 @@ static void patch_plan(Plan& P, uint8_t* base) {
         for (int l = 1; l < kMaxLevels; ++l) { fix(it.nd[l]); fix(it.wt[l]); }
     }
     for (auto& ch : P.chunks) {
-        fix(ch.boxes); fix(ch.owner); fix(ch.lohi); fix(ch.prevl); fix(ch.chosen); fix(ch.back);
+        fix(ch.boxes); fix(ch.owner); fix(ch.lohi); fix(ch.prevl); fix(ch.chosen); fix(ch.back); fix(ch.counters);
         for (int l = 1; l < kMaxLevels; ++l) fix(ch.recon[l]);
         for (int k = 0; k < ch.count; ++k) P.items[ch.first + k].box = ch.boxes + 4 * k;
     }
@@ -983,25 +1020,32 @@ MapImg make_img<MapImg>(const RItem& it, const RChunk& ch) {
 template <typename Img>
 static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables& T) {
     size_t rounds = 0;
-    for (auto& pr : P.pairs) rounds = std::max(rounds, pr.size());
+    for (auto& pr : P.pairs)
+        for (const SeamPair& p : pr) rounds = std::max(rounds, static_cast<size_t>(p.round) + 1);
     T.round_chunks.assign(rounds, {});
     T.round_off.assign(rounds, 0);
-    // round-major for the batched form; the per-chunk forms index by (chunk, pair)
+    // round-major, chunk order within a round (the per-chunk forms launch
+    // the contiguous entries of their chunk range)
     for (size_t k = 0; k < rounds; ++k) {
         T.round_off[k] = static_cast<int>(out.size());
         for (size_t c = 0; c < P.chunks.size(); ++c) {
-            if (k >= P.pairs[c].size()) continue;
             const RChunk& ch = P.chunks[c];
-            const auto [i, j] = P.pairs[c][k];
-            const RItem& a = P.items[ch.first + i];
-            const RItem& b = P.items[ch.first + j];
-            SeamWork<Img> s{};
-            s.imgI = make_img<Img>(a, ch);
-            s.imgJ = make_img<Img>(b, ch);
-            s.covI = a.cov; s.covJ = b.cov; s.boxes = ch.boxes; s.ci = i; s.cj = j; s.w = ch.w;
-            s.owner = ch.owner; s.lohi = ch.lohi; s.prevl = ch.prevl; s.chosen = ch.chosen; s.back = ch.back;
-            out.push_back(s);
-            T.round_chunks[k].push_back(static_cast<int>(c));
+            const size_t lines = static_cast<size_t>(std::max(ch.w, ch.h));
+            for (const SeamPair& p : P.pairs[c]) {
+                if (p.round != static_cast<int>(k)) continue;
+                const RItem& a = P.items[ch.first + p.i];
+                const RItem& b = P.items[ch.first + p.j];
+                const size_t sl = static_cast<size_t>(p.slot) * lines;
+                SeamWork<Img> s{};
+                s.imgI = make_img<Img>(a, ch);
+                s.imgJ = make_img<Img>(b, ch);
+                s.covI = a.cov; s.covJ = b.cov; s.boxes = ch.boxes; s.ci = p.i; s.cj = p.j; s.w = ch.w;
+                s.owner = ch.owner;
+                s.lohi = ch.lohi + 2 * sl; s.prevl = ch.prevl + sl; s.chosen = ch.chosen + sl; s.back = ch.back + 64 * sl;
+                s.contested = ch.counters + p.i * ch.count + p.j;
+                out.push_back(s);
+                T.round_chunks[k].push_back(static_cast<int>(c));
+            }
         }
     }
 }
@@ -1039,7 +1083,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
     {
         const int i0 = P.chunks[c0].first;
         const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
-        if (i1 > i0) RL(st, "box_init", 0.0, rbox_init_kernel<<<(i1 - i0 + 63) / 64, 64, 0, s>>>(T.items + i0, i1 - i0));
+        if (i1 > i0) RL(st, "box_init", 0.0, rbox_init_kernel<<<(i1 - i0 + 63) / 64, 64, 0, s>>>(T.items + i0, T.chunks, i1 - i0));
     }
     // warp
     span(P.warp, off, n);
@@ -1062,7 +1106,6 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
     RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
     // seams: the k-th pair of every chunk in range, in pair order
     {
-        ProfScope ps_(ctx, s, "seam_dp", 0.0);
         for (size_t k = 0; k < T.round_chunks.size(); ++k) {
             const auto& rc = T.round_chunks[k];
             int first = -1, cnt = 0;
@@ -1075,15 +1118,20 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
             int maxLines = 1;
             for (int k2 = c0; k2 < c1; ++k2) maxLines = std::max({maxLines, P.chunks[k2].w, P.chunks[k2].h});
             const dim3 sg((maxLines + kScanLinesPerBlock - 1) / kScanLinesPerBlock, cnt);
-            if (MM == kModeFP64) {
-                seam_scan_table_kernel<RgbaImg<double, double>><<<sg, 256, 0, s>>>(T.seam64 + first);
-                seam_dp_table_kernel<RgbaImg<double, double>, double><<<cnt, 1024, 0, s>>>(T.seam64 + first);
-            } else if (MM == kModePlain) {
-                seam_scan_table_kernel<RgbaImg<float, float>><<<sg, 256, 0, s>>>(T.seam32 + first);
-                seam_dp_table_kernel<RgbaImg<float, float>, float><<<cnt, 1024, 0, s>>>(T.seam32 + first);
-            } else {
-                seam_scan_table_kernel<MapImg><<<sg, 256, 0, s>>>(T.seamMap + first);
-                seam_dp_table_kernel<MapImg, double><<<cnt, 1024, 0, s>>>(T.seamMap + first);
+            {
+                ProfScope ps_(ctx, s, "seam_scan", 0.0);
+                if (MM == kModeFP64) seam_scan_table_kernel<RgbaImg<double, double>><<<sg, 256, 0, s>>>(T.seam64 + first);
+                else if (MM == kModePlain) seam_scan_table_kernel<RgbaImg<float, float>><<<sg, 256, 0, s>>>(T.seam32 + first);
+                else seam_scan_table_kernel<MapImg><<<sg, 256, 0, s>>>(T.seamMap + first);
+            }
+            {
+                ProfScope ps_(ctx, s, "seam_dp", 0.0);
+                if (MM == kModeFP64)
+                    seam_dp_table_kernel<RgbaImg<double, double>, double><<<cnt, 1024, 0, s>>>(T.seam64 + first);
+                else if (MM == kModePlain)
+                    seam_dp_table_kernel<RgbaImg<float, float>, float><<<cnt, 1024, 0, s>>>(T.seam32 + first);
+                else
+                    seam_dp_table_kernel<MapImg, double><<<cnt, 1024, 0, s>>>(T.seamMap + first);
             }
             if (hipGetLastError() != hipSuccess) {
                 st.set(I360HIP_KERNEL_LAUNCH_FAILED, "seam launch failed");

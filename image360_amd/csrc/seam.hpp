@@ -61,7 +61,7 @@ template <int U = 4>
 __device__ __forceinline__ void seam_scan_body(const uint8_t* __restrict__ covI, const uint8_t* __restrict__ covJ,
                                                const int* __restrict__ boxes, int ci, int cj, int w,
                                                const int32_t* __restrict__ owner, int* __restrict__ lohi,
-                                               int* __restrict__ chosen, int line) {
+                                               int* __restrict__ chosen, int* __restrict__ contested, int line) {
     const int* bI = boxes + 4 * ci;
     const int* bJ = boxes + 4 * cj;
     const auto* gbI = gptr(bI);
@@ -122,6 +122,7 @@ __device__ __forceinline__ void seam_scan_body(const uint8_t* __restrict__ covI,
         gptr(lohi)[2 * line] = lo;
         gptr(lohi)[2 * line + 1] = hi;
         gptr(chosen)[line] = -1;
+        if (hi >= lo) atomicAdd(contested, 1);
     }
 }
 
@@ -133,13 +134,13 @@ __global__ void __launch_bounds__(256) seam_scan_kernel(const uint8_t* __restric
                                                         const uint8_t* __restrict__ covJ,
                                                         const int* __restrict__ boxes, int ci, int cj, int w,
                                                         const int32_t* __restrict__ owner, int* __restrict__ lohi,
-                                                        int* __restrict__ chosen) {
+                                                        int* __restrict__ chosen, int* __restrict__ contested) {
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body(covI, covJ, boxes, ci, cj, w, owner, lohi, chosen, line);
+    seam_scan_body(covI, covJ, boxes, ci, cj, w, owner, lohi, chosen, contested, line);
 }
 
 template <class Img, typename M>
-__device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
+__device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, Img imgI, Img imgJ,
                                                        const uint8_t* __restrict__ covI,
                                                        const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
                                                        int ci, int cj, int w, int32_t* __restrict__ owner,
@@ -159,7 +160,10 @@ __device__ __forceinline__ void seam_dp_body(Img imgI, Img imgJ,
     const int lines = rowScan ? bh : bw, len = rowScan ? bw : bh;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
 
-    // Phase 1 (crossover and band of every line) ran in seam_scan_kernel.
+    // Phase 1 (crossover and band of every line) ran in seam_scan_kernel,
+    // which also counted the contested lines; none means no DP and no
+    // reassignment (SeamBlend.cpp:409).
+    if (*gptr(contested) == 0) return;
     __syncthreads();
 
     // Phase 2: DP over contested lines (SeamBlend.cpp:362-407).
@@ -284,8 +288,9 @@ __global__ void __launch_bounds__(1024) seam_dp_kernel(Img imgI, Img imgJ, const
                                                        const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
                                                        int ci, int cj, int w, int32_t* __restrict__ owner,
                                                        int* __restrict__ lohi, int* __restrict__ prevl,
-                                                       int* __restrict__ chosen, uint8_t* __restrict__ back) {
-    seam_dp_body<Img, M>(imgI, imgJ, covI, covJ, boxes, ci, cj, w, owner, lohi, prevl, chosen, back);
+                                                       int* __restrict__ chosen, uint8_t* __restrict__ back,
+                                                       const int* __restrict__ contested) {
+    seam_dp_body<Img, M>(contested, imgI, imgJ, covI, covJ, boxes, ci, cj, w, owner, lohi, prevl, chosen, back);
 }
 
 // One pair of one chunk per table entry (batched form: every chunk's k-th
@@ -302,20 +307,21 @@ struct SeamWork {
     int* prevl;
     int* chosen;
     uint8_t* back;
+    int* contested;  // contested-line count of this pair, cleared before the render
 };
 
 template <class Img>
 __global__ void __launch_bounds__(256) seam_scan_table_kernel(const SeamWork<Img>* __restrict__ work) {
     const SeamWork<Img> wk = work[blockIdx.y];
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body(wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi, wk.chosen, line);
+    seam_scan_body(wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi, wk.chosen, wk.contested, line);
 }
 
 template <class Img, typename M>
 __global__ void __launch_bounds__(1024) seam_dp_table_kernel(const SeamWork<Img>* __restrict__ work) {
     const SeamWork<Img> wk = work[blockIdx.x];
-    seam_dp_body<Img, M>(wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi,
-                         wk.prevl, wk.chosen, wk.back);
+    seam_dp_body<Img, M>(wk.contested, wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner,
+                         wk.lohi, wk.prevl, wk.chosen, wk.back);
 }
 
 }  // namespace i360

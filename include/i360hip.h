@@ -34,7 +34,7 @@ extern "C" {
 #define I360HIP_ABI_VERSION 1
 
 /* Result codes: the values of WindowsApp::Compute::ComputeResult
- * (WindowsApp.Compute/HeaderFiles/ComputeTypes.h:202-220). */
+ * (WindowsApp.Compute/HeaderFiles/ComputeTypes.h:32-50). */
 enum {
     I360HIP_SUCCESS = 0,
     I360HIP_NO_GPU = 1,
@@ -54,7 +54,7 @@ enum {
 typedef struct i360hip_ctx i360hip_ctx;
 
 /* Device identity, the fields of WindowsApp::Compute::GpuInfo
- * (ComputeTypes.h:188-199) plus the gfx architecture name. */
+ * (ComputeTypes.h:18-29) plus the gfx architecture name. */
 typedef struct i360hip_device_info {
     int device_id;
     char name[256];

@@ -118,6 +118,36 @@ def coverage_distance(cov):
     return out
 
 
+def ownership(dists):
+    """ComputeOwnership: argmax distance > 0, ties to the lowest index."""
+    dists = [np.ascontiguousarray(d, dtype=np.float32) for d in dists]
+    h, w = dists[0].shape
+    owner = np.zeros((h, w), np.int32)
+    lib().orc_ownership(_ptr_array(dists, f32p), len(dists), w, h, _p(owner, i32p))
+    return owner
+
+
+def cover_box(cov):
+    cov = np.ascontiguousarray(cov, dtype=np.uint8)
+    h, w = cov.shape
+    box = np.zeros(4, np.int32)
+    ok = lib().orc_cover_box(_p(cov, u8p), w, h, _p(box, i32p))
+    return box if ok else None
+
+
+def refine_seam_dp(imgI, imgJ, distI, distJ, ci, cj, boxI, boxJ, ov, owner):
+    """RefineSeamDP of pair (ci, cj) on RGB48 images, updating `owner` in place."""
+    imgI = np.ascontiguousarray(imgI, dtype=np.uint16)
+    imgJ = np.ascontiguousarray(imgJ, dtype=np.uint16)
+    distI = np.ascontiguousarray(distI, dtype=np.float32)
+    distJ = np.ascontiguousarray(distJ, dtype=np.float32)
+    assert owner.dtype == np.int32 and owner.flags.c_contiguous
+    w = owner.shape[1]
+    b = [np.ascontiguousarray(x, dtype=np.int32) for x in (boxI, boxJ, ov)]
+    lib().orc_refine_seam_dp(_p(imgI, u16p), _p(imgJ, u16p), _p(distI, f32p), _p(distJ, f32p), w, ci, cj,
+                             _p(b[0], i32p), _p(b[1], i32p), _p(b[2], i32p), _p(owner, i32p))
+
+
 def blend_chunk(warped, num_bands=5, want_owner=False):
     warped = [np.ascontiguousarray(x, dtype=np.uint16) for x in warped]
     h, w = warped[0].shape[:2]

@@ -9,7 +9,7 @@ import pytest
 from image360_amd import _abi
 from oracle import pyoracle as po
 from tests.gpu_util import note
-from tests.render_cases import psnr, small_case
+from tests.render_cases import grid_case, psnr, small_case
 
 pytestmark = pytest.mark.gpu
 
@@ -147,3 +147,25 @@ def test_seven_bands_and_uneven_chunks(hip):
     p, po_ = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION, bands=7)
     for k in range(len(g.chunks)):
         assert np.array_equal(b[k], p[k])
+
+
+@pytest.mark.parametrize("chunk", [None, 160])
+def test_grid_pairs_side_by_side(hip, chunk):
+    # a 3x2 grid: chunks hold seam pairs with no contributor in common, which
+    # the render runs in the same round
+    g, frames = grid_case(seed=7, chunk=chunk)
+    dev = _upload(frames)
+    note(f"render grid chunk={chunk}")
+    outs, owners = _gpu_render(hip, g, dev, _abi.RENDER_PLAIN)
+    for k, c in enumerate(g.chunks):
+        want, want_owner = _oracle(g, frames, c, fp64=False)
+        assert np.array_equal(owners[k], want_owner), k
+        assert np.max(np.abs(outs[k] - want)) <= 2e-6, k
+    outs64, own64 = _gpu_render(hip, g, dev, _abi.RENDER_FP64)
+    for k, c in enumerate(g.chunks):
+        want, want_owner = _oracle(g, frames, c, fp64=True)
+        assert np.array_equal(own64[k], want_owner), k
+        assert np.max(np.abs(outs64[k] - want)) <= 1e-9, k
+    b, bo = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED)
+    for k in range(len(g.chunks)):
+        assert np.array_equal(bo[k], own64[k]), k
