@@ -4,7 +4,10 @@
 
 Every ``csrc/*.hip`` file is compiled with hipcc for gfx950 only, then
 linked into one shared library exporting the C ABI of
-``include/i360hip.h``.  All device code is compiled with
+``include/i360hip.h``.  The C++ host side (``host/*.cpp``: the
+IComputeBackend implementation and its factory glue) is plain C++17 over that
+ABI, built with g++ into ``lib/libi360hip_backend.so``, and
+``tests/cpp/hip_backend_test.cpp`` into ``lib/hip_backend_test``.  All device code is compiled with
 ``-ffp-contract=off`` (the bit-exact kernels depend on it; the fast kernels
 request fused multiply-adds explicitly with fmaf) and without packed-FP32
 instruction selection.
@@ -55,9 +58,35 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    build_host()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+HOST = os.path.join(PKG, "host")
+HOST_LIB = os.path.join(LIB_DIR, "libi360hip_backend.so")
+HOST_TEST = os.path.join(LIB_DIR, "hip_backend_test")
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
+HOST_FLAGS = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter",
+              f"-I{HOST}", f"-I{os.path.join(HOST, 'contract')}", f"-I{os.path.join(ROOT, 'include')}"]
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{r.stderr[-6000:]}")
+
+
+def build_host() -> None:
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    _run([CXX, *HOST_FLAGS, "-fPIC", "-shared", *srcs, f"-L{LIB_DIR}", "-li360hip", "-Wl,-rpath,$ORIGIN",
+          "-o", HOST_LIB + ".tmp"], "host library build")
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    test_src = os.path.join(ROOT, "tests", "cpp", "hip_backend_test.cpp")
+    _run([CXX, *HOST_FLAGS, test_src, f"-L{LIB_DIR}", "-li360hip_backend", "-li360hip", "-lpthread",
+          "-Wl,-rpath,$ORIGIN", "-o", HOST_TEST + ".tmp"], "host test build")
+    os.replace(HOST_TEST + ".tmp", HOST_TEST)
 
 
 if __name__ == "__main__":
