@@ -52,8 +52,8 @@ class ChunkDesc(C.Structure):
 
 
 PIX_F16, PIX_F32 = 0, 1
-RENDER_FP64, RENDER_PLAIN, RENDER_PRECISION, RENDER_BATCHED = 0, 1, 2, 3
-RENDER_NAMES = {0: "fp64", 1: "plain", 2: "precision", 3: "batched"}
+RENDER_FP64, RENDER_PLAIN, RENDER_PRECISION, RENDER_BATCHED, RENDER_SURROGATE = 0, 1, 2, 3, 4
+RENDER_NAMES = {0: "fp64", 1: "plain", 2: "precision", 3: "batched", 4: "surrogate"}
 
 
 # name -> (restype, argtypes)
@@ -80,6 +80,15 @@ SIGNATURES = {
     "i360hip_warp_perspective_u16_dev": (i32, [vp, vp, i32, i32, vp, i32, i32, vp, i32, i32, vp]),
     "i360hip_apply_gain_u16_dev": (i32, [vp, vp, sz, C.c_float, vp]),
     "i360hip_blend_chunk_u16_dev": (i32, [vp, vp, i32, i32, i32, i32, vp, vp, vp]),
+    "i360hip_surrogate_forward": (i32, [vp, vp, vp, vp, i32, i32, i32, vp, vp]),
+    "i360hip_surrogate_gradients": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
+    "i360hip_surrogate_train_step": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, C.c_float, i32, vp, vp]),
+    "i360hip_surrogate_round_bf16": (i32, [vp, vp, vp, vp]),
+    "i360hip_surrogate_set": (i32, [vp, vp, vp, C.c_float]),
+    "i360hip_surrogate_chunk_level": (i32, [i32, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+    "i360hip_surrogate_features": (i32, [vp, C.POINTER(Frame), i32, C.POINTER(ChunkDesc), i32, C.POINTER(i32), i32,
+                                         vp, vp, vp]),
+    "i360hip_surrogate_stats": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
 }
 
 

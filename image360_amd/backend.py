@@ -29,6 +29,31 @@ def _c_contig(a, dtype):
     return a
 
 
+def _ordered(fn):
+    """Device-pointer calls without a stream run on the context's own
+    stream; the buffers they take are usually torch tensors written and read
+    on torch's streams, so such a call is fenced by device synchronisations
+    on both sides.  Calls given a stream are asynchronous on it."""
+    import functools
+    import inspect
+
+    pos = list(inspect.signature(fn).parameters).index("stream")
+
+    @functools.wraps(fn)
+    def wrapper(*args, **kwargs):
+        stream = kwargs.get("stream", args[pos] if len(args) > pos else None)
+        if stream is not None:
+            return fn(*args, **kwargs)
+        import torch
+
+        torch.cuda.synchronize()
+        rc = fn(*args, **kwargs)
+        torch.cuda.synchronize()
+        return rc
+
+    return wrapper
+
+
 class HipComputeBackend:
     """IComputeBackend over the HIP library (one device context)."""
 
@@ -136,11 +161,13 @@ class HipComputeBackend:
     RobustMergeAccumulate = TileFftMerge = StructureTensorKernelRegression = BlockMatchAlign
 
     # ---- device-pointer operations (asynchronous on `stream`)
+    @_ordered
     def median_stack_dev(self, input_ptrs, output_ptr, num_scalars, sigma=2.0, stream=None) -> int:
         arr = (C.c_void_p * max(1, len(input_ptrs)))(*input_ptrs)
         return lib().i360hip_median_stack_u16_dev(self._ctx, arr, len(input_ptrs), output_ptr, num_scalars,
                                                   float(sigma), stream)
 
+    @_ordered
     def demosaic_dev(self, cfa_ptr, width, height, black, cam_mul, rgb_cam, filters, out_ptr, f32=False,
                      stream=None) -> int:
         cm = np.ascontiguousarray(cam_mul, dtype=np.float32).reshape(4)
@@ -148,14 +175,17 @@ class HipComputeBackend:
         fn = lib().i360hip_demosaic_bayer_f32_dev if f32 else lib().i360hip_demosaic_bayer_u16_dev
         return fn(self._ctx, cfa_ptr, width, height, black, _ptr(cm), _ptr(rc_), filters, out_ptr, stream)
 
+    @_ordered
     def warp_dev(self, src_ptr, srcW, srcH, dst_ptr, dstW, dstH, inv_h, offX, offY, stream=None) -> int:
         m = np.ascontiguousarray(inv_h, dtype=np.float32).reshape(9)
         return lib().i360hip_warp_perspective_u16_dev(self._ctx, src_ptr, srcW, srcH, dst_ptr, dstW, dstH, _ptr(m),
                                                       offX, offY, stream)
 
+    @_ordered
     def gain_dev(self, ptr, n, gain, stream=None) -> int:
         return lib().i360hip_apply_gain_u16_dev(self._ctx, ptr, n, float(gain), stream)
 
+    @_ordered
     def blend_dev(self, warped_ptrs, width, height, out_ptr, num_bands=DEFAULT_BANDS, owner_ptr=None,
                   stream=None) -> int:
         arr = (C.c_void_p * max(1, len(warped_ptrs)))(*warped_ptrs)
@@ -182,6 +212,7 @@ class HipComputeBackend:
         cont = (C.c_int * max(1, len(contributors)))(*[int(v) for v in contributors])
         return fa, ca, cont
 
+    @_ordered
     def render_chunks(self, frames, chunks, contributors, num_bands, mode, stream=None) -> int:
         """i360hip_render_chunks: frames = [{ptr, w, h, fmt, gain, H}], chunks =
         [{x, y, w, h, first, count, out, owner}] (device pointers)."""
@@ -194,6 +225,49 @@ class HipComputeBackend:
         self.check(lib().i360hip_render_map_error(self._ctx, fa, len(frames), ca, len(chunks), cont, C.byref(out)),
                    "render_map_error")
         return out.value
+
+    # ---- learned surrogate (device pointers; see include/i360hip.h) ----------
+    @_ordered
+    def surrogate_forward(self, weights, biases, feat, n, h, w, out, stream=None) -> int:
+        return lib().i360hip_surrogate_forward(self._ctx, weights, biases, feat, n, h, w, out, stream)
+
+    @_ordered
+    def surrogate_gradients(self, weights, biases, feat, target, n, h, w, grad_w, grad_b, loss, stream=None) -> int:
+        return lib().i360hip_surrogate_gradients(self._ctx, weights, biases, feat, target, n, h, w, grad_w, grad_b,
+                                                 loss, stream)
+
+    @_ordered
+    def surrogate_train_step(self, master, biases, adam, weights, feat, target, n, h, w, lr, step, loss,
+                             stream=None) -> int:
+        return lib().i360hip_surrogate_train_step(self._ctx, master, biases, adam, weights, feat, target, n, h, w,
+                                                  lr, step, loss, stream)
+
+    @_ordered
+    def surrogate_round_bf16(self, master, weights, stream=None) -> int:
+        return lib().i360hip_surrogate_round_bf16(self._ctx, master, weights, stream)
+
+    def surrogate_set(self, weights, biases, threshold_mse=1e-5) -> int:
+        return lib().i360hip_surrogate_set(self._ctx, weights, biases, threshold_mse)
+
+    @staticmethod
+    def surrogate_chunk_level(w, h, num_bands):
+        """(K, level width, level height); K = 0 when the chunk has < 4 levels."""
+        k, lw, lh = C.c_int(0), C.c_int(0), C.c_int(0)
+        rc = lib().i360hip_surrogate_chunk_level(w, h, num_bands, C.byref(k), C.byref(lw), C.byref(lh))
+        if rc != SUCCESS:
+            raise HipError(rc, "surrogate_chunk_level", "invalid arguments")
+        return k.value, lw.value, lh.value
+
+    @_ordered
+    def surrogate_features(self, frames, chunks, contributors, num_bands, feat, target, stream=None) -> int:
+        fa, ca, cont = self._render_args(frames, chunks, contributors)
+        return lib().i360hip_surrogate_features(self._ctx, fa, len(frames), ca, len(chunks), cont, num_bands, feat,
+                                                target, stream)
+
+    def surrogate_stats(self):
+        t, f = C.c_int(0), C.c_int(0)
+        self.check(lib().i360hip_surrogate_stats(self._ctx, C.byref(t), C.byref(f)), "surrogate_stats")
+        return t.value, f.value
 
     def profile_enable(self, on=True) -> None:
         lib().i360hip_profile_enable(self._ctx, 1 if on else 0)

@@ -86,6 +86,11 @@ struct i360hip_ctx {
     char last_error[512] = {};
     hipDeviceProp_t props{};
     std::vector<i360::DevBuf> bufs;  // indexed scratch slots
+    // learned surrogate parameters for the SURROGATE render form
+    const uint16_t* surr_weights = nullptr;  // device copy (a scratch slot)
+    const float* surr_biases = nullptr;
+    float surr_threshold = 1e-5f;
+    int surr_tiles = 0, surr_fallback = -1;  // last surrogate render
     std::vector<i360::DevBuf> pinned;
 };
 

@@ -52,6 +52,22 @@ struct RChunk {
     int* chosen;
     uint8_t* back;
     int* counters;  // per-pair contested-line counts (count*count ints)
+    // learned surrogate (SURROGATE form and its training-data export)
+    int sk;           // pyramid level the surrogate predicts (levels-3); 0 = none for this chunk
+    int spix0;        // first level-sk pixel of this chunk in the surrogate buffers
+    int stile0, stx;  // first 16x16 surrogate tile of this chunk, tiles per row
+};
+
+// Surrogate buffers of a render (NHWC, indexed by RChunk::spix0 + pixel).
+struct SurrBufs {
+    float* feat;  // 16 features per level-K pixel
+    float* aux;   // 4 floats per level-K pixel (first blur of the composite)
+    float* out;   // 4 network outputs per level-K pixel
+    float* tmp0;  // 32 floats per level-K pixel (network ping-pong)
+    float* tmp1;
+    float* target;  // export only: the plain blend's level-K reconstruction, 3 per pixel
+    int* tflag;   // per 16x16 tile: fall back to the plain blend
+    int* cflag;   // per chunk: some tile falls back (the deep levels are computed)
 };
 
 // A tile of some stage: id = item or chunk index, (tx, ty) tile coordinates.

@@ -26,6 +26,7 @@
 #include "chamfer.hpp"
 #include "render.hpp"
 #include "seam.hpp"
+#include "surrogate.hpp"
 
 namespace i360 {
 
@@ -129,6 +130,9 @@ __device__ __forceinline__ void fetch_rgb(const void* src, int fmt, size_t idx, 
     }
 }
 
+typedef uint32_t u4a __attribute__((ext_vector_type(4), aligned(4)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
 // Per-tile FP32 local map reduced from the FP64 inverse: for the tile-local
 // offset d = (dx, dy), source = s0 + (A d) / (1 + g . d) exactly (the
 // projective map re-centred on the tile origin), s0 split into an integer
@@ -136,6 +140,7 @@ __device__ __forceinline__ void fetch_rgb(const void* src, int fmt, size_t idx, 
 struct TileMap {
     int ix, iy;
     float fx, fy, a00, a01, a10, a11, g0, g1, q0;
+    float margin;  // bound on |FP32 position - FP64 position| (source px) inside the tile
     int ok;
 };
 
@@ -159,6 +164,12 @@ __device__ __forceinline__ TileMap tile_map(const double* h, double X0, double Y
     t.g0 = static_cast<float>(h[6] / q0);
     t.g1 = static_cast<float>(h[7] / q0);
     t.q0 = static_cast<float>(q0);
+    // position error: binary32 Jacobian terms (relative 2^-24 each) times
+    // offsets <= 64, the division, and two roundings of the absolute
+    // position; the 2e-3 px floor is ~100x the measured error
+    const double jac = fabs(t.a00) + fabs(t.a01) + fabs(t.a10) + fabs(t.a11);
+    const double mag = fmax(fabs(sx), fabs(sy)) + 64.0 * jac;
+    t.margin = static_cast<float>(2e-3 + 64.0 * jac * 1.2e-6 + mag * 2.4e-7);
     t.ok = 1;
     return t;
 }
@@ -187,11 +198,31 @@ __device__ __forceinline__ bool map_pixel(const RItem& it, const RChunk& ch, con
         // the FP32 tile map; bilinear sampling is continuous across integer
         // boundaries, so a position that lands a hair past the last valid
         // cell is resolved to that cell's edge.
-        if (!fp64_valid(it, ch, x, y)) return false;
+        //
+        // Far from the horizon (den > 1/2, so |q| > 5e-7 and the FP64 map
+        // is regular) the FP32 position decides whenever it is farther than
+        // the tile's error bound from the validity edges (u = 0, u = fw-1,
+        // v = 0, v = fh-1); only pixels within that band, a thin ring at the
+        // footprint edge, evaluate the FP64 map.
         const float den = (1.0f + tm.g0 * static_cast<float>(dx)) + tm.g1 * static_cast<float>(dy);
-        const float ox = (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) / den;
-        const float oy = (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) / den;
-        const float lx = tm.fx + ox, ly = tm.fy + oy;
+        float lx, ly;
+        if (den > 0.5f) {
+            const float r = 1.0f / den;
+            lx = tm.fx + (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) * r;
+            ly = tm.fy + (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) * r;
+            const float ux = static_cast<float>(tm.ix) + lx, uy = static_cast<float>(tm.iy) + ly;
+            const float ex = static_cast<float>(it.fw - 1), ey = static_cast<float>(it.fh - 1);
+            const float m = tm.margin;
+            if (fminf(fabsf(ux), fabsf(ux - ex)) < m || fminf(fabsf(uy), fabsf(uy - ey)) < m) {
+                if (!fp64_valid(it, ch, x, y)) return false;
+            } else if (!(ux >= 0.0f && ux < ex && uy >= 0.0f && uy < ey)) {
+                return false;
+            }
+        } else {
+            if (!fp64_valid(it, ch, x, y)) return false;
+            lx = tm.fx + (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) / den;
+            ly = tm.fy + (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) / den;
+        }
         const float flx = floorf(lx), fly = floorf(ly);
         sx0 = tm.ix + static_cast<int>(flx);
         sy0 = tm.iy + static_cast<int>(fly);
@@ -241,6 +272,45 @@ __device__ __forceinline__ bool map_pixel(const RItem& it, const RChunk& ch, con
     return true;
 }
 
+// Raw 16 bytes holding texels idx, idx+1 of an FP16 RGB frame: the
+// 4-byte-aligned window at or below byte 6*idx, clamped into the frame;
+// `sh` receives the pair's byte offset in the window (0, 2, 4 or, for the
+// frame's very last pair when its size is 2 mod 4, 6).
+__device__ __forceinline__ u4a load_pair_f16(const void* src, size_t idx, size_t nbytes, uint32_t& sh) {
+    const size_t o = idx * 6;
+    const size_t o4 = min(o & ~static_cast<size_t>(3), (nbytes - 16) & ~static_cast<size_t>(3));
+    sh = static_cast<uint32_t>(o - o4);
+    // the window is only 4-byte aligned: address it directly (gptr's
+    // parameter would carry the vector type's natural 16-byte alignment)
+    return *(const I360_GLOBAL u4a*)(static_cast<const uint8_t*>(src) + o4);
+}
+
+template <typename M>
+__device__ __forceinline__ void decode_pair_f16(const u4a& q, uint32_t sh, const void* src, size_t idx, M (&a)[3],
+                                                M (&b)[3]) {
+    if (sh > 4) {  // last pair of an odd-sized frame: texel by texel
+        fetch_rgb<M>(src, I360HIP_PIX_F16, idx, a);
+        fetch_rgb<M>(src, I360HIP_PIX_F16, idx + 1, b);
+        return;
+    }
+    const bool s = sh >= 4;
+    const uint32_t bits = (sh & 2u) * 8u;
+    const uint32_t w0 = s ? q.y : q.x, w1 = s ? q.z : q.y, w2 = s ? q.w : q.z, w3 = s ? 0u : q.w;
+    const h2v d0 = __builtin_bit_cast(h2v, __builtin_amdgcn_alignbit(w1, w0, bits));
+    const h2v d1 = __builtin_bit_cast(h2v, __builtin_amdgcn_alignbit(w2, w1, bits));
+    const h2v d2 = __builtin_bit_cast(h2v, __builtin_amdgcn_alignbit(w3, w2, bits));
+    a[0] = static_cast<M>(static_cast<float>(d0.x));
+    a[1] = static_cast<M>(static_cast<float>(d0.y));
+    a[2] = static_cast<M>(static_cast<float>(d1.x));
+    b[0] = static_cast<M>(static_cast<float>(d1.y));
+    b[1] = static_cast<M>(static_cast<float>(d2.x));
+    b[2] = static_cast<M>(static_cast<float>(d2.y));
+}
+
+// Warp of one item's tile: every thread maps its 4 pixels first, then
+// issues all 8 texel-row loads (FP16 frames: unconditional, clamped
+// addresses), then blends and stores, so the loads of a thread are in
+// flight together instead of one pixel after another.
 template <int MODE>
 __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ items,
                                                     const RChunk* __restrict__ chunks,
@@ -248,6 +318,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
     using M = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::M;
     using S = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::S;
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    constexpr int R = 4;
     __shared__ TileMap smap;
     __shared__ int sbox[4];
     for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
@@ -263,39 +334,74 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
         __syncthreads();
         const TileMap tm = smap;
         const int* fb = it.lbox[0];
-        const bool useGain = fabsf(it.gain - 1.0f) >= 1e-6f;
-        int bx0 = INT_MAX, by0 = INT_MAX, bx1 = -1, by1 = -1;
+        const void* src = it.src;
+        const int fw = it.fw, fmt = it.fmt;
+        const size_t nbytes = static_cast<size_t>(fw) * it.fh * (fmt == I360HIP_PIX_F16 ? 6 : 12);
+        const float gain = it.gain;
+        const bool useGain = fabsf(gain - 1.0f) >= 1e-6f;
         const int lx = threadIdx.x & 63;
+        const int x = tx0 + lx;
+        // phase 1: map
+        bool in[R], ok[R];
+        size_t i0[R];
+        M fx[R], fy[R];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int ly = (threadIdx.x >> 6) * 4 + r;
-            const int x = tx0 + lx, y = ty0 + ly;
-            if (x >= ch.w || y >= ch.h || !in_box(fb, x, y)) continue;
+        for (int r = 0; r < R; ++r) {
+            const int ly = (threadIdx.x >> 6) * R + r;
+            const int y = ty0 + ly;
+            in[r] = x < ch.w && y < ch.h && in_box(fb, x, y);
+            int sx0 = 0, sy0 = 0;
+            fx[r] = M(0);
+            fy[r] = M(0);
+            ok[r] = in[r] && map_pixel<MM, M>(it, ch, tm, x, y, lx, ly, sx0, sy0, fx[r], fy[r]);
+            i0[r] = ok[r] ? static_cast<size_t>(sy0) * fw + sx0 : 0;
+        }
+        // phase 2: loads
+        M a[R][3], b[R][3], c[R][3], d[R][3];
+        if (fmt == I360HIP_PIX_F16) {
+            u4a q0[R], q1[R];
+            uint32_t s0[R], s1[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                q0[r] = load_pair_f16(src, i0[r], nbytes, s0[r]);
+                q1[r] = load_pair_f16(src, i0[r] + fw, nbytes, s1[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                decode_pair_f16<M>(q0[r], s0[r], src, i0[r], a[r], b[r]);
+                decode_pair_f16<M>(q1[r], s1[r], src, i0[r] + fw, c[r], d[r]);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                fetch_rgb<M>(src, fmt, i0[r], a[r]);
+                fetch_rgb<M>(src, fmt, i0[r] + 1, b[r]);
+                fetch_rgb<M>(src, fmt, i0[r] + fw, c[r]);
+                fetch_rgb<M>(src, fmt, i0[r] + fw + 1, d[r]);
+            }
+        }
+        // phase 3: blend, store
+        int bx0 = INT_MAX, by0 = INT_MAX, bx1 = -1, by1 = -1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!in[r]) continue;
+            const int y = ty0 + (threadIdx.x >> 6) * R + r;
             const size_t p = static_cast<size_t>(y) * ch.w + x;
-            int sx0, sy0;
-            M fx, fy;
             M v[4] = {M(0), M(0), M(0), M(0)};
-            const bool ok = map_pixel<MM, M>(it, ch, tm, x, y, lx, ly, sx0, sy0, fx, fy);
-            if (ok) {
-                const M omx = M(1) - fx, omy = M(1) - fy;
-                const M w00 = omx * omy, w10 = fx * omy, w01 = omx * fy, w11 = fx * fy;
-                M a[3], b[3], c[3], d[3];
-                const size_t i0 = static_cast<size_t>(sy0) * it.fw + sx0;
-                fetch_rgb<M>(it.src, it.fmt, i0, a);
-                fetch_rgb<M>(it.src, it.fmt, i0 + 1, b);
-                fetch_rgb<M>(it.src, it.fmt, i0 + it.fw, c);
-                fetch_rgb<M>(it.src, it.fmt, i0 + it.fw + 1, d);
+            if (ok[r]) {
+                const M omx = M(1) - fx[r], omy = M(1) - fy[r];
+                const M w00 = omx * omy, w10 = fx[r] * omy, w01 = omx * fy[r], w11 = fx[r] * fy[r];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    M val = ((w00 * a[k] + w10 * b[k]) + w01 * c[k]) + w11 * d[k];
-                    if (useGain) val = val * static_cast<M>(it.gain);
+                    M val = ((w00 * a[r][k] + w10 * b[r][k]) + w01 * c[r][k]) + w11 * d[r][k];
+                    if (useGain) val = val * static_cast<M>(gain);
                     v[k] = val < M(0) ? M(0) : (M(1) < val ? M(1) : val);
                 }
                 v[3] = M(1);
                 bx0 = min(bx0, x); by0 = min(by0, y); bx1 = max(bx1, x); by1 = max(by1, y);
             }
             st4(it.warped, p, v, static_cast<S*>(nullptr));
-            gptr(it.cov)[p] = ok ? 1 : 0;
+            gptr(it.cov)[p] = ok[r] ? 1 : 0;
         }
         if (bx1 >= 0) {
             atomicMin(&sbox[0], bx0); atomicMin(&sbox[1], by0); atomicMax(&sbox[2], bx1); atomicMax(&sbox[3], by1);
@@ -439,10 +545,22 @@ __device__ __forceinline__ M blur5m(M a, M b, M m, M d, M e) {
 
 // One tile of level l+1 for one item: level-l samples outside the item's
 // support read as zero.
+// Surrogate phases of the pyramid stages: 0 = every level; 1 = the levels
+// up to the surrogate level K of each chunk; 2 = the levels above K, only
+// for chunks with a fallback tile (cflag).  Chunks without a surrogate
+// level do everything in phase 1.
+__device__ __forceinline__ bool phase_skip(const RChunk& ch, int c, int lvl, int phase, const int* cflag) {
+    if (phase == 0) return false;
+    const int K = ch.sk >= 1 ? ch.sk : kMaxLevels;
+    if (phase == 1) return lvl > K;
+    return lvl <= K || !cflag[c];
+}
+
 template <int MODE, bool L0>
 __global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ items,
                                                       const RChunk* __restrict__ chunks,
-                                                      const RTile* __restrict__ tiles, int ntiles, int l) {
+                                                      const RTile* __restrict__ tiles, int ntiles, int l,
+                                                      int phase, const int* __restrict__ cflag) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
@@ -451,6 +569,7 @@ __global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ 
         const RTile tl = tiles[e];
         const RItem& it = items[tl.id];
         const RChunk& ch = chunks[it.chunk];
+        if (phase_skip(ch, it.chunk, l + 1, phase, cflag)) continue;  // block-uniform
         const int w = ch.lw[l], h = ch.lh[l], ow = ch.lw[l + 1], oh = ch.lh[l + 1];
         const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
         const int* sb = L0 ? it.box : it.lbox[l];
@@ -575,13 +694,18 @@ constexpr int kCompTW = 64, kCompTH = 4;
 template <int MODE>
 __global__ void __launch_bounds__(256) rcompose_kernel(const RItem* __restrict__ items,
                                                        const RChunk* __restrict__ chunks,
-                                                       const RTile* __restrict__ tiles, int ntiles, int l) {
+                                                       const RTile* __restrict__ tiles, int ntiles, int l,
+                                                       int phase, const int* __restrict__ cflag) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
     for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
         const RTile tl = tiles[e];
         const RChunk& ch = chunks[tl.id];
+        // compose at level l is needed below K (phase 1) and at K and above
+        // for chunks that fall back (phase 2)
+        if (phase == 1 && l >= (ch.sk >= 1 ? ch.sk : kMaxLevels)) continue;
+        if (phase == 2 && (ch.sk < 1 || l < ch.sk || !cflag[tl.id])) continue;
         const int x = tl.tx * kCompTW + (threadIdx.x & 63), y = tl.ty * kCompTH + (threadIdx.x >> 6);
         const int w = ch.lw[l], h = ch.lh[l];
         if (x >= w || y >= h) continue;
@@ -615,6 +739,206 @@ __global__ void __launch_bounds__(256) rcompose_kernel(const RItem* __restrict__
         }
         const M o4[4] = {band[0], band[1], band[2], M(0)};
         st4(ch.recon[l], p, o4, static_cast<M*>(nullptr));
+    }
+}
+
+// ---------------------------------------------------------- surrogate ---
+// Per level-K pixel of a chunk (tiles: 64x4 at level K), the network's
+// inputs: the Gaussians of the two largest-weight contributors (ties to
+// the lower index; the second is the first where only one weighs), the
+// seam mask wA / (wA + wB), the single-level composite sum_i w_i G_i; the
+// two Laplacian scales of the composite (channels 10-15) come from
+// rsblur_kernel.
+template <int MODE>
+__global__ void __launch_bounds__(256) rsfeat_kernel(const RItem* __restrict__ items,
+                                                     const RChunk* __restrict__ chunks,
+                                                     const RTile* __restrict__ tiles, int ntiles, SurrBufs sb) {
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    using M = typename Form<MM>::M;
+    using S = typename Form<MM>::S;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RChunk& ch = chunks[tl.id];
+        const int l = ch.sk;
+        const int x = tl.tx * kCompTW + (threadIdx.x & 63), y = tl.ty * kCompTH + (threadIdx.x >> 6);
+        const int w = ch.lw[l], h = ch.lh[l];
+        if (x >= w || y >= h) continue;
+        const size_t p = static_cast<size_t>(y) * w + x;
+        M wa = M(-1), wb = M(-1), ga[3] = {M(0), M(0), M(0)}, gb[3] = {M(0), M(0), M(0)};
+        M base[3] = {M(0), M(0), M(0)};
+        for (int k = 0; k < ch.count; ++k) {
+            const RItem& it = items[ch.first + k];
+            if (!in_box(it.lbox[l], x, y)) continue;
+            const M wt = ld1<M, S>(it.wt[l], p);
+            if (wt == M(0)) continue;
+            M g[3];
+            gauss_m<M, S>(it.nd[l], it.lbox[l], w, x, y, g);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) base[q] = base[q] + wt * g[q];
+            if (wt > wa) {
+                wb = wa;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) { gb[q] = ga[q]; ga[q] = g[q]; }
+                wa = wt;
+            } else if (wt > wb) {
+                wb = wt;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) gb[q] = g[q];
+            }
+        }
+        if (wb < M(0)) {  // one contributor (or none)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) gb[q] = ga[q];
+            wb = M(0);
+        }
+        if (wa < M(0)) wa = M(0);
+        const M m = (wa + wb > M(0)) ? wa / (wa + wb) : M(0);
+        float* f = sb.feat + (static_cast<size_t>(ch.spix0) + p) * kSurrIn;
+        const float v[16] = {static_cast<float>(ga[0]), static_cast<float>(ga[1]), static_cast<float>(ga[2]),
+                             static_cast<float>(gb[0]), static_cast<float>(gb[1]), static_cast<float>(gb[2]),
+                             static_cast<float>(m), static_cast<float>(base[0]), static_cast<float>(base[1]),
+                             static_cast<float>(base[2]), 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4)
+            *reinterpret_cast<float4*>(f + 4 * q4) = make_float4(v[4 * q4], v[4 * q4 + 1], v[4 * q4 + 2], v[4 * q4 + 3]);
+    }
+}
+
+// [1,4,6,4,1]/16 (x) [1,4,6,4,1]/16 blur of a 3-channel level-K plane of a
+// chunk at dilation `dil`, replicate borders.
+__device__ __forceinline__ void blur25(const float* src, int stride, int ch0, int w, int h, int x, int y, int dil,
+                                       float (&o)[3]) {
+    const float k[5] = {1.f / 16, 4.f / 16, 6.f / 16, 4.f / 16, 1.f / 16};
+    o[0] = o[1] = o[2] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int yy = min(max(y + (j - 2) * dil, 0), h - 1);
+        float r[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int xx = min(max(x + (i - 2) * dil, 0), w - 1);
+            const float* q = src + (static_cast<size_t>(yy) * w + xx) * stride + ch0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) r[c] = fmaf(k[i], q[c], r[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = fmaf(k[j], r[c], o[c]);
+    }
+}
+
+// pass 0: aux = B1 = blur(composite);  pass 1: feat[10..12] = composite - B1,
+// feat[13..15] = B1 - blur_dilated2(B1).
+__global__ void __launch_bounds__(256) rsblur_kernel(const RChunk* __restrict__ chunks,
+                                                     const RTile* __restrict__ tiles, int ntiles, SurrBufs sb,
+                                                     int pass) {
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RChunk& ch = chunks[tl.id];
+        const int l = ch.sk;
+        const int x = tl.tx * kCompTW + (threadIdx.x & 63), y = tl.ty * kCompTH + (threadIdx.x >> 6);
+        const int w = ch.lw[l], h = ch.lh[l];
+        if (x >= w || y >= h) continue;
+        const size_t q = static_cast<size_t>(ch.spix0) + static_cast<size_t>(y) * w + x;
+        const float* feat = sb.feat + static_cast<size_t>(ch.spix0) * kSurrIn;
+        const float* aux = sb.aux + static_cast<size_t>(ch.spix0) * 4;
+        float o[3];
+        if (pass == 0) {
+            blur25(feat, kSurrIn, 7, w, h, x, y, 1, o);
+            *reinterpret_cast<float4*>(sb.aux + q * 4) = make_float4(o[0], o[1], o[2], 0.f);
+        } else {
+            blur25(aux, 4, 0, w, h, x, y, 2, o);
+            const float4 b1 = *reinterpret_cast<const float4*>(sb.aux + q * 4);
+            float* f = sb.feat + q * kSurrIn;
+            const float4 c = *reinterpret_cast<const float4*>(f + 4);  // 4..7
+            const float4 d = *reinterpret_cast<const float4*>(f + 8);  // 8..11
+            const float base[3] = {c.w, d.x, d.y};
+            *reinterpret_cast<float4*>(f + 8) = make_float4(d.x, d.y, base[0] - b1.x, base[1] - b1.y);
+            *reinterpret_cast<float4*>(f + 12) = make_float4(base[2] - b1.z, b1.x - o[0], b1.y - o[1], b1.z - o[2]);
+        }
+    }
+}
+
+// Chunk flags start at 0 for chunks with a surrogate level and at 1 (plain
+// blend throughout) for chunks without one.
+__global__ void rscflag_init_kernel(const RChunk* __restrict__ chunks, int nch, int* __restrict__ cflag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nch) cflag[c] = chunks[c].sk >= 1 ? 0 : 1;
+}
+
+// One block per 16x16 surrogate tile: fall back when the mean predicted
+// squared error of the low band (10^out[3]) exceeds `thr`.
+__global__ void __launch_bounds__(256) rsflag_kernel(const SurrImg* __restrict__ imgs,
+                                                     const SurrTile* __restrict__ tiles, SurrBufs sb, float thr) {
+    __shared__ float ssum[256];
+    __shared__ int scnt[256];
+    const SurrTile tl = tiles[blockIdx.x];
+    const SurrImg im = imgs[tl.img];
+    const int x = tl.tx * 16 + (threadIdx.x & 15), y = tl.ty * 16 + (threadIdx.x >> 4);
+    float e = 0.f;
+    int n = 0;
+    if (x < im.w && y < im.h) {
+        const size_t q = static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x;
+        e = exp10f(fminf(sb.out[q * 4 + 3], 10.f));
+        n = 1;
+    }
+    ssum[threadIdx.x] = e;
+    scnt[threadIdx.x] = n;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + k];
+            scnt[threadIdx.x] += scnt[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int f = ssum[0] > thr * static_cast<float>(max(scnt[0], 1)) ? 1 : 0;
+        sb.tflag[blockIdx.x] = f;
+        if (f) atomicOr(&sb.cflag[im.tag], 1);
+    }
+}
+
+// recon_K = composite + residual on every tile that does not fall back.
+template <int MODE>
+__global__ void __launch_bounds__(256) rsapply_kernel(const RChunk* __restrict__ chunks,
+                                                      const RTile* __restrict__ tiles, int ntiles, SurrBufs sb) {
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    using M = typename Form<MM>::M;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RChunk& ch = chunks[tl.id];
+        const int l = ch.sk;
+        const int x = tl.tx * kCompTW + (threadIdx.x & 63), y = tl.ty * kCompTH + (threadIdx.x >> 6);
+        const int w = ch.lw[l], h = ch.lh[l];
+        if (x >= w || y >= h) continue;
+        if (sb.cflag[tl.id] && sb.tflag[ch.stile0 + (y >> 4) * ch.stx + (x >> 4)]) continue;
+        const size_t p = static_cast<size_t>(y) * w + x;
+        const size_t q = static_cast<size_t>(ch.spix0) + p;
+        const float* f = sb.feat + q * kSurrIn;
+        const float4 o = *reinterpret_cast<const float4*>(sb.out + q * 4);
+        const M v[4] = {f[7] + kSurrResScale * o.x, f[8] + kSurrResScale * o.y, f[9] + kSurrResScale * o.z, M(0)};
+        st4(ch.recon[l], p, v, static_cast<M*>(nullptr));
+    }
+}
+
+// Training-data export: the plain blend's level-K reconstruction.
+template <int MODE>
+__global__ void __launch_bounds__(256) rsexport_kernel(const RChunk* __restrict__ chunks,
+                                                       const RTile* __restrict__ tiles, int ntiles, SurrBufs sb) {
+    constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
+    using M = typename Form<MM>::M;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RChunk& ch = chunks[tl.id];
+        const int l = ch.sk;
+        const int x = tl.tx * kCompTW + (threadIdx.x & 63), y = tl.ty * kCompTH + (threadIdx.x >> 6);
+        const int w = ch.lw[l], h = ch.lh[l];
+        if (x >= w || y >= h) continue;
+        const size_t p = static_cast<size_t>(y) * w + x;
+        M r[4];
+        ld4(ch.recon[l], p, r, static_cast<M*>(nullptr));
+        float* t = sb.target + (static_cast<size_t>(ch.spix0) + p) * 3;
+        t[0] = r[0]; t[1] = r[1]; t[2] = r[2];
     }
 }
 
@@ -677,6 +1001,17 @@ struct SeamPair {
     int i, j, round, slot;
 };
 
+enum SurrMode : int { kSurrNone = 0, kSurrRender = 1, kSurrExport = 2 };
+
+// What a surrogate render (or export) adds to a batched render.
+struct SurrRun {
+    int mode = kSurrNone;
+    SurrBufs bufs{};
+    const uint16_t* weights = nullptr;
+    const float* biases = nullptr;
+    float threshold = 0.0f;  // mean predicted squared error above which a tile falls back
+};
+
 struct Plan {
     std::vector<RItem> items;
     std::vector<RChunk> chunks;
@@ -692,6 +1027,13 @@ struct Plan {
     int max_w = 0;
     int mode = 0;
     uint64_t key = 0;
+    // surrogate (batched form only)
+    int surr = kSurrNone;
+    int snpix = 0;                   // level-K pixels over all chunks
+    std::vector<SurrImg> simgs;      // one per chunk with a surrogate level
+    std::vector<SurrTile> stiles;    // 16x16 tiles of those images
+    Range sfeat;                     // 64x4 level-K tiles (RTile, id = chunk)
+    SurrBufs sbufs{};                // offsets until patched
 };
 
 inline size_t A256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
@@ -752,14 +1094,16 @@ uint64_t mix(uint64_t h, uint64_t v) {
 
 // Builds the plan (host tables + workspace layout); returns false on error.
 static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc, const int* contrib,
-                       int bands, int mode, Plan& P, Status& st) {
+                       int bands, int mode, Plan& P, Status& st, int surr = kSurrNone) {
     P = Plan{};
     P.mode = mode;
+    P.surr = (mode == kModeBatched) ? surr : kSurrNone;
     const bool batched = mode == kModeBatched;
     const size_t sS = (mode == kModeFP64) ? 8 : (mode == kModePlain ? 4 : 2);
     const size_t sM = (mode == kModeFP64) ? 8 : 4;
     uint64_t key = mix(0, static_cast<uint64_t>(mode));
     key = mix(key, static_cast<uint64_t>(bands));
+    key = mix(key, static_cast<uint64_t>(P.surr));
     // layout offsets are relative; absolute pointers are patched after allocation
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
@@ -791,6 +1135,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         ch.levels = ClampBands(cc.width, cc.height, bands);
         ch.lw[0] = cc.width; ch.lh[0] = cc.height;
         for (int l = 1; l < ch.levels; ++l) { ch.lw[l] = (ch.lw[l - 1] + 1) / 2; ch.lh[l] = (ch.lh[l - 1] + 1) / 2; }
+        ch.sk = (P.surr != kSurrNone && ch.levels >= 4) ? ch.levels - 3 : 0;
         ch.first = static_cast<int>(P.items.size());
         const size_t n = static_cast<size_t>(cc.width) * cc.height;
         std::vector<std::pair<int, int>> used;  // (frame, local)
@@ -885,6 +1230,31 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         P.max_w = std::max(P.max_w, cc.width);
         if (!batched) slot_max = std::max(slot_max, off);
     }
+    if (P.surr != kSurrNone) {
+        // level-K images of every chunk with a surrogate level, side by side
+        for (size_t c = 0; c < P.chunks.size(); ++c) {
+            RChunk& ch = P.chunks[c];
+            if (ch.sk < 1) continue;
+            const int w = ch.lw[ch.sk], h = ch.lh[ch.sk];
+            ch.spix0 = P.snpix;
+            ch.stile0 = static_cast<int>(P.stiles.size());
+            ch.stx = (w + 15) / 16;
+            const int img = static_cast<int>(P.simgs.size());
+            P.simgs.push_back(SurrImg{P.snpix, w, h, static_cast<int>(c)});
+            for (int ty = 0; ty < (h + 15) / 16; ++ty)
+                for (int tx = 0; tx < ch.stx; ++tx) P.stiles.push_back(SurrTile{img, tx, ty, 0});
+            P.snpix += w * h;
+        }
+        const size_t px = static_cast<size_t>(std::max(1, P.snpix));
+        auto mk = [&](size_t b) { return reinterpret_cast<float*>(take(b)); };
+        P.sbufs.feat = mk(px * kSurrIn * 4);
+        P.sbufs.aux = mk(px * 16);
+        P.sbufs.out = mk(px * 16);
+        P.sbufs.tmp0 = mk(px * kSurrHidden * 4);
+        P.sbufs.tmp1 = mk(px * kSurrHidden * 4);
+        P.sbufs.tflag = reinterpret_cast<int*>(take(4 * std::max<size_t>(1, P.stiles.size())));
+        P.sbufs.cflag = reinterpret_cast<int*>(take(4 * std::max<size_t>(1, P.chunks.size())));
+    }
     P.bytes = batched ? off : slot_max;
     P.key = key;
     // pointer patching happens in patch_plan once the workspace exists
@@ -959,6 +1329,12 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             if (l < ch.levels) add(c, 0, (ch.lw[l] - 1) / kCompTW, 0, (ch.lh[l] - 1) / kCompTH);
             P.comp[c][l].n = static_cast<int>(P.tiles.size()) - P.comp[c][l].off;
         }
+    P.sfeat.off = static_cast<int>(P.tiles.size());
+    for (int c = 0; c < nch; ++c) {
+        const RChunk& ch = P.chunks[c];
+        if (ch.sk >= 1) add(c, 0, (ch.lw[ch.sk] - 1) / kCompTW, 0, (ch.lh[ch.sk] - 1) / kCompTH);
+    }
+    P.sfeat.n = static_cast<int>(P.tiles.size()) - P.sfeat.off;
     return true;
 }
 
@@ -967,6 +1343,10 @@ static void patch_plan(Plan& P, uint8_t* base) {
         using T = std::remove_reference_t<decltype(p)>;
         p = reinterpret_cast<T>(base + reinterpret_cast<uintptr_t>(p));
     };
+    if (P.surr != kSurrNone) {
+        fix(P.sbufs.feat); fix(P.sbufs.aux); fix(P.sbufs.out); fix(P.sbufs.tmp0); fix(P.sbufs.tmp1);
+        fix(P.sbufs.tflag); fix(P.sbufs.cflag);
+    }
     for (auto& it : P.items) {
         fix(it.warped); fix(it.cov); fix(it.dist);
         for (int l = 1; l < kMaxLevels; ++l) { fix(it.nd[l]); fix(it.wt[l]); }
@@ -992,6 +1372,9 @@ struct DevTables {
     SeamWork<RgbaImg<double, double>>* seam64 = nullptr;
     SeamWork<RgbaImg<float, float>>* seam32 = nullptr;
     SeamWork<MapImg>* seamMap = nullptr;
+    SurrImg* simgs = nullptr;
+    SurrTile* stiles = nullptr;
+    int nst = 0;
     std::vector<std::vector<int>> round_chunks;  // seam round k -> chunk ids
     std::vector<int> round_off;                  // offset of round k in the seam table
 };
@@ -1068,7 +1451,8 @@ static unsigned persistent_grid(i360hip_ctx* ctx, int n) {
 
 // Enqueues the stages of chunk `c` (or of every chunk when c < 0).
 template <int MODE>
-static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, hipStream_t s, Status& st) {
+static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, hipStream_t s, Status& st,
+                    const SurrRun* sr = nullptr) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
@@ -1139,29 +1523,78 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
             }
         }
     }
-    // pyramids
+    // pyramids (with the surrogate: the levels up to K first, the network,
+    // the levels above K only for chunks that fall back, then the rest)
     int maxL = 1;
     for (int k = c0; k < c1; ++k) maxL = std::max(maxL, P.chunks[k].levels);
-    for (int l = 0; l + 1 < maxL; ++l) {
-        off = P.red[c0][l].off;
-        n = 0;
-        for (int k = c0; k < c1; ++k) n += P.red[k][l].n;
-        if (n == 0) continue;
-        const double bytes = static_cast<double>(n) * kRedTW * kRedTH * (4.0 * (4 * sS + sS) + (4 * sS + sS));
-        if (l == 0)
-            RL(st, "reduce_l0", bytes, rreduce_kernel<MODE, true><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l));
-        else
-            RL(st, "reduce", bytes, rreduce_kernel<MODE, false><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l));
+    const bool surr = sr && sr->mode == kSurrRender;
+    const bool exporting = sr && sr->mode == kSurrExport;
+    const int* cflag = sr ? sr->bufs.cflag : nullptr;
+    auto reduces = [&](int phase) {
+        for (int l = 0; l + 1 < maxL; ++l) {
+            off = P.red[c0][l].off;
+            n = 0;
+            for (int k = c0; k < c1; ++k) n += P.red[k][l].n;
+            if (n == 0) continue;
+            const double bytes = static_cast<double>(n) * kRedTW * kRedTH * (4.0 * (4 * sS + sS) + (4 * sS + sS));
+            if (l == 0)
+                RL(st, "reduce_l0", bytes, rreduce_kernel<MODE, true><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
+            else
+                RL(st, "reduce", bytes, rreduce_kernel<MODE, false><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
+        }
+    };
+    auto composes = [&](int phase) {
+        for (int l = maxL - 1; l >= 1; --l) {
+            off = P.comp[c0][l].off;
+            n = 0;
+            for (int k = c0; k < c1; ++k) n += P.comp[k][l].n;
+            if (n == 0) continue;
+            const double bytes = static_cast<double>(n) * kCompTW * kCompTH * (4 * sM + 2.0 * (4 * sS + sS));
+            RL(st, "compose_level", bytes,
+               rcompose_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
+        }
+    };
+    if (surr) {
+        const int nch = static_cast<int>(P.chunks.size());
+        RL(st, "surrogate_flags", 0.0, rscflag_init_kernel<<<(nch + 255) / 256, 256, 0, s>>>(T.chunks, nch, sr->bufs.cflag));
     }
-    for (int l = maxL - 1; l >= 1; --l) {
-        off = P.comp[c0][l].off;
-        n = 0;
-        for (int k = c0; k < c1; ++k) n += P.comp[k][l].n;
-        if (n == 0) continue;
-        const double bytes = static_cast<double>(n) * kCompTW * kCompTH * (4 * sM + 2.0 * (4 * sS + sS));
-        RL(st, "compose_level", bytes,
-           rcompose_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l));
+    reduces(surr ? 1 : 0);
+    if constexpr (MODE == kModeBatched) if (sr && P.sfeat.n > 0) {
+        const int fo = P.sfeat.off, fn = P.sfeat.n;
+        const double px = static_cast<double>(P.snpix);
+        RL(st, "surrogate_features", px * 64.0,
+           rsfeat_kernel<MODE><<<persistent_grid(ctx, fn), 256, 0, s>>>(T.items, T.chunks, T.tiles + fo, fn, sr->bufs));
+        RL(st, "surrogate_blur", px * 80.0,
+           rsblur_kernel<<<persistent_grid(ctx, fn), 256, 0, s>>>(T.chunks, T.tiles + fo, fn, sr->bufs, 0));
+        RL(st, "surrogate_blur", px * 96.0,
+           rsblur_kernel<<<persistent_grid(ctx, fn), 256, 0, s>>>(T.chunks, T.tiles + fo, fn, sr->bufs, 1));
+        if (surr) {
+            {
+                ProfScope ps_(ctx, s, "surrogate_network", 0.0);
+                if (surr_forward_tables(T.simgs, T.stiles, T.nst, sr->weights, sr->biases, sr->bufs.feat, sr->bufs.out,
+                                        sr->bufs.tmp0, sr->bufs.tmp1, s) != hipSuccess) {
+                    st.set(I360HIP_KERNEL_LAUNCH_FAILED, "surrogate network launch failed");
+                    return;
+                }
+            }
+            RL(st, "surrogate_flags", px * 16.0,
+               rsflag_kernel<<<T.nst, 256, 0, s>>>(T.simgs, T.stiles, sr->bufs, sr->threshold));
+        }
     }
+    if (surr) {
+        reduces(2);
+        composes(2);
+        if constexpr (MODE == kModeBatched) if (P.sfeat.n > 0)
+            RL(st, "surrogate_apply", static_cast<double>(P.snpix) * 48.0,
+               rsapply_kernel<MODE><<<persistent_grid(ctx, P.sfeat.n), 256, 0, s>>>(T.chunks, T.tiles + P.sfeat.off, P.sfeat.n, sr->bufs));
+        composes(1);
+    } else {
+        composes(0);
+    }
+    if constexpr (MODE == kModeBatched) if (exporting && P.sfeat.n > 0)
+        RL(st, "surrogate_export", static_cast<double>(P.snpix) * 28.0,
+           rsexport_kernel<MODE><<<persistent_grid(ctx, P.sfeat.n), 256, 0, s>>>(T.chunks, T.tiles + P.sfeat.off, P.sfeat.n, sr->bufs));
+    if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
     RL(st, "final_level0", area0 * (4.0 + 4 * sS + 12.0),
        rfinal_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
@@ -1206,6 +1639,8 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     const size_t oW = take(sizeof(ChamferWork) * std::max<size_t>(1, P.chamfer.size()));
     const size_t oS = take(std::max({sizeof(s64[0]) * s64.size(), sizeof(s32[0]) * s32.size(),
                                      sizeof(s16[0]) * s16.size(), size_t(256)}));
+    const size_t oSI = take(sizeof(SurrImg) * std::max<size_t>(1, P.simgs.size()));
+    const size_t oST = take(sizeof(SurrTile) * std::max<size_t>(1, P.stiles.size()));
     std::vector<uint8_t> host(off);
     std::memcpy(host.data() + oI, P.items.data(), sizeof(RItem) * P.items.size());
     std::memcpy(host.data() + oC, P.chunks.data(), sizeof(RChunk) * P.chunks.size());
@@ -1214,6 +1649,8 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     if (!s64.empty()) std::memcpy(host.data() + oS, s64.data(), sizeof(s64[0]) * s64.size());
     if (!s32.empty()) std::memcpy(host.data() + oS, s32.data(), sizeof(s32[0]) * s32.size());
     if (!s16.empty()) std::memcpy(host.data() + oS, s16.data(), sizeof(s16[0]) * s16.size());
+    if (!P.simgs.empty()) std::memcpy(host.data() + oSI, P.simgs.data(), sizeof(SurrImg) * P.simgs.size());
+    if (!P.stiles.empty()) std::memcpy(host.data() + oST, P.stiles.data(), sizeof(SurrTile) * P.stiles.size());
     auto* dev = static_cast<uint8_t*>(ctx_scratch(ctx, slot, off, st));
     if (!st.ok()) return;
     if (hipMemcpyAsync(dev, host.data(), off, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1228,14 +1665,30 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     T.seam64 = reinterpret_cast<decltype(T.seam64)>(dev + oS);
     T.seam32 = reinterpret_cast<decltype(T.seam32)>(dev + oS);
     T.seamMap = reinterpret_cast<decltype(T.seamMap)>(dev + oS);
+    T.simgs = reinterpret_cast<SurrImg*>(dev + oSI);
+    T.stiles = reinterpret_cast<SurrTile*>(dev + oST);
+    T.nst = static_cast<int>(P.stiles.size());
 }
 
 static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc,
-                       const int* contrib, int bands, int mode, hipStream_t s, Status& st) {
+                       const int* contrib, int bands, int api_mode, hipStream_t s, Status& st,
+                       float* export_feat = nullptr, float* export_target = nullptr) {
     RenderCache& rc = cache_of(ctx);
+    const int surr = export_feat ? kSurrExport : (api_mode == I360HIP_RENDER_SURROGATE ? kSurrRender : kSurrNone);
+    const int mode = surr != kSurrNone ? kModeBatched : api_mode;
+    if (surr == kSurrRender && !ctx->surr_weights) {
+        st.set(I360HIP_INVALID_PARAM, "the SURROGATE form needs parameters (i360hip_surrogate_set)");
+        return;
+    }
     Plan P;
-    if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st)) return;
-    const bool reuse = !ctx->profiling && rc.graph && rc.key == P.key && rc.mode == mode && mode == kModeBatched;
+    if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st, surr)) return;
+    SurrRun sr;
+    sr.mode = surr;
+    sr.weights = ctx->surr_weights;
+    sr.biases = ctx->surr_biases;
+    sr.threshold = ctx->surr_threshold;
+    const bool graphable = mode == kModeBatched && surr != kSurrExport;
+    const bool reuse = !ctx->profiling && rc.graph && rc.key == P.key && rc.mode == api_mode && graphable;
     if (reuse) {
         if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
         return;
@@ -1251,9 +1704,11 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     if (!st.ok()) return;
     rc.plan = std::move(P);
     rc.key = rc.plan.key;
-    rc.mode = mode;
+    rc.mode = api_mode;
     const Plan& Q = rc.plan;
     const DevTables& T = rc.tables;
+    sr.bufs = Q.sbufs;
+    sr.bufs.target = export_target;
     const int ni = static_cast<int>(Q.items.size());
     auto body = [&](hipStream_t ss) {
         if (ni > 0) rmap_kernel<<<(ni + 63) / 64, 64, 0, ss>>>(T.items, ni);
@@ -1268,11 +1723,19 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
                 for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModePrecision>(ctx, Q, T, c, ss, st);
                 break;
             default:
-                if (!Q.chunks.empty()) enqueue<kModeBatched>(ctx, Q, T, -1, ss, st);
+                if (!Q.chunks.empty()) enqueue<kModeBatched>(ctx, Q, T, -1, ss, st, surr != kSurrNone ? &sr : nullptr);
                 break;
         }
     };
-    if (mode != kModeBatched || ctx->profiling) {
+    if (surr == kSurrExport) {
+        body(s);
+        if (st.ok() && Q.snpix > 0 &&
+            hipMemcpyAsync(export_feat, Q.sbufs.feat, sizeof(float) * kSurrIn * static_cast<size_t>(Q.snpix),
+                           hipMemcpyDeviceToDevice, s) != hipSuccess)
+            st.set(I360HIP_DEVICE_ERROR, "surrogate feature copy failed");
+        return;
+    }
+    if (!graphable || ctx->profiling) {
         body(s);
         return;
     }
@@ -1313,7 +1776,7 @@ int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num
         std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument.");
         return I360HIP_INVALID_PARAM;
     }
-    if (num_bands < 1 || num_bands > I360HIP_MAX_BANDS || mode < I360HIP_RENDER_FP64 || mode > I360HIP_RENDER_BATCHED) {
+    if (num_bands < 1 || num_bands > I360HIP_MAX_BANDS || mode < I360HIP_RENDER_FP64 || mode > I360HIP_RENDER_SURROGATE) {
         std::snprintf(ctx->last_error, sizeof(ctx->last_error), "numBands must be 1-12 and mode a known form.");
         return I360HIP_INVALID_PARAM;
     }
@@ -1367,6 +1830,105 @@ int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int 
     float f;
     std::memcpy(&f, &hb, sizeof(f));
     *max_error_px = static_cast<double>(f);
+    return I360HIP_SUCCESS;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int i360hip_surrogate_set(i360hip_ctx* ctx, const uint16_t* weights, const float* biases, float threshold_mse) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!weights || !biases || !(threshold_mse >= 0.0f)) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument or negative threshold.");
+        return I360HIP_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Status st;
+    if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
+    constexpr int kSlotSurrParams = 18;
+    const size_t wb = A256(2 * static_cast<size_t>(I360HIP_SURR_WEIGHTS));
+    auto* dev = static_cast<uint8_t*>(ctx_scratch(ctx, kSlotSurrParams, wb + 4 * I360HIP_SURR_BIASES, st));
+    if (!st.ok()) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        return st.code;
+    }
+    if (hipMemcpyAsync(dev, weights, 2 * static_cast<size_t>(I360HIP_SURR_WEIGHTS), hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(dev + wb, biases, 4 * static_cast<size_t>(I360HIP_SURR_BIASES), hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "surrogate parameter copy failed");
+        return I360HIP_DEVICE_ERROR;
+    }
+    ctx->surr_weights = reinterpret_cast<const uint16_t*>(dev);
+    ctx->surr_biases = reinterpret_cast<const float*>(dev + wb);
+    if (ctx->surr_threshold != threshold_mse) {
+        // the threshold is a kernel argument of a captured graph
+        RenderCache& rc = cache_of(ctx);
+        if (rc.graph) (void)hipGraphExecDestroy(rc.graph);
+        rc.graph = nullptr;
+        ctx->surr_threshold = threshold_mse;
+    }
+    return I360HIP_SUCCESS;
+}
+
+int i360hip_surrogate_chunk_level(int width, int height, int num_bands, int* level, int* level_width,
+                                  int* level_height) {
+    if (!level || !level_width || !level_height || width <= 0 || height <= 0 || num_bands < 1 ||
+        num_bands > I360HIP_MAX_BANDS)
+        return I360HIP_INVALID_PARAM;
+    const int L = ClampBands(width, height, num_bands);
+    int w = width, h = height;
+    const int K = L >= 4 ? L - 3 : 0;
+    for (int l = 0; l < K; ++l) { w = (w + 1) / 2; h = (h + 1) / 2; }
+    *level = K;
+    *level_width = K ? w : 0;
+    *level_height = K ? h : 0;
+    return I360HIP_SUCCESS;
+}
+
+int i360hip_surrogate_features(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                               const i360hip_chunk* chunks, int num_chunks, const int* contributors, int num_bands,
+                               float* feat, float* target, void* stream) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!feat || !target || (!frames && num_frames > 0) || (!chunks && num_chunks > 0) ||
+        (!contributors && num_chunks > 0) || num_bands < 1 || num_bands > I360HIP_MAX_BANDS) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument or numBands outside 1-12.");
+        return I360HIP_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Status st;
+    if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    run_render(ctx, frames, num_frames, chunks, num_chunks, contributors, num_bands, I360HIP_RENDER_BATCHED, s, st,
+               feat, target);
+    if (!st.ok()) std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+    return st.code;
+}
+
+int i360hip_surrogate_stats(i360hip_ctx* ctx, int* tiles, int* fallback_tiles) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!tiles || !fallback_tiles) return I360HIP_INVALID_PARAM;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    RenderCache& rc = cache_of(ctx);
+    const Plan& P = rc.plan;
+    if (P.surr != kSurrRender || P.stiles.empty()) {
+        *tiles = 0;
+        *fallback_tiles = 0;
+        return I360HIP_SUCCESS;
+    }
+    std::vector<int> tf(P.stiles.size()), cf(P.chunks.size());
+    if (hipSetDevice(ctx->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(tf.data(), P.sbufs.tflag, 4 * tf.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(cf.data(), P.sbufs.cflag, 4 * cf.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "surrogate flag read-back failed");
+        return I360HIP_DEVICE_ERROR;
+    }
+    int fb = 0;
+    for (size_t t = 0; t < tf.size(); ++t) fb += tf[t] ? 1 : 0;
+    *tiles = static_cast<int>(tf.size());
+    *fallback_tiles = fb;
     return I360HIP_SUCCESS;
 }
 

@@ -157,7 +157,7 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
     const float cI = rowScan ? static_cast<float>(bI[0] + bI[2]) * 0.5f : static_cast<float>(bI[1] + bI[3]) * 0.5f;
     const float cJ = rowScan ? static_cast<float>(bJ[0] + bJ[2]) * 0.5f : static_cast<float>(bJ[1] + bJ[3]) * 0.5f;
     const bool iLow = cI <= cJ;
-    const int lines = rowScan ? bh : bw, len = rowScan ? bw : bh;
+    const int lines = rowScan ? bh : bw;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
 
     // Phase 1 (crossover and band of every line) ran in seam_scan_kernel,

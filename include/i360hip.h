@@ -160,6 +160,14 @@ int i360hip_blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int
  *             another;
  *   BATCHED   PRECISION arithmetic, every chunk of the call in one
  *             persistent launch per stage, replayed from a HIP graph.
+ *   SURROGATE BATCHED, with the learned surrogate (i360hip_surrogate_set)
+ *             predicting each chunk's low band at pyramid level
+ *             K = levels-3 in place of the levels K .. levels-1 of the
+ *             multi-band compose; every 16x16 level-K tile whose predicted
+ *             error exceeds the threshold falls back to the plain blend
+ *             (chunks without such a tile skip the reduce and compose of
+ *             the levels above K).  Chunks with fewer than 4 levels are
+ *             blended plainly.
  * Frames: interleaved RGB (3 values per pixel), FP16 or FP32, device
  * memory.  Chunks name their contributors by index into `frames` through
  * contributors[first .. first+count-1] (at most 32 are used, in order).
@@ -172,7 +180,8 @@ enum {
     I360HIP_RENDER_FP64 = 0,
     I360HIP_RENDER_PLAIN = 1,
     I360HIP_RENDER_PRECISION = 2,
-    I360HIP_RENDER_BATCHED = 3
+    I360HIP_RENDER_BATCHED = 3,
+    I360HIP_RENDER_SURROGATE = 4
 };
 
 typedef struct i360hip_frame {
@@ -201,6 +210,71 @@ int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num
 int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
                              const i360hip_chunk* chunks, int num_chunks, const int* contributors,
                              double* max_error_px);
+
+/* ---- learned blend surrogate ----------------------------------------------
+ * A 6-layer 3x3 convolutional network, 16 -> 32 -> 32 -> 32 -> 32 -> 32 -> 4
+ * channels, ReLU between layers, zero padding, BF16 weights and FP32
+ * activations / accumulation.  It predicts, at pyramid level K = bands-3 of
+ * a chunk, the blended low band of the multi-band compose from per-pixel
+ * features (the two largest-weight contributors' level-K Gaussians, the
+ * seam mask, the single-level composite and two Laplacian scales of it);
+ * output channels 0-2 are the residual to the composite (times 1/64), 3 the
+ * predicted log10 of its own mean squared error.  No counterpart in the
+ * reference (an ONNX model is planned there, docs/COMPUTATIONAL_PHOTOGRAPHY.md:
+ * 328-360).
+ *
+ * Parameters: weights, per layer HWIO ([ky][kx][cin][cout]), layers back to
+ * back (I360HIP_SURR_WEIGHTS values); biases, per layer [cout]
+ * (I360HIP_SURR_BIASES floats).  Features / targets: NHWC FP32 batches of n
+ * images of h x w.  All pointers are device pointers; calls are
+ * asynchronous on `stream` (NULL = the context's stream). */
+#define I360HIP_SURR_LAYERS 6
+#define I360HIP_SURR_IN 16
+#define I360HIP_SURR_HIDDEN 32
+#define I360HIP_SURR_OUT 4
+#define I360HIP_SURR_WEIGHTS 42624 /* 9*(16*32 + 4*32*32 + 32*4) */
+#define I360HIP_SURR_BIASES 164    /* 5*32 + 4 */
+
+/* out: n*h*w*4 floats. */
+int i360hip_surrogate_forward(i360hip_ctx* ctx, const uint16_t* weights, const float* biases, const float* feat,
+                              int n, int h, int w, float* out, void* stream);
+/* Training loss (scalar) and its gradients for targets `target` (n*h*w*3,
+ * the blended low band):
+ *   L = mean_p [ 1e4 * mean_c d_c^2 + 1e-2 * (o_3 - log10(mean_c d_c^2 + 1e-10))^2 ],
+ *   d = o_{0..2}/64 - (target - composite), composite = feat channels 7-9;
+ * the error term's target is held constant. */
+int i360hip_surrogate_gradients(i360hip_ctx* ctx, const uint16_t* weights, const float* biases, const float* feat,
+                                const float* target, int n, int h, int w, float* grad_weights, float* grad_biases,
+                                float* loss, void* stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected at `step` >= 1)
+ * on FP32 master weights and biases; refreshes the BF16 weights.
+ * adam_state: 2*(I360HIP_SURR_WEIGHTS + I360HIP_SURR_BIASES) floats (first
+ * and second moments of the weights, then of the biases), zero at step 1. */
+int i360hip_surrogate_train_step(i360hip_ctx* ctx, float* master_weights, float* biases, float* adam_state,
+                                 uint16_t* weights, const float* feat, const float* target, int n, int h, int w,
+                                 float lr, int step, float* loss, void* stream);
+/* BF16 (round to nearest even) copy of FP32 master weights. */
+int i360hip_surrogate_round_bf16(i360hip_ctx* ctx, const float* master_weights, uint16_t* weights, void* stream);
+
+/* Parameters of the SURROGATE render form (device pointers, copied; the
+ * threshold is a mean squared error per 16x16 level-K tile: 1e-5 is the
+ * 50 dB PSNR equivalent on [0,1] RGB). */
+int i360hip_surrogate_set(i360hip_ctx* ctx, const uint16_t* weights, const float* biases, float threshold_mse);
+/* Level K the surrogate works at for a chunk of width x height at
+ * num_bands (0 when the chunk has fewer than 4 levels) and its size. */
+int i360hip_surrogate_chunk_level(int width, int height, int num_bands, int* level, int* level_width,
+                                  int* level_height);
+/* Training data from the plain multi-band blend (BATCHED arithmetic): for
+ * every chunk with a surrogate level, in order, its level-K features
+ * (level_width*level_height*16 floats into `feat`) and the blend's level-K
+ * reconstruction (level_width*level_height*3 floats into `target`),
+ * images back to back.  The chunks' `out` and `owner` are not written (nor
+ * read).  Device pointers; asynchronous on `stream`. */
+int i360hip_surrogate_features(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                               const i360hip_chunk* chunks, int num_chunks, const int* contributors, int num_bands,
+                               float* feat, float* target, void* stream);
+/* Tiles of the last SURROGATE render and how many fell back; synchronous. */
+int i360hip_surrogate_stats(i360hip_ctx* ctx, int* tiles, int* fallback_tiles);
 
 /* ---- device-pointer forms (asynchronous on `stream`) ------------------- */
 

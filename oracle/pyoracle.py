@@ -283,3 +283,38 @@ def ref_multiband_compose(warped, owner, num_bands):
     ref().ref_multiband_compose(_ptr_array(warped), _ptr_array(dists, f32p), _p(owner, i32p), len(warped), w, h,
                                 num_bands, _p(out, u16p))
     return out
+
+
+# ------------------------------------------------------------- surrogate --
+SURR_WEIGHTS = 42624
+SURR_BIASES = 164
+
+
+def surr_forward(w, b, feat):
+    """Network output (n, h, w, 4) in binary64, rounded to float."""
+    w = np.ascontiguousarray(w, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    feat = np.ascontiguousarray(feat, np.float32)
+    n, h, wd, _ = feat.shape
+    out = np.zeros((n, h, wd, 4), np.float32)
+    lib().orc_surr_forward(_p(w, f32p), _p(b, f32p), _p(feat, f32p), n, h, wd, _p(out, f32p))
+    return out
+
+
+def surr_loss_grad(w, b, feat, target, le_in=None, want_le=False):
+    """(loss, grad_w, grad_b[, le]) in binary64; le_in freezes the error
+    term's target per pixel."""
+    w = np.ascontiguousarray(w, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    feat = np.ascontiguousarray(feat, np.float32)
+    target = np.ascontiguousarray(target, np.float32)
+    n, h, wd, _ = feat.shape
+    gw = np.zeros(SURR_WEIGHTS, np.float64)
+    gb = np.zeros(SURR_BIASES, np.float64)
+    loss = np.zeros(1, np.float64)
+    le = np.zeros(n * h * wd, np.float64)
+    lin = None if le_in is None else np.ascontiguousarray(le_in, np.float64)
+    lib().orc_surr_loss_grad(_p(w, f32p), _p(b, f32p), _p(feat, f32p), _p(target, f32p), n, h, wd,
+                             _p(gw, f64p), _p(gb, f64p), _p(loss, f64p),
+                             None if lin is None else _p(lin, f64p), _p(le, f64p))
+    return (float(loss[0]), gw, gb, le) if want_le else (float(loss[0]), gw, gb)
