@@ -47,7 +47,10 @@ def parse():
     ap.add_argument("--psnr-chunks", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
-    ap.add_argument("--forms", default="plain,precision,batched")
+    ap.add_argument("--forms", default="plain,precision,batched,surrogate")
+    ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "image360_amd", "surrogate_weights.npz"))
+    ap.add_argument("--surrogate-threshold", type=float, default=1e-5,
+                    help="mean predicted squared error per tile above which the surrogate falls back (1e-5 = 50 dB)")
     return ap.parse_args()
 
 
@@ -107,6 +110,38 @@ def cpu_baseline(geom, frames_host_fn, chunks, threads, bands):
                       f"{[len(c.contributors) for c in chunks]}) of the same canvas, one chunk per task on "
                       f"{threads} threads, {tiers.get(tier, tier)} tier, RGB48 frames, reference "
                       f"BlendChunkContributors pipeline with {bands} bands; {dt:.1f} s wall"}
+
+
+def load_surrogate(hip, args, log):
+    """Sets the surrogate's parameters from the trained file, or trains a
+    small network on synthetic panoramas when the file is absent.  Returns
+    a description of where the parameters came from."""
+    import torch
+
+    from image360_amd import surrogate as sg
+
+    if os.path.exists(args.surrogate_weights):
+        bits, biases = sg.load_params(args.surrogate_weights)
+        src = os.path.relpath(args.surrogate_weights, ROOT)
+    else:
+        log("surrogate: no trained parameters; training a small network (400 steps on 3 panoramas)")
+        fs, ts = [], []
+        for seed in (21, 22, 23):
+            g, frames = sg.panorama(seed, rows=2, cols=3, fw=1200, fh=800, focal=1000.0, chunk=1024)
+            fd, cd, ct, sizes = sg.describe(g, frames)
+            f, t = sg.level_patches(hip, fd, cd, ct, args.bands, sizes, patch=16)
+            fs.append(f)
+            ts.append(t)
+        tr = sg.Trainer(hip, seed=4)
+        tr.train(torch.cat(fs), torch.cat(ts), 400, batch=32, lr=2e-3, lr_end=2e-4)
+        bits = tr.weights.cpu().numpy().view(np.uint16)
+        biases = tr.bias.cpu().numpy()
+        src = "trained in this run (400 steps)"
+    wt = torch.from_numpy(bits.view(np.int16)).cuda()
+    bt = torch.from_numpy(biases).cuda()
+    torch.cuda.synchronize()
+    hip.check(hip.surrogate_set(wt.data_ptr(), bt.data_ptr(), args.surrogate_threshold), "surrogate_set")
+    return src
 
 
 def main():
@@ -174,16 +209,27 @@ def main():
 
     forms = {}
     wanted = set(args.forms.split(","))
+    surr_src = None
+    if "surrogate" in wanted:
+        surr_src = load_surrogate(hip, args, log)
+    outputs = {}
     for name, mode, steps in (("plain", _abi.RENDER_PLAIN, 1), ("precision", _abi.RENDER_PRECISION, 1),
-                              ("batched", _abi.RENDER_BATCHED, args.steps)):
+                              ("batched", _abi.RENDER_BATCHED, args.steps),
+                              ("surrogate", _abi.RENDER_SURROGATE, args.steps)):
         if name not in wanted and name != "batched":
+            continue
+        if name == "surrogate" and surr_src is None:
             continue
         ms = timed(mode, steps, args.warmup)
         forms[name] = {"mp_s": round(out_px * steps / (ms / 1e3) / 1e6, 1), "ms_per_step": round(ms / steps, 2),
                        "steps": steps}
+        if name == "surrogate":
+            tiles, fb = hip.surrogate_stats()
+            forms[name].update({"tiles": tiles, "fallback_tiles": fb, "weights": surr_src})
+            log(f"surrogate : {fb}/{tiles} level-K tiles fell back to the plain blend")
         log(f"{name:10s}: {ms / steps:9.2f} ms/step  {forms[name]['mp_s']:9.1f} MP/s")
-    head = forms["batched"]
-    batched_out = out.clone()
+        if name in ("batched", "surrogate"):
+            outputs[name] = out.clone()
 
     # per-kernel event timing of one eager step of the batched launch sequence
     hip.profile_enable(True)
@@ -207,9 +253,9 @@ def main():
                 "latency_bound_share": round(sum(v["total_ms"] for k, v in stats.items()
                                                  if v["alg_bytes"] == 0) / tot, 3)}
 
-    # accuracy: FP64 form on a sample of chunks vs the batched output
+    # accuracy: FP64 form on a sample of chunks vs the batched / surrogate outputs
     sample = sorted(range(len(chunks)), key=lambda k: (-len(chunks[k].contributors), -chunks[k].area))[:args.psnr_chunks]
-    sse, cnt, maxerr = 0.0, 0, 0.0
+    acc = {name: [0.0, 0, 0.0] for name in outputs}
     ref_out = torch.empty(max([chunks[k].area * 3 for k in sample] + [1]), dtype=torch.float32, device=dev)
     for k in sample:
         c = chunks[k]
@@ -220,17 +266,25 @@ def main():
             render_args = ([d], c.contributors)
             hip.check(hip.render_chunks(fdesc, render_args[0], render_args[1], args.bands, _abi.RENDER_FP64, sh), "fp64")
         torch.cuda.synchronize()
-        a = batched_out[int(offs[k]): int(offs[k + 1])].double()
         b = ref_out[: c.area * 3].double()
-        e = (a - b)
-        sse += float((e * e).sum())
-        cnt += e.numel()
-        maxerr = max(maxerr, float(e.abs().max()))
-    mse = sse / max(cnt, 1)
-    psnr = float("inf") if mse == 0 else 10 * np.log10(1.0 / mse)
+        for name, o in outputs.items():
+            e = o[int(offs[k]): int(offs[k + 1])].double() - b
+            acc[name][0] += float((e * e).sum())
+            acc[name][1] += e.numel()
+            acc[name][2] = max(acc[name][2], float(e.abs().max()))
+    for name, (sse, cnt, maxerr) in acc.items():
+        mse = sse / max(cnt, 1)
+        forms[name]["psnr_db_vs_fp64"] = round(float("inf") if mse == 0 else 10 * np.log10(1.0 / mse), 2)
+        forms[name]["max_abs_err_vs_fp64"] = maxerr
+        log(f"{name} vs FP64 on {len(sample)} chunks: PSNR {forms[name]['psnr_db_vs_fp64']:.2f} dB, "
+            f"max abs {maxerr:.2e}")
     map_err = hip.render_map_error(fdesc, cdesc, contrib)
-    log(f"batched vs FP64 on {len(sample)} chunks: PSNR {psnr:.2f} dB, max abs {maxerr:.2e}; "
-        f"FP32 tile map vs FP64 map: {map_err:.2e} px")
+    log(f"FP32 tile map vs FP64 map: {map_err:.2e} px")
+    # headline: the fastest of the batched / surrogate forms that holds 50 dB
+    ok = [n for n in ("batched", "surrogate") if n in forms and forms[n].get("psnr_db_vs_fp64", 0) >= 50.0]
+    head_name = max(ok, key=lambda n: forms[n]["mp_s"]) if ok else "batched"
+    head = forms[head_name]
+    psnr, maxerr = head.get("psnr_db_vs_fp64"), head.get("max_abs_err_vs_fp64")
 
     cpu = None
     if not args.no_cpu:
@@ -255,11 +309,13 @@ def main():
         "config": {"workload": "gigapixel panorama render: 48 FP16 RGB frames of 6000x4000 (6x8 grid, 25% overlap, "
                                "focal 5000 px, +-2 deg yaw/pitch jitter, FP64 homographies), every 4096^2 chunk of "
                                "the canvas, warp + chamfer ownership + DP seams + 7-level Laplacian blend",
-                   "form": "batched (persistent per-stage launches over all chunks, HIP graph)",
+                   "form": head_name + (" (persistent per-stage launches over all chunks, HIP graph)"
+                                        if head_name == "batched" else
+                                        " (batched + learned low band at level bands-3, per-tile fallback)"),
                    "bands": args.bands, "chunks": len(chunks), "output_mp": round(out_px / 1e6, 1),
                    "canvas": [geom.canvas_w, geom.canvas_h],
-                   "forms_mp_s": {k: v["mp_s"] for k, v in forms.items()},
-                   "psnr_db_vs_fp64": round(psnr, 2), "max_abs_err_vs_fp64": maxerr,
+                   "forms": forms,
+                   "psnr_db_vs_fp64": psnr, "max_abs_err_vs_fp64": maxerr,
                    "psnr_sample_chunks": len(sample), "map_max_err_px": map_err,
                    "speedup_vs_plain": round(head["mp_s"] / forms["plain"]["mp_s"], 2) if "plain" in forms else None},
         "roofline": roofline, "cpu_baseline": cpu,

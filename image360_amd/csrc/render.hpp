@@ -28,6 +28,7 @@ struct RItem {
     float hinv32[9];  // binary32 inverse of the binary32 forward matrix (plain form)
     int chunk;        // chunk index
     int local;        // index within the chunk's contributor list
+    int wtile0, wtilen;  // this item's warp tiles (global tile indices)
     void* warped;     // chunk-sized RGBA (coverage in A), storage type of the form
     uint8_t* cov;     // chunk-sized coverage bits
     float* dist;      // chunk-sized chamfer distances (inside the coverage box)

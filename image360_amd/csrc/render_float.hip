@@ -307,32 +307,44 @@ __device__ __forceinline__ void decode_pair_f16(const u4a& q, uint32_t sh, const
     b[2] = static_cast<M>(static_cast<float>(d2.y));
 }
 
-// Warp of one item's tile: every thread maps its 4 pixels first, then
-// issues all 8 texel-row loads (FP16 frames: unconditional, clamped
-// addresses), then blends and stores, so the loads of a thread are in
-// flight together instead of one pixel after another.
+// Per-tile FP32 maps of the precision forms, one thread per warp tile
+// (the FP64 reduction runs fully parallel here instead of serially inside
+// the warp's tile loop).
+__global__ void __launch_bounds__(256) rtilemap_kernel(const RItem* __restrict__ items,
+                                                       const RChunk* __restrict__ chunks,
+                                                       const RTile* __restrict__ tiles, int ntiles,
+                                                       TileMap* __restrict__ maps) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntiles) return;
+    const RTile tl = tiles[e];
+    const RItem& it = items[tl.id];
+    const RChunk& ch = chunks[it.chunk];
+    maps[e] = tile_map(it.hinv, static_cast<double>(ch.x + tl.tx * kWarpTW), static_cast<double>(ch.y + tl.ty * kWarpTH));
+}
+
+// Warp of one item's tile.  Each wavefront owns 64x4 pixels and works
+// alone (no LDS, no barrier): it maps its pixels (precision forms: from the
+// tile's precomputed FP32 map), issues all 8 texel-row loads of a thread
+// together (FP16 frames: unconditional, clamped addresses), blends,
+// stores, and writes its partial coverage box (min/max over the wave by
+// lane exchange) for rbox_reduce_kernel.
 template <int MODE>
 __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ items,
                                                     const RChunk* __restrict__ chunks,
-                                                    const RTile* __restrict__ tiles, int ntiles) {
+                                                    const RTile* __restrict__ tiles, int ntiles,
+                                                    const TileMap* __restrict__ maps, int4* __restrict__ wbox) {
     using M = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::M;
     using S = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::S;
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     constexpr int R = 4;
-    __shared__ TileMap smap;
-    __shared__ int sbox[4];
+    const int wave = threadIdx.x >> 6;
     for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
         const RTile tl = tiles[e];
         const RItem& it = items[tl.id];
         const RChunk& ch = chunks[it.chunk];
         const int tx0 = tl.tx * kWarpTW, ty0 = tl.ty * kWarpTH;
-        if (threadIdx.x == 0) {
-            if (MM == kModePrecision)
-                smap = tile_map(it.hinv, static_cast<double>(ch.x + tx0), static_cast<double>(ch.y + ty0));
-            sbox[0] = INT_MAX; sbox[1] = INT_MAX; sbox[2] = -1; sbox[3] = -1;
-        }
-        __syncthreads();
-        const TileMap tm = smap;
+        TileMap tm{};
+        if (MM == kModePrecision) tm = maps[e];
         const int* fb = it.lbox[0];
         const void* src = it.src;
         const int fw = it.fw, fmt = it.fmt;
@@ -347,7 +359,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
         M fx[R], fy[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int ly = (threadIdx.x >> 6) * R + r;
+            const int ly = wave * R + r;
             const int y = ty0 + ly;
             in[r] = x < ch.w && y < ch.h && in_box(fb, x, y);
             int sx0 = 0, sy0 = 0;
@@ -385,7 +397,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!in[r]) continue;
-            const int y = ty0 + (threadIdx.x >> 6) * R + r;
+            const int y = ty0 + wave * R + r;
             const size_t p = static_cast<size_t>(y) * ch.w + x;
             M v[4] = {M(0), M(0), M(0), M(0)};
             if (ok[r]) {
@@ -403,13 +415,50 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
             st4(it.warped, p, v, static_cast<S*>(nullptr));
             gptr(it.cov)[p] = ok[r] ? 1 : 0;
         }
-        if (bx1 >= 0) {
-            atomicMin(&sbox[0], bx0); atomicMin(&sbox[1], by0); atomicMax(&sbox[2], bx1); atomicMax(&sbox[3], by1);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            bx0 = min(bx0, __shfl_xor(bx0, o));
+            by0 = min(by0, __shfl_xor(by0, o));
+            bx1 = max(bx1, __shfl_xor(bx1, o));
+            by1 = max(by1, __shfl_xor(by1, o));
         }
+        if (lx == 0) {
+            typedef int i4v __attribute__((ext_vector_type(4)));
+            i4v v;
+            v.x = bx0; v.y = by0; v.z = bx1; v.w = by1;
+            *gptr(reinterpret_cast<i4v*>(wbox) + static_cast<size_t>(e) * 4 + wave) = v;
+        }
+    }
+}
+
+// Coverage box of each item from its warp tiles' per-wave partial boxes
+// (one block per item).
+__global__ void __launch_bounds__(256) rbox_reduce_kernel(RItem* __restrict__ items, int n,
+                                                          const int4* __restrict__ wbox) {
+    __shared__ int red[4][256];
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const RItem& it = items[i];
+        int b0 = INT_MAX, b1 = INT_MAX, b2 = -1, b3 = -1;
+        const size_t q0 = static_cast<size_t>(it.wtile0) * 4, q1 = q0 + static_cast<size_t>(it.wtilen) * 4;
+        for (size_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+            const int4 v = wbox[q];
+            b0 = min(b0, v.x); b1 = min(b1, v.y); b2 = max(b2, v.z); b3 = max(b3, v.w);
+        }
+        red[0][threadIdx.x] = b0; red[1][threadIdx.x] = b1; red[2][threadIdx.x] = b2; red[3][threadIdx.x] = b3;
         __syncthreads();
-        if (threadIdx.x == 0 && sbox[2] >= 0) {
-            atomicMin(&it.box[0], sbox[0]); atomicMin(&it.box[1], sbox[1]);
-            atomicMax(&it.box[2], sbox[2]); atomicMax(&it.box[3], sbox[3]);
+        for (int k = 128; k > 0; k >>= 1) {
+            if (threadIdx.x < k) {
+                red[0][threadIdx.x] = min(red[0][threadIdx.x], red[0][threadIdx.x + k]);
+                red[1][threadIdx.x] = min(red[1][threadIdx.x], red[1][threadIdx.x + k]);
+                red[2][threadIdx.x] = max(red[2][threadIdx.x], red[2][threadIdx.x + k]);
+                red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + k]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            int* b = it.box;
+            if (red[2][0] >= 0) { b[0] = red[0][0]; b[1] = red[1][0]; b[2] = red[2][0]; b[3] = red[3][0]; }
+            else { b[0] = INT_MAX; b[1] = INT_MAX; b[2] = -1; b[3] = -1; }
         }
         __syncthreads();
     }
@@ -442,8 +491,15 @@ __global__ void __launch_bounds__(256) rmap_error_kernel(const RItem* __restrict
             const double u = ((m[0] * X + m[1] * Y) + m[2]) / q, v = ((m[3] * X + m[4] * Y) + m[5]) / q;
             if (!(u >= 0.0 && v >= 0.0 && u < it.fw - 1 && v < it.fh - 1)) continue;
             const float den = (1.0f + tm.g0 * static_cast<float>(lx)) + tm.g1 * static_cast<float>(ly);
-            const float ox = (tm.a00 * static_cast<float>(lx) + tm.a01 * static_cast<float>(ly)) / den;
-            const float oy = (tm.a10 * static_cast<float>(lx) + tm.a11 * static_cast<float>(ly)) / den;
+            float ox, oy;  // as map_pixel computes them
+            if (den > 0.5f) {
+                const float rd = 1.0f / den;
+                ox = (tm.a00 * static_cast<float>(lx) + tm.a01 * static_cast<float>(ly)) * rd;
+                oy = (tm.a10 * static_cast<float>(lx) + tm.a11 * static_cast<float>(ly)) * rd;
+            } else {
+                ox = (tm.a00 * static_cast<float>(lx) + tm.a01 * static_cast<float>(ly)) / den;
+                oy = (tm.a10 * static_cast<float>(lx) + tm.a11 * static_cast<float>(ly)) / den;
+            }
             const double eu = (tm.ix + static_cast<double>(tm.fx + ox)) - u;
             const double ev = (tm.iy + static_cast<double>(tm.fy + oy)) - v;
             worst = fmaxf(worst, static_cast<float>(fmax(fabs(eu), fabs(ev))));
@@ -1295,8 +1351,11 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         const RChunk& ch = P.chunks[c];
         P.warp[c].off = static_cast<int>(P.tiles.size());
         for (int k = 0; k < ch.count; ++k) {
-            const int* b = P.items[ch.first + k].lbox[0];
+            RItem& it = P.items[ch.first + k];
+            const int* b = it.lbox[0];
+            it.wtile0 = static_cast<int>(P.tiles.size());
             add(ch.first + k, b[0] / kWarpTW, b[2] / kWarpTW, b[1] / kWarpTH, b[3] / kWarpTH);
+            it.wtilen = static_cast<int>(P.tiles.size()) - it.wtile0;
         }
         P.warp[c].n = static_cast<int>(P.tiles.size()) - P.warp[c].off;
     }
@@ -1375,6 +1434,8 @@ struct DevTables {
     SurrImg* simgs = nullptr;
     SurrTile* stiles = nullptr;
     int nst = 0;
+    TileMap* tmaps = nullptr;  // per warp tile (precision forms)
+    int4* wbox = nullptr;      // per warp tile and wavefront: partial coverage box
     std::vector<std::vector<int>> round_chunks;  // seam round k -> chunk ids
     std::vector<int> round_off;                  // offset of round k in the seam table
 };
@@ -1473,9 +1534,18 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
     span(P.warp, off, n);
     double area0 = 0;
     for (int k = c0; k < c1; ++k) area0 += static_cast<double>(P.chunks[k].w) * P.chunks[k].h;
+    if (MM == kModePrecision && n > 0)
+        RL(st, "tile_map", 0.0, rtilemap_kernel<<<(n + 255) / 256, 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, T.tmaps + off));
     RL(st, MODE == kModeFP64 ? "warp_fp64" : (MODE == kModePlain ? "warp_f32" : "warp_f16"),
        static_cast<double>(n) * kWarpTW * kWarpTH * (6.0 + 4 * sS + 1),
-       rwarp_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+       rwarp_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
+                                                                    T.tmaps + off, T.wbox + 4 * static_cast<size_t>(off)));
+    {
+        const int i0 = P.chunks[c0].first;
+        const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
+        if (i1 > i0)
+            RL(st, "box_reduce", 0.0, rbox_reduce_kernel<<<std::min(i1 - i0, 1024), 256, 0, s>>>(T.items + i0, i1 - i0, T.wbox));
+    }
     // chamfer over the items of the chunk range
     {
         const int i0 = P.chunks[c0].first;
@@ -1641,7 +1711,12 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
                                      sizeof(s16[0]) * s16.size(), size_t(256)}));
     const size_t oSI = take(sizeof(SurrImg) * std::max<size_t>(1, P.simgs.size()));
     const size_t oST = take(sizeof(SurrTile) * std::max<size_t>(1, P.stiles.size()));
-    std::vector<uint8_t> host(off);
+    size_t nwarp = 0;
+    for (const auto& r : P.warp) nwarp += r.n;
+    const size_t upload = off;  // device-only arrays follow the uploaded tables
+    const size_t oTM = take(sizeof(TileMap) * std::max<size_t>(1, nwarp));
+    const size_t oWB = take(sizeof(int4) * 4 * std::max<size_t>(1, nwarp));
+    std::vector<uint8_t> host(upload);
     std::memcpy(host.data() + oI, P.items.data(), sizeof(RItem) * P.items.size());
     std::memcpy(host.data() + oC, P.chunks.data(), sizeof(RChunk) * P.chunks.size());
     std::memcpy(host.data() + oT, P.tiles.data(), sizeof(RTile) * P.tiles.size());
@@ -1653,7 +1728,7 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     if (!P.stiles.empty()) std::memcpy(host.data() + oST, P.stiles.data(), sizeof(SurrTile) * P.stiles.size());
     auto* dev = static_cast<uint8_t*>(ctx_scratch(ctx, slot, off, st));
     if (!st.ok()) return;
-    if (hipMemcpyAsync(dev, host.data(), off, hipMemcpyHostToDevice, s) != hipSuccess ||
+    if (hipMemcpyAsync(dev, host.data(), upload, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         st.set(I360HIP_DEVICE_ERROR, "render table upload failed");
         return;
@@ -1668,6 +1743,8 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     T.simgs = reinterpret_cast<SurrImg*>(dev + oSI);
     T.stiles = reinterpret_cast<SurrTile*>(dev + oST);
     T.nst = static_cast<int>(P.stiles.size());
+    T.tmaps = reinterpret_cast<TileMap*>(dev + oTM);
+    T.wbox = reinterpret_cast<int4*>(dev + oWB);
 }
 
 static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc,

@@ -59,7 +59,7 @@ def bf16_values(x):
 
 # ------------------------------------------------------------ training data --
 def level_patches(hip, frames_desc, chunks_desc, contributors, bands, chunk_sizes, patch=32, seam_only=True,
-                  stream=None):
+                  stream=None, stride=None):
     """Features / targets of the plain blend at each chunk's surrogate level
     (i360hip_surrogate_features), cut into patch x patch tiles; returns two
     torch tensors (N, patch, patch, 16) and (N, patch, patch, 3) on the GPU.
@@ -82,8 +82,9 @@ def level_patches(hip, frames_desc, chunks_desc, contributors, bands, chunk_size
         f = feat[off: off + lw * lh].view(lh, lw, 16)
         t = target[off: off + lw * lh].view(lh, lw, 3)
         off += lw * lh
-        for y in range(0, lh - patch + 1, patch):
-            for x in range(0, lw - patch + 1, patch):
+        step = stride or patch
+        for y in range(0, lh - patch + 1, step):
+            for x in range(0, lw - patch + 1, step):
                 fp = f[y: y + patch, x: x + patch]
                 if seam_only:
                     m = fp[..., 6]
@@ -216,7 +217,7 @@ def main():
 
     ap = argparse.ArgumentParser(description="train the blend surrogate on synthetic panoramas (GPU)")
     ap.add_argument("--out", default=None)
-    ap.add_argument("--panoramas", type=int, default=6)
+    ap.add_argument("--panoramas", type=int, default=40)
     ap.add_argument("--patches", type=int, default=20000)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=64)
@@ -228,9 +229,9 @@ def main():
         fs, ts = [], []
         t0 = time.time()
         for k in range(a.panoramas):
-            g, frames = panorama(a.seed + 101 * k)
+            g, frames = panorama(a.seed + 101 * k, rows=4, cols=6)
             fd, cd, ct, sizes = describe(g, frames)
-            f, t = level_patches(hip, fd, cd, ct, a.bands, sizes)
+            f, t = level_patches(hip, fd, cd, ct, a.bands, sizes, stride=16)
             fs.append(f)
             ts.append(t)
             log(f"panorama {k}: {len(g.chunks)} chunks, {f.shape[0]} seam patches ({time.time() - t0:.1f} s)")
