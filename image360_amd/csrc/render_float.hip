@@ -680,6 +680,92 @@ __global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ 
     }
 }
 
+// Level 0 -> 1 of the FP16 forms: the tile's level-0 window (19 rows x 132
+// columns, replicate-clamped, zero outside the item's coverage box) is
+// staged once into LDS, every element with one load issued up front, and
+// both blur passes read LDS.  Same arithmetic as rreduce_kernel<MODE, true>.
+constexpr int kRed0Cols = 2 * kRedTW + 4;
+template <int MODE>
+__global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__ items,
+                                                       const RChunk* __restrict__ chunks,
+                                                       const RTile* __restrict__ tiles, int ntiles,
+                                                       int phase, const int* __restrict__ cflag) {
+    using M = float;
+    constexpr int NE = kRedRows * kRed0Cols;  // staged elements
+    constexpr int PER = (NE + 255) / 256;
+    __shared__ h4 sq[NE];
+    __shared__ uint8_t sw[NE];
+    __shared__ M tmp[5][kRedRows][kRedTW];
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+        const RTile tl = tiles[e];
+        const RItem& it = items[tl.id];
+        const RChunk& ch = chunks[it.chunk];
+        if (phase_skip(ch, it.chunk, 1, phase, cflag)) continue;  // block-uniform
+        const int w = ch.lw[0], h = ch.lh[0], ow = ch.lw[1], oh = ch.lh[1];
+        const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
+        const int* sb = it.box;
+        const h4* wp = static_cast<const h4*>(it.warped);
+        h4 qv[PER];
+        int ov[PER];
+        bool inb[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            const int rr = i / kRed0Cols, j = i % kRed0Cols;
+            const int cx = min(max(2 * ox0 - 2 + j, 0), w - 1);
+            const int ry = min(max(2 * oy0 - 2 + rr, 0), h - 1);
+            inb[k] = i < NE && in_box(sb, cx, ry);
+            const size_t p = inb[k] ? static_cast<size_t>(ry) * w + cx : 0;
+            qv[k] = gptr(wp)[p];
+            ov[k] = gptr(ch.owner)[p];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            if (i >= NE) continue;
+            h4 z;
+            z.x = z.y = z.z = z.w = static_cast<_Float16>(0.0f);
+            sq[i] = inb[k] ? qv[k] : z;
+            sw[i] = (inb[k] && ov[k] == it.local) ? 1 : 0;
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kRedRows * kRedTW; idx += 256) {
+            const int rr = idx / kRedTW, xx = idx % kRedTW;
+            if (ox0 + xx >= ow) continue;
+            M v[5][5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const int j = 2 * xx + k;
+                const h4 q = sq[rr * kRed0Cols + j];
+                const M q0 = static_cast<M>(q.x), q1 = static_cast<M>(q.y), q2 = static_cast<M>(q.z), q3 = static_cast<M>(q.w);
+                v[k][0] = q0 * q3;
+                v[k][1] = q1 * q3;
+                v[k][2] = q2 * q3;
+                v[k][3] = q3;
+                v[k][4] = sw[rr * kRed0Cols + j] ? M(1) : M(0);
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) tmp[c][rr][xx] = blur5m<M>(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c]);
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kRedTH * kRedTW; idx += 256) {
+            const int yy = idx / kRedTW, xx = idx % kRedTW;
+            const int ox = ox0 + xx, oy = oy0 + yy;
+            if (ox >= ow || oy >= oh) continue;
+            const int r = 2 * yy + 2;
+            M o[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                o[c] = blur5m<M>(tmp[c][r - 2][xx], tmp[c][r - 1][xx], tmp[c][r][xx], tmp[c][r + 1][xx], tmp[c][r + 2][xx]);
+            const size_t q = static_cast<size_t>(oy) * ow + ox;
+            const M nd[4] = {o[0], o[1], o[2], o[3]};
+            st4(it.nd[1], q, nd, static_cast<_Float16*>(nullptr));
+            st1<_Float16, M>(it.wt[1], q, o[4]);
+        }
+        __syncthreads();
+    }
+}
+
 // ----------------------------------------------------------- compose ---
 template <typename M>
 struct Coord {
@@ -1607,7 +1693,9 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
             for (int k = c0; k < c1; ++k) n += P.red[k][l].n;
             if (n == 0) continue;
             const double bytes = static_cast<double>(n) * kRedTW * kRedTH * (4.0 * (4 * sS + sS) + (4 * sS + sS));
-            if (l == 0)
+            if (l == 0 && MM == kModePrecision)
+                RL(st, "reduce_l0", bytes, rreduce0_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, phase, cflag));
+            else if (l == 0)
                 RL(st, "reduce_l0", bytes, rreduce_kernel<MODE, true><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
             else
                 RL(st, "reduce", bytes, rreduce_kernel<MODE, false><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
