@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--forms", default="plain,precision,batched,surrogate")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch configuration")
     ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "image360_amd", "surrogate_weights.npz"))
     ap.add_argument("--surrogate-threshold", type=float, default=1e-5,
                     help="mean predicted squared error per tile above which the surrogate falls back (1e-5 = 50 dB)")
@@ -207,6 +208,13 @@ def main():
             torch.cuda.synchronize()
         return ev0.elapsed_time(ev1)
 
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.time()
+        with torch.cuda.stream(stream):
+            torch.cuda.synchronize()
+            tuning = hip.render_autotune(fdesc, cdesc, contrib, args.bands)
+        log(f"autotune ({time.time() - t0:.1f} s): {tuning}")
     forms = {}
     wanted = set(args.forms.split(","))
     surr_src = None
@@ -316,7 +324,7 @@ def main():
                    "canvas": [geom.canvas_w, geom.canvas_h],
                    "forms": forms,
                    "psnr_db_vs_fp64": psnr, "max_abs_err_vs_fp64": maxerr,
-                   "psnr_sample_chunks": len(sample), "map_max_err_px": map_err,
+                   "psnr_sample_chunks": len(sample), "map_max_err_px": map_err, "launch_tuning": tuning,
                    "speedup_vs_plain": round(head["mp_s"] / forms["plain"]["mp_s"], 2) if "plain" in forms else None},
         "roofline": roofline, "cpu_baseline": cpu,
     }

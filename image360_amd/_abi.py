@@ -57,6 +57,11 @@ RENDER_NAMES = {0: "fp64", 1: "plain", 2: "precision", 3: "batched", 4: "surroga
 
 
 # name -> (restype, argtypes)
+class Tuning(C.Structure):
+    _fields_ = [("chunk_groups", C.c_int), ("chamfer_strip", C.c_int), ("grid_share", C.c_int),
+                ("ms", C.c_double)]
+
+
 SIGNATURES = {
     "i360hip_create": (i32, [i32, C.POINTER(vp)]),
     "i360hip_destroy": (None, [vp]),
@@ -89,6 +94,9 @@ SIGNATURES = {
     "i360hip_surrogate_features": (i32, [vp, C.POINTER(Frame), i32, C.POINTER(ChunkDesc), i32, C.POINTER(i32), i32,
                                          vp, vp, vp]),
     "i360hip_surrogate_stats": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
+    "i360hip_render_autotune": (i32, [vp, C.POINTER(Frame), i32, C.POINTER(ChunkDesc), i32, C.POINTER(i32), i32,
+                                      C.POINTER(Tuning)]),
+    "i360hip_render_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
 }
 
 

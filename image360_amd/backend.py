@@ -219,6 +219,20 @@ class HipComputeBackend:
         fa, ca, cont = self._render_args(frames, chunks, contributors)
         return lib().i360hip_render_chunks(self._ctx, fa, len(frames), ca, len(chunks), cont, num_bands, mode, stream)
 
+    def render_autotune(self, frames, chunks, contributors, num_bands) -> dict:
+        """i360hip_render_autotune: picks and keeps the fastest launch
+        configuration of the batched form for these chunks (synchronous)."""
+        fa, ca, cont = self._render_args(frames, chunks, contributors)
+        t = _abi.Tuning()
+        self.check(lib().i360hip_render_autotune(self._ctx, fa, len(frames), ca, len(chunks), cont, num_bands,
+                                                 C.byref(t)), "render_autotune")
+        return {"chunk_groups": t.chunk_groups, "chamfer_strip": t.chamfer_strip, "grid_share": t.grid_share,
+                "ms": t.ms}
+
+    def render_set_tuning(self, chunk_groups=0, chamfer_strip=0, grid_share=0) -> None:
+        t = _abi.Tuning(chunk_groups, chamfer_strip, grid_share, 0.0)
+        self.check(lib().i360hip_render_set_tuning(self._ctx, C.byref(t)), "render_set_tuning")
+
     def render_map_error(self, frames, chunks, contributors) -> float:
         fa, ca, cont = self._render_args(frames, chunks, contributors)
         out = C.c_double(0)

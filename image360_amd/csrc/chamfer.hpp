@@ -197,12 +197,14 @@ inline int ChamferThreads(int w, int S) {
     return ((cols + 127) / 128) * 128;  // even and odd strips in separate wavefronts
 }
 
-// Strip width: 8 columns (measured fastest on 4096-wide chunks: fewer
-// steps than 4 columns, shorter per-step chains than 16), 16 where a row
-// would not fit one 1024-thread block.  I360HIP_CHAMFER_STRIP (4, 8 or 16)
-// overrides it with any width that still fits.
-inline int ChamferStrip(int w) {
+// Strip width: `want` (4, 8 or 16; an autotuned choice) when a row of w
+// columns fits one 1024-thread block at that width, else 8 columns (the
+// measured best on 4096-wide chunks: fewer steps than 4 columns, shorter
+// per-step chains than 16), 16 where 8 does not fit.
+// I360HIP_CHAMFER_STRIP overrides both.
+inline int ChamferStrip(int w, int want = 0) {
     int s = w <= 8192 ? 8 : 16;
+    if ((want == 4 || want == 8 || want == 16) && w <= 1024 * want) s = want;
     if (const char* e = std::getenv("I360HIP_CHAMFER_STRIP")) {
         const int v = std::atoi(e);
         if ((v == 4 || v == 8 || v == 16) && w <= 1024 * v) s = v;
@@ -211,9 +213,9 @@ inline int ChamferStrip(int w) {
 }
 
 // Launches forward then backward passes over `n` work items (device table).
-inline hipError_t launch_chamfer(const ChamferWork* d_work, int n, int max_w, hipStream_t s) {
+inline hipError_t launch_chamfer(const ChamferWork* d_work, int n, int max_w, hipStream_t s, int want_strip = 0) {
     if (n <= 0) return hipSuccess;
-    const int S = ChamferStrip(max_w);
+    const int S = ChamferStrip(max_w, want_strip);
     const int nt = ChamferThreads(max_w, S);
     if (S == 4) {
         chamfer_kernel<4, 8, true><<<n, nt, 0, s>>>(d_work);

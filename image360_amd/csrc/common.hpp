@@ -91,6 +91,9 @@ struct i360hip_ctx {
     const float* surr_biases = nullptr;
     float surr_threshold = 1e-5f;
     int surr_tiles = 0, surr_fallback = -1;  // last surrogate render
+    // launch configuration of the batched render (i360hip_render_autotune;
+    // 0 = built-in default)
+    int tune_groups = 0, tune_strip = 0, tune_share = 0;
     std::vector<i360::DevBuf> pinned;
 };
 

@@ -510,16 +510,18 @@ __global__ void __launch_bounds__(256) rmap_error_kernel(const RItem* __restrict
 }
 
 // ---------------------------------------------------------- ownership ---
-__global__ void __launch_bounds__(256) rowner_kernel(const RItem* __restrict__ items,
-                                                     const RChunk* __restrict__ chunks,
-                                                     const RTile* __restrict__ tiles, int ntiles) {
+// One thread per pixel of a 64x16 tile (1024 threads): a thread's work is
+// one row's pixel, so no row waits behind another row's store.
+__global__ void __launch_bounds__(1024) rowner_kernel(const RItem* __restrict__ items,
+                                                      const RChunk* __restrict__ chunks,
+                                                      const RTile* __restrict__ tiles, int ntiles) {
     for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
         const RTile tl = tiles[e];
         const RChunk& ch = chunks[tl.id];
         const int lx = threadIdx.x & 63;
-        for (int r = 0; r < 4; ++r) {
-            const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
-            if (x >= ch.w || y >= ch.h) continue;
+        {
+            const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6);
+            if (x >= ch.w || y >= ch.h) continue;  // the next tile
             const size_t p = static_cast<size_t>(y) * ch.w + x;
             int best = -1;
             float bd = 0.0f;
@@ -1085,10 +1087,11 @@ __global__ void __launch_bounds__(256) rsexport_kernel(const RChunk* __restrict_
 }
 
 // Level 0: only the owner carries weight (0/1 owner mask); FP32 RGB out.
+// One thread per pixel of a 64x16 tile (1024 threads).
 template <int MODE>
-__global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ items,
-                                                     const RChunk* __restrict__ chunks,
-                                                     const RTile* __restrict__ tiles, int ntiles) {
+__global__ void __launch_bounds__(1024) rfinal_kernel(const RItem* __restrict__ items,
+                                                      const RChunk* __restrict__ chunks,
+                                                      const RTile* __restrict__ tiles, int ntiles) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
@@ -1096,13 +1099,15 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         const RTile tl = tiles[e];
         const RChunk& ch = chunks[tl.id];
         const int lx = threadIdx.x & 63;
-        for (int r = 0; r < 4; ++r) {
-            const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
-            if (x >= ch.w || y >= ch.h) continue;
-            const size_t p = static_cast<size_t>(y) * ch.w + x;
-            auto* dst = gptr(ch.out + p * 3);
-            const int o = gptr(ch.owner)[p];
-            if (o < 0) { dst[0] = 0.0f; dst[1] = 0.0f; dst[2] = 0.0f; continue; }
+        const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6);
+        if (x >= ch.w || y >= ch.h) continue;  // the next tile (no barrier in this kernel)
+        const size_t p = static_cast<size_t>(y) * ch.w + x;
+        const int o = gptr(ch.owner)[p];
+        if (ch.owner_out) gptr(ch.owner_out)[p] = o;
+        auto* dst = gptr(ch.out + p * 3);
+        if (o < 0) {
+            dst[0] = 0.0f; dst[1] = 0.0f; dst[2] = 0.0f;
+        } else {
             const RItem& it = items[ch.first + o];
             M q[4];
             ld4(it.warped, p, q, static_cast<S*>(nullptr));
@@ -1123,14 +1128,6 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                 M t = v[k] < M(1) ? v[k] : M(1);
                 t = M(0) < t ? t : M(0);
                 dst[k] = static_cast<float>(t);
-            }
-        }
-        if (ch.owner_out) {
-            for (int r = 0; r < 4; ++r) {
-                const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6) * 4 + r;
-                if (x >= ch.w || y >= ch.h) continue;
-                const size_t p = static_cast<size_t>(y) * ch.w + x;
-                gptr(ch.owner_out)[p] = gptr(ch.owner)[p];
             }
         }
     }
@@ -1591,20 +1588,27 @@ static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables
         }                                                                                           \
     } while (0)
 
-static unsigned persistent_grid(i360hip_ctx* ctx, int n) {
-    const int cap = ctx->props.multiProcessorCount * 8;
+// Share of the chip a persistent launch asks for: 8 blocks per CU, divided
+// among the chunk groups that run concurrently (set while they are
+// enqueued).
+static thread_local int g_grid_share = 1;
+
+static unsigned persistent_grid(i360hip_ctx* ctx, int n, int blocks_per_cu = 8) {
+    const int cap = std::max(1, ctx->props.multiProcessorCount * blocks_per_cu / g_grid_share);
     return static_cast<unsigned>(std::max(1, std::min(n, cap)));
 }
 
 // Enqueues the stages of chunk `c` (or of every chunk when c < 0).
+// Launches the render of chunks [c0, c1) on stream s.  Every stage's tiles
+// of a chunk range are contiguous, so each stage is one launch over the
+// range.
 template <int MODE>
-static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, hipStream_t s, Status& st,
+static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0, int c1, hipStream_t s, Status& st,
                     const SurrRun* sr = nullptr) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
     const size_t sS = sizeof(S), sM = sizeof(M);
-    const int c0 = c < 0 ? 0 : c, c1 = c < 0 ? static_cast<int>(P.chunks.size()) : c + 1;
     auto span = [&](const std::vector<Plan::Range>& v, int& off, int& n) {
         off = v[c0].off;
         n = 0;
@@ -1637,13 +1641,13 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
         const int i0 = P.chunks[c0].first;
         const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
         ProfScope ps_(ctx, s, "chamfer", 0.0);
-        if (i1 > i0 && launch_chamfer(T.chamfer + i0, i1 - i0, P.max_w, s) != hipSuccess) {
+        if (i1 > i0 && launch_chamfer(T.chamfer + i0, i1 - i0, P.max_w, s, ctx->tune_strip) != hipSuccess) {
             st.set(I360HIP_KERNEL_LAUNCH_FAILED, "chamfer launch failed");
             return;
         }
     }
     span(P.own, off, n);
-    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n, 2), 1024, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
     // seams: the k-th pair of every chunk in range, in pair order
     {
         for (size_t k = 0; k < T.round_chunks.size(); ++k) {
@@ -1755,7 +1759,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c, 
     if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
     RL(st, "final_level0", area0 * (4.0 + 4 * sS + 12.0),
-       rfinal_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+       rfinal_kernel<MODE><<<persistent_grid(ctx, n, 2), 1024, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
 }
 
 struct RenderCache {
@@ -1765,6 +1769,10 @@ struct RenderCache {
     DevTables tables;
     hipGraphExec_t graph = nullptr;
     hipStream_t graph_stream = nullptr;
+    // chunk groups of the batched form: auxiliary streams and their events
+    hipEvent_t fork = nullptr;
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> join;
 };
 
 static RenderCache& cache_of(i360hip_ctx* ctx) {
@@ -1776,6 +1784,12 @@ void render_release(i360hip_ctx* ctx) {
     if (!ctx->render_cache) return;
     auto* rc = static_cast<RenderCache*>(ctx->render_cache);
     if (rc->graph) (void)hipGraphExecDestroy(rc->graph);
+    for (auto a : rc->aux) {
+        (void)hipStreamSynchronize(a);
+        (void)hipStreamDestroy(a);
+    }
+    for (auto e : rc->join) (void)hipEventDestroy(e);
+    if (rc->fork) (void)hipEventDestroy(rc->fork);
     delete rc;
     ctx->render_cache = nullptr;
 }
@@ -1835,6 +1849,81 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     T.wbox = reinterpret_cast<int4*>(dev + oWB);
 }
 
+// Chunk groups of the batched form: contiguous chunk ranges of about equal
+// output area, each rendered on its own stream (the context's auxiliary
+// streams, forked from and joined back into `s`, also under graph capture),
+// so the latency-bound stages of one group (chamfer, seam DP) overlap the
+// bandwidth-bound stages of the others.  The count comes from the
+// autotuner (i360hip_render_autotune), default 4; I360HIP_RENDER_GROUPS
+// overrides it (1 = one launch sequence over every chunk).
+constexpr int kMaxGroups = 8;
+
+static int render_groups(const i360hip_ctx* ctx, int nch) {
+    int g = ctx->tune_groups > 0 ? ctx->tune_groups : 4;
+    if (const char* e = std::getenv("I360HIP_RENDER_GROUPS")) g = std::atoi(e);
+    return std::max(1, std::min({g, kMaxGroups, nch}));
+}
+
+static bool aux_streams(i360hip_ctx* ctx, int n) {
+    RenderCache& rc = cache_of(ctx);
+    if (!rc.fork) {
+        if (hipEventCreateWithFlags(&rc.fork, hipEventDisableTiming) != hipSuccess) return false;
+    }
+    while (static_cast<int>(rc.aux.size()) < n) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return false;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(st);
+            return false;
+        }
+        rc.aux.push_back(st);
+        rc.join.push_back(ev);
+    }
+    return true;
+}
+
+static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, hipStream_t s, Status& st) {
+    const int nch = static_cast<int>(P.chunks.size());
+    const int G = render_groups(ctx, nch);
+    if (G <= 1 || !aux_streams(ctx, G - 1)) {
+        enqueue<kModeBatched>(ctx, P, T, 0, nch, s, st);
+        return;
+    }
+    RenderCache& rc = cache_of(ctx);
+    double total = 0;
+    for (const auto& c : P.chunks) total += static_cast<double>(c.w) * c.h;
+    // split points: equal shares of the output area
+    std::vector<int> cut{0};
+    double acc = 0;
+    for (int c = 0; c < nch && static_cast<int>(cut.size()) < G; ++c) {
+        acc += static_cast<double>(P.chunks[c].w) * P.chunks[c].h;
+        if (acc >= total * static_cast<double>(cut.size()) / G && c + 1 < nch) cut.push_back(c + 1);
+    }
+    cut.push_back(nch);
+    const int ng = static_cast<int>(cut.size()) - 1;
+    if (hipEventRecord(rc.fork, s) != hipSuccess) {
+        st.set(I360HIP_DEVICE_ERROR, "render fork failed");
+        return;
+    }
+    for (int g = 1; g < ng; ++g)
+        if (hipStreamWaitEvent(rc.aux[g - 1], rc.fork, 0) != hipSuccess) {
+            st.set(I360HIP_DEVICE_ERROR, "render fork failed");
+            return;
+        }
+    int share = ctx->tune_share > 0 ? ctx->tune_share : 1;
+    if (const char* e = std::getenv("I360HIP_GRID_SHARE")) share = std::max(1, std::atoi(e));
+    g_grid_share = share;
+    for (int g = 0; g < ng && st.ok(); ++g)
+        enqueue<kModeBatched>(ctx, P, T, cut[g], cut[g + 1], g == 0 ? s : rc.aux[g - 1], st);
+    g_grid_share = 1;
+    for (int g = 1; g < ng; ++g)
+        if (hipEventRecord(rc.join[g - 1], rc.aux[g - 1]) != hipSuccess ||
+            hipStreamWaitEvent(s, rc.join[g - 1], 0) != hipSuccess) {
+            if (st.ok()) st.set(I360HIP_DEVICE_ERROR, "render join failed");
+        }
+}
+
 static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc,
                        const int* contrib, int bands, int api_mode, hipStream_t s, Status& st,
                        float* export_feat = nullptr, float* export_target = nullptr) {
@@ -1853,6 +1942,7 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     sr.biases = ctx->surr_biases;
     sr.threshold = ctx->surr_threshold;
     const bool graphable = mode == kModeBatched && surr != kSurrExport;
+    P.key = mix(P.key, static_cast<uint64_t>(ctx->tune_groups * 1000003 + ctx->tune_strip * 101 + ctx->tune_share));
     const bool reuse = !ctx->profiling && rc.graph && rc.key == P.key && rc.mode == api_mode && graphable;
     if (reuse) {
         if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
@@ -1879,16 +1969,21 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
         if (ni > 0) rmap_kernel<<<(ni + 63) / 64, 64, 0, ss>>>(T.items, ni);
         switch (mode) {
             case kModeFP64:
-                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModeFP64>(ctx, Q, T, c, ss, st);
+                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModeFP64>(ctx, Q, T, c, c + 1, ss, st);
                 break;
             case kModePlain:
-                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModePlain>(ctx, Q, T, c, ss, st);
+                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModePlain>(ctx, Q, T, c, c + 1, ss, st);
                 break;
             case kModePrecision:
-                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModePrecision>(ctx, Q, T, c, ss, st);
+                for (int c = 0; c < static_cast<int>(Q.chunks.size()) && st.ok(); ++c) enqueue<kModePrecision>(ctx, Q, T, c, c + 1, ss, st);
                 break;
             default:
-                if (!Q.chunks.empty()) enqueue<kModeBatched>(ctx, Q, T, -1, ss, st, surr != kSurrNone ? &sr : nullptr);
+                if (Q.chunks.empty()) break;
+                if (surr != kSurrNone) {
+                    enqueue<kModeBatched>(ctx, Q, T, 0, static_cast<int>(Q.chunks.size()), ss, st, &sr);
+                    break;
+                }
+                enqueue_groups(ctx, Q, T, ss, st);
                 break;
         }
     };
@@ -2094,6 +2189,87 @@ int i360hip_surrogate_stats(i360hip_ctx* ctx, int* tiles, int* fallback_tiles) {
     for (size_t t = 0; t < tf.size(); ++t) fb += tf[t] ? 1 : 0;
     *tiles = static_cast<int>(tf.size());
     *fallback_tiles = fb;
+    return I360HIP_SUCCESS;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int i360hip_render_autotune(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                            const i360hip_chunk* chunks, int num_chunks, const int* contributors, int num_bands,
+                            i360hip_tuning* out) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!out || (!frames && num_frames > 0) || (!chunks && num_chunks > 0) || (!contributors && num_chunks > 0) ||
+        num_bands < 1 || num_bands > I360HIP_MAX_BANDS) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument or numBands outside 1-12.");
+        return I360HIP_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Status st;
+    if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "autotune event creation failed");
+        return I360HIP_DEVICE_ERROR;
+    }
+    // time one render (graph captured on the first call, then replayed) of
+    // the batched form with the context's current configuration
+    auto measure = [&](double& ms) {
+        run_render(ctx, frames, num_frames, chunks, num_chunks, contributors, num_bands, I360HIP_RENDER_BATCHED, s,
+                   st);  // capture (or eager) + first launch
+        if (!st.ok() || hipStreamSynchronize(s) != hipSuccess) return false;
+        float best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            (void)hipEventRecord(e0, s);
+            run_render(ctx, frames, num_frames, chunks, num_chunks, contributors, num_bands, I360HIP_RENDER_BATCHED,
+                       s, st);
+            (void)hipEventRecord(e1, s);
+            if (!st.ok() || hipEventSynchronize(e1) != hipSuccess) return false;
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            best = std::min(best, t);
+        }
+        ms = best;
+        return true;
+    };
+    // coordinate search: chamfer strip, then chunk groups, then grid share
+    int bestG = 4, bestS = 0, bestShare = 1;
+    double bestMs = 1e300;
+    auto trial = [&](int G, int S, int share) {
+        ctx->tune_groups = G;
+        ctx->tune_strip = S;
+        ctx->tune_share = share;
+        double ms = 0;
+        if (!measure(ms)) return;
+        if (ms < bestMs) { bestMs = ms; bestG = G; bestS = S; bestShare = share; }
+    };
+    for (int S : {4, 8, 16}) if (st.ok()) trial(4, S, 1);
+    for (int G : {1, 2, 3}) if (st.ok()) trial(G, bestS, 1);
+    if (st.ok() && bestG > 1) trial(bestG, bestS, 2);
+    ctx->tune_groups = bestG;
+    ctx->tune_strip = bestS;
+    ctx->tune_share = bestShare;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!st.ok()) {
+        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        return st.code;
+    }
+    out->chunk_groups = bestG;
+    out->chamfer_strip = bestS;
+    out->grid_share = bestShare;
+    out->ms = bestMs;
+    return I360HIP_SUCCESS;
+}
+
+int i360hip_render_set_tuning(i360hip_ctx* ctx, const i360hip_tuning* tuning) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tune_groups = tuning ? std::max(0, tuning->chunk_groups) : 0;
+    ctx->tune_strip = tuning ? std::max(0, tuning->chamfer_strip) : 0;
+    ctx->tune_share = tuning ? std::max(0, tuning->grid_share) : 0;
     return I360HIP_SUCCESS;
 }
 

@@ -204,6 +204,29 @@ int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num
                           const i360hip_chunk* chunks, int num_chunks, const int* contributors,
                           int num_bands, int mode, void* stream);
 
+/* Launch configuration of the BATCHED (and SURROGATE) form: chunk groups
+ * rendered on concurrent streams (1-8), chamfer strip width (4, 8, 16; 0 =
+ * built-in choice), persistent-grid share per group (1 = every launch asks
+ * for the whole chip). */
+typedef struct i360hip_tuning {
+    int chunk_groups;
+    int chamfer_strip;
+    int grid_share;
+    double ms; /* autotune: time of one render with this configuration */
+} i360hip_tuning;
+
+/* Autotuner: renders the given chunks in the BATCHED form under candidate
+ * configurations (a coordinate search over strip width, chunk groups and
+ * grid share, each timed with HIP events as the best of two graph
+ * replays), keeps the fastest in the context for later renders and
+ * returns it.  Writes the chunks' outputs; synchronous.  No counterpart in
+ * the reference. */
+int i360hip_render_autotune(i360hip_ctx* ctx, const i360hip_frame* frames, int num_frames,
+                            const i360hip_chunk* chunks, int num_chunks, const int* contributors, int num_bands,
+                            i360hip_tuning* out);
+/* Sets the configuration (NULL or zero fields = built-in defaults). */
+int i360hip_render_set_tuning(i360hip_ctx* ctx, const i360hip_tuning* tuning);
+
 /* Largest |source coordinate| difference (pixels) between the PRECISION
  * form's per-tile FP32 map and the FP64 map over every output pixel of the
  * given chunks and contributors; synchronous. */

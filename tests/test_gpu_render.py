@@ -169,3 +169,31 @@ def test_grid_pairs_side_by_side(hip, chunk):
     b, bo = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED)
     for k in range(len(g.chunks)):
         assert np.array_equal(bo[k], own64[k]), k
+
+
+def test_autotune_and_launch_configurations_agree(hip):
+    # the launch configuration (chunk groups, chamfer strip, grid share)
+    # changes timing only: every configuration renders the same bytes
+    g, frames = grid_case(seed=9, chunk=160)
+    dev = _upload(frames)
+    note("render autotune")
+    ref, ref_own = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION)
+    frames_d = [{"ptr": t.data_ptr(), "w": int(g.frame_w[i]), "h": int(g.frame_h[i]), "fmt": _abi.PIX_F16,
+                 "gain": float(g.gains[i]), "H": g.H[i]} for i, t in enumerate(dev)]
+    import torch
+
+    outs = [torch.zeros((c.h, c.w, 3), dtype=torch.float32, device="cuda") for c in g.chunks]
+    descs, contrib = [], []
+    for c, o in zip(g.chunks, outs):
+        descs.append({"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": len(contrib), "count": len(c.contributors),
+                      "out": o.data_ptr(), "owner": None})
+        contrib += c.contributors
+    torch.cuda.synchronize()
+    t = hip.render_autotune(frames_d, descs, contrib, BANDS)
+    assert 1 <= t["chunk_groups"] <= 8 and t["chamfer_strip"] in (0, 4, 8, 16) and t["ms"] > 0, t
+    for cfg in [(1, 4, 1), (3, 16, 2), (2, 8, 1), (0, 0, 0)]:
+        hip.render_set_tuning(*cfg)
+        b, bo = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED)
+        for k in range(len(g.chunks)):
+            assert np.array_equal(b[k], ref[k]), (cfg, k)
+            assert np.array_equal(bo[k], ref_own[k]), (cfg, k)
