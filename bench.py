@@ -113,6 +113,26 @@ def cpu_baseline(geom, frames_host_fn, chunks, threads, bands):
                       f"BlendChunkContributors pipeline with {bands} bands; {dt:.1f} s wall"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_gigapixel_batched_hbm_traffic.json")
+
+
+def measured_traffic(kernel, tuning):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated against a
+    1 GiB fill and copy), when they were taken with the same number of
+    chunk groups (a launch covers one group); else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = prof.get("kernels", {}).get(kernel)
+    groups = (tuning or {}).get("chunk_groups", 4)
+    if not k or groups != 4:
+        return None
+    return k["traffic_bytes_per_launch"]
+
+
 def load_surrogate(hip, args, log):
     """Sets the surrogate's parameters from the trained file, or trains a
     small network on synthetic panoramas when the file is absent.  Returns
@@ -255,7 +275,8 @@ def main():
     dom_name, dom = max(hbm.items(), key=lambda kv: kv[1]["total_ms"])
     achieved = dom["alg_bytes"] / (dom["total_ms"] / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom_name, tuning),
                 "avg_launch_us": round(dom["total_ms"] / dom["launches"] * 1e3, 2),
                 "time_share": round(dom["total_ms"] / tot, 3),
                 "latency_bound_share": round(sum(v["total_ms"] for k, v in stats.items()
