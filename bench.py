@@ -49,6 +49,8 @@ def parse():
                     help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--forms", default="plain,precision,batched,surrogate")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch configuration")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the burst median+demosaic and uneven-chunk measurements")
     ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "image360_amd", "surrogate_weights.npz"))
     ap.add_argument("--surrogate-threshold", type=float, default=1e-5,
                     help="mean predicted squared error per tile above which the surrogate falls back (1e-5 = 50 dB)")
@@ -131,6 +133,85 @@ def measured_traffic(kernel, tuning):
     if not k or groups != 4:
         return None
     return k["traffic_bytes_per_launch"]
+
+
+def bench_burst(hip, torch, log):
+    """Median-of-16 over aligned 14-bit RGGB frames of 4000x3000 (integer
+    shifts 0-8 px), then bilinear demosaic to FP32 RGB: GB/s of input read
+    by the median (16 x the aligned plane) and end to end."""
+    from image360_amd import synth
+
+    w, h = 4000, 3000
+    frames, shifts = synth.burst_frames(w=w, h=h)
+    # align: frame k's pixel (x', y') is scene pixel (x' + dx_k, y' + dy_k);
+    # crop pixel (X, Y) is scene pixel (X + 8, Y + 8), i.e. frame pixel
+    # (X + 8 - dx_k, Y + 8 - dy_k).  A plane pointer offset aligns whole rows
+    # (the last columns of each row fall outside the crop and are
+    # discarded); hc = h - 9 keeps every read of every frame inside it.
+    hc = h - 9
+    n = hc * w
+    ptrs = [f.data_ptr() + 2 * ((8 - dy) * w + (8 - dx)) for f, (dx, dy) in zip(frames, shifts)]
+    med = torch.empty(n, dtype=torch.int16, device="cuda")
+    rgb = torch.empty((hc, w, 3), dtype=torch.float32, device="cuda")
+    cam = [2.0, 1.0, 1.6, 1.0]
+    rc = [[1.7, -0.5, -0.2, 0.0], [-0.2, 1.5, -0.3, 0.0], [0.05, -0.55, 1.5, 0.0]]
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    with torch.cuda.stream(stream):
+        torch.cuda.synchronize()
+        for rep in range(3):
+            ev[0].record(stream)
+            hip.check(hip.median_stack_dev(ptrs, med.data_ptr(), n, 2.0, sh), "median_dev")
+            ev[1].record(stream)
+            hip.check(hip.demosaic_dev(med.data_ptr(), w, hc, 512, cam, rc, 0x94949494, rgb.data_ptr(), f32=True,
+                                       stream=sh), "demosaic_dev")
+            ev[2].record(stream)
+        torch.cuda.synchronize()
+    t_med = ev[0].elapsed_time(ev[1])
+    t_all = ev[0].elapsed_time(ev[2])
+    read = 16 * n * 2
+    r = {"frames": "16 x 4000x3000 u16 RGGB, shifts 0-8 px", "median_ms": round(t_med, 3),
+         "median_input_gb_s": round(read / (t_med / 1e3) / 1e9, 1), "median_plus_demosaic_ms": round(t_all, 3),
+         "input_gb_s": round(read / (t_all / 1e3) / 1e9, 1)}
+    log(f"burst median-of-16 + demosaic: {r}")
+    return r
+
+
+def bench_uneven(hip, torch, geom, fdesc, args, log, count=512):
+    """512 chunks with log-uniform edges in [256, 4096] over the gigapixel
+    frames, 6 bands: the batched form (one persistent launch per stage,
+    graph) against a loop of per-chunk renders of the same arithmetic."""
+    from image360_amd import _abi, synth
+
+    g = synth.uneven_chunks(count=count, base=geom)
+    chunks = g.chunks
+    area = sum(c.area for c in chunks)
+    offs = np.cumsum([0] + [c.area * 3 for c in chunks])
+    out = torch.empty(int(offs[-1]), dtype=torch.float32, device="cuda")
+    contrib, descs = [], []
+    for k, c in enumerate(chunks):
+        descs.append({"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": len(contrib), "count": len(c.contributors),
+                      "out": out.data_ptr() + 4 * int(offs[k]), "owner": None})
+        contrib += c.contributors
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    res = {"chunks": len(chunks), "output_mp": round(area / 1e6, 1), "bands": 6}
+    for name, mode, steps in (("batched", _abi.RENDER_BATCHED, 3), ("per_chunk_loop", _abi.RENDER_PRECISION, 1)):
+        with torch.cuda.stream(stream):
+            hip.check(hip.render_chunks(fdesc, descs, contrib, 6, mode, sh), "uneven warmup")
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                hip.check(hip.render_chunks(fdesc, descs, contrib, 6, mode, sh), "uneven")
+            e1.record(stream)
+            torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        res[name] = {"ms": round(ms, 2), "mp_s": round(area / (ms / 1e3) / 1e6, 1)}
+    res["batched_vs_loop"] = round(res["batched"]["mp_s"] / res["per_chunk_loop"]["mp_s"], 2)
+    log(f"uneven chunks: {res}")
+    return res
 
 
 def load_surrogate(hip, args, log):
@@ -315,6 +396,11 @@ def main():
     head = forms[head_name]
     psnr, maxerr = head.get("psnr_db_vs_fp64"), head.get("max_abs_err_vs_fp64")
 
+    extra = None
+    if not args.no_extra:
+        extra = {"burst_median_demosaic": bench_burst(hip, torch, log),
+                 "uneven_chunks": bench_uneven(hip, torch, geom, fdesc, args, log)}
+
     cpu = None
     if not args.no_cpu:
         threads = cpu_threads(args.cpu_threads)
@@ -347,7 +433,7 @@ def main():
                    "psnr_db_vs_fp64": psnr, "max_abs_err_vs_fp64": maxerr,
                    "psnr_sample_chunks": len(sample), "map_max_err_px": map_err, "launch_tuning": tuning,
                    "speedup_vs_plain": round(head["mp_s"] / forms["plain"]["mp_s"], 2) if "plain" in forms else None},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "other_configs": extra,
     }
     print(json.dumps(line), flush=True)
     hip.Shutdown()

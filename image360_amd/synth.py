@@ -242,3 +242,33 @@ def to_u16(frame_f32):
     import torch
 
     return torch.round(frame_f32 * 65535.0).clamp_(0, 65535).to(torch.int32).to(torch.int16)
+
+
+# ------------------------------------------------------------ burst input --
+def burst_frames(seed=20261203, n=16, w=4000, h=3000, max_shift=8, device="cuda"):
+    """`n` 14-bit RGGB mosaics (uint16) of one seeded scene, each shifted by
+    an integer (dx, dy) in [0, max_shift] with Poisson-Gaussian noise (gain
+    0.5 DN per electron, read noise 3 DN); returns (frames, shifts).  The
+    scene is sum-of-sinusoids texture plus rectangles."""
+    import torch
+
+    rng = np.random.default_rng(seed + 3)
+    scene = make_scene(seed, w + max_shift + 1, h + max_shift + 1, cell=256)
+    full = render_frame_torch(scene, np.eye(3), w + max_shift, h + max_shift, device=device)
+    # RGGB mosaic of the full-size scene, 14-bit
+    cfa = torch.empty((h + max_shift, w + max_shift), dtype=torch.float32, device=device)
+    cfa[0::2, 0::2] = full[0::2, 0::2, 0]
+    cfa[0::2, 1::2] = full[0::2, 1::2, 1]
+    cfa[1::2, 0::2] = full[1::2, 0::2, 1]
+    cfa[1::2, 1::2] = full[1::2, 1::2, 2]
+    signal = cfa * 12000.0 + 600.0
+    g = torch.Generator(device=device).manual_seed(seed)
+    frames, shifts = [], []
+    for k in range(n):
+        dx, dy = int(rng.integers(0, max_shift + 1)), int(rng.integers(0, max_shift + 1))
+        win = signal[dy: dy + h, dx: dx + w]
+        e = torch.poisson(win / 0.5, generator=g) * 0.5
+        dn = e + torch.randn(win.shape, generator=g, device=device) * 3.0
+        frames.append(dn.round_().clamp_(0, 16383).to(torch.int32).to(torch.int16).contiguous())
+        shifts.append((dx, dy))
+    return frames, shifts

@@ -193,3 +193,31 @@ def test_gain_literals_and_dev(hip):
     hip.check(hip.gain_dev(dev.data_ptr(), host.size, 1.37), "gain_dev")
     torch.cuda.synchronize()
     assert np.array_equal(dev.cpu().numpy().view(np.uint16), po.apply_gain(host, 1.37))
+
+
+def test_burst_median_and_demosaic_full_size(hip):
+    # the burst configuration at full size: 16 aligned 14-bit RGGB frames of
+    # 4000x3000, median-of-16 bit-exact against the oracle, then the FP32
+    # demosaic of the median bit-exact against the oracle
+    import torch
+
+    from image360_amd import synth
+
+    w, h = 4000, 3000
+    frames, shifts = synth.burst_frames(w=w, h=h)
+    hc = h - 9
+    n = hc * w
+    ptrs = [f.data_ptr() + 2 * ((8 - dy) * w + (8 - dx)) for f, (dx, dy) in zip(frames, shifts)]
+    med = torch.empty(n, dtype=torch.int16, device="cuda")
+    note("burst median 16 x 4000x3000")
+    hip.check(hip.median_stack_dev(ptrs, med.data_ptr(), n), "median_dev")
+    host = [f.cpu().numpy().view(np.uint16).ravel() for f in frames]
+    planes = [hf[(8 - dy) * w + (8 - dx): (8 - dy) * w + (8 - dx) + n] for hf, (dx, dy) in zip(host, shifts)]
+    got = med.cpu().numpy().view(np.uint16)
+    assert np.array_equal(got, po.median_stack(planes, 2.0))
+    cam = [2.0, 1.0, 1.6, 1.0]
+    rc = np.array([[1.7, -0.5, -0.2, 0.0], [-0.2, 1.5, -0.3, 0.0], [0.05, -0.55, 1.5, 0.0]], np.float32)
+    rgb = torch.empty((hc, w, 3), dtype=torch.float32, device="cuda")
+    hip.check(hip.demosaic_dev(med.data_ptr(), w, hc, 512, cam, rc, 0x94949494, rgb.data_ptr(), f32=True), "f32")
+    want = po.demosaic(got.reshape(hc, w), 512, np.array(cam, np.float32), rc, 0x94949494, f32=True)
+    assert np.array_equal(rgb.cpu().numpy(), want)
