@@ -82,12 +82,12 @@ __global__ void __launch_bounds__(256) coverage_u16_kernel(PlaneTable imgs, int 
 }
 
 // Work table of the chamfer launch, written on the device (no host staging).
-__global__ void chamfer_table_kernel(ChamferWork* work, const uint8_t* cov, float* dist, const int* boxes, int c,
-                                     int w, int h) {
+__global__ void chamfer_table_kernel(ChamferWork* work, const uint8_t* cov, uint8_t* scov, float* dist,
+                                     const int* boxes, int c, int w, int h) {
     const int i = threadIdx.x;
     const size_t n = static_cast<size_t>(w) * h;
-    const size_t np = static_cast<size_t>(DistPitch(w)) * h;
-    if (i < c) work[i] = ChamferWork{cov + n * i, dist + np * i, boxes + 4 * i, w, h};
+    const size_t np = static_cast<size_t>(DistPitch(w)) * ShearRows(w, h);
+    if (i < c) work[i] = ChamferWork{cov + n * i, scov + np * i, dist + np * i, boxes + 4 * i, w, h};
 }
 
 __global__ void box_init_kernel(int* boxes, int c) {
@@ -96,14 +96,15 @@ __global__ void box_init_kernel(int* boxes, int c) {
 }
 
 // ----------------------------------------------------------- ownership --
-// Distances outside a contributor's coverage box are 0 (never written).
+// Distances outside a contributor's coverage box are 0 (never written);
+// the planes are the chamfer's sheared ones (shift = ChamferShear(w)).
 __global__ void __launch_bounds__(256) ownership_kernel(const float* __restrict__ dist, const int* __restrict__ boxes,
-                                                        int c, int w, size_t n, int32_t* __restrict__ owner) {
+                                                        int c, int w, size_t n, int shift, int32_t* __restrict__ owner) {
     const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int x = static_cast<int>(p % w), y = static_cast<int>(p / w);
-    const size_t np = static_cast<size_t>(DistPitch(w)) * (n / w);
-    const size_t q = static_cast<size_t>(y) * DistPitch(w) + x;
+    const size_t np = static_cast<size_t>(DistPitch(w)) * ShearRows(w, static_cast<int>(n / w));
+    const size_t q = ShearAt(x, y, shift, DistPitch(w));
     int best = -1;
     float bd = 0.0f;
     for (int i = 0; i < c; ++i) {
@@ -405,7 +406,8 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     // workspace layout
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
-    const size_t oCov = take(n * c), oDist = take(static_cast<size_t>(DistPitch(w)) * h * c * 4);
+    const size_t sheared = static_cast<size_t>(DistPitch(w)) * ShearRows(w, h);
+    const size_t oCov = take(n * c), oScov = take(sheared * c), oDist = take(sheared * c * 4);
     const size_t oOwner = take(n * 4), oBox = take(4 * 4 * c);
     const size_t oLohi = take(8 * static_cast<size_t>(maxLines)), oPrev = take(4 * static_cast<size_t>(maxLines));
     const size_t oChosen = take(4 * static_cast<size_t>(maxLines)), oBack = take(64 * static_cast<size_t>(maxLines));
@@ -421,6 +423,7 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     auto* base = static_cast<uint8_t*>(ctx_scratch(ctx, kSlotBlend, off, st));
     if (!st.ok()) return;
     auto* cov = base + oCov;
+    auto* scov = base + oScov;
     auto* dist = reinterpret_cast<float*>(base + oDist);
     auto* owner = reinterpret_cast<int32_t*>(base + oOwner);
     auto* boxes = reinterpret_cast<int*>(base + oBox);
@@ -436,7 +439,7 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     // 2. chamfer, forward then backward (one block per contributor)
     {
         auto* dwork = reinterpret_cast<ChamferWork*>(base + oWork);
-        BL_LAUNCH(st, "chamfer_table", 0, chamfer_table_kernel<<<1, 64, 0, s>>>(dwork, cov, dist, boxes, c, w, h));
+        BL_LAUNCH(st, "chamfer_table", 0, chamfer_table_kernel<<<1, 64, 0, s>>>(dwork, cov, scov, dist, boxes, c, w, h));
         {
             ProfScope ps(ctx, s, "chamfer", 13.0 * n * c);
             if (launch_chamfer(dwork, c, w, s) != hipSuccess) {
@@ -446,7 +449,8 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
         }
     }
     // 3. ownership
-    BL_LAUNCH(st, "ownership", 4.0 * n * (c + 1), ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, boxes, c, w, n, owner));
+    BL_LAUNCH(st, "ownership", 4.0 * n * (c + 1), ownership_kernel<<<GridFor(n, 256), 256, 0, s>>>(dist, boxes, c, w, n,
+                                                                                              ChamferShear(w), owner));
     // 4. pairwise seams in the reference's order
     int* counters = reinterpret_cast<int*>(base + oCnt);
     BL_LAUNCH(st, "fill_i32", 0, fill_i32_kernel<<<4, 256, 0, s>>>(counters, size_t(c) * c, 0));

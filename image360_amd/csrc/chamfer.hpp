@@ -19,6 +19,13 @@
 //
 // Each thread prefetches its next P rows of input into registers, so the
 // per-step critical path holds no global-memory latency.
+//
+// Sheared planes: at a given step, strip t works on row y0 + (T - t)/2, so
+// in a row-major plane the 64 lanes of a wavefront would touch 64 different
+// rows.  Distances (and a copy of the coverage the forward pass reads) are
+// therefore stored sheared: pixel (x, y) lives at row y + ((x / S) >> 1) of
+// its plane.  All strips of one parity, one wavefront, then touch a single
+// row at every step, in both passes, and their loads and stores coalesce.
 #pragma once
 
 #include <cstdlib>
@@ -29,23 +36,75 @@ namespace i360 {
 
 struct ChamferWork {
     const uint8_t* cov;  // w*h coverage (0/1), row pitch w
-    float* dist;         // distances, row pitch DistPitch(w), written inside the box only
+    uint8_t* scov;       // sheared copy of the coverage box (written by chamfer_shear_kernel)
+    float* dist;         // sheared distances, written inside the box only
     const int* box;      // x0, y0, x1, y1 (x1 < 0: nothing covered)
     int w, h;
 };
 
-// Row pitch (floats) of a distance plane: a multiple of 4 so every strip
-// is one 16-byte access.
-__host__ __device__ inline int DistPitch(int w) { return (w + 3) & ~3; }
+// Row pitch (elements) of the sheared planes: a multiple of 16, so a strip
+// of 4, 8 or 16 columns starting at a multiple of its width never crosses
+// its row.
+__host__ __device__ inline int DistPitch(int w) { return (w + 15) & ~15; }
+
+// Narrowest strip a launch may pick for chunks of width w (one strip per
+// thread, at most 1024 threads).
+__host__ __device__ inline int ChamferMinStrip(int w) { return w <= 4096 ? 4 : (w <= 8192 ? 8 : 16); }
+
+// Rows of a sheared plane of an h-row chunk: room for the shear of the
+// last column at the narrowest strip.
+__host__ __device__ inline int ShearRows(int w, int h) { return h + (((w - 1) / ChamferMinStrip(w)) >> 1) + 1; }
+
+// Element offset of pixel (x, y) in a sheared plane; shift = log2(S) + 1.
+__host__ __device__ inline size_t ShearAt(int x, int y, int shift, int pitch) {
+    return static_cast<size_t>(y + (x >> shift)) * pitch + x;
+}
+__host__ __device__ constexpr int ShearShift(int S) { return S == 4 ? 3 : (S == 8 ? 4 : 5); }
 
 constexpr float kChamferDiag = 1.41421356f;  // SeamBlend.cpp:209
 
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint32_t ld_u32(const void* p) { return *gptr(static_cast<const uint32_t*>(p)); }
 __device__ __forceinline__ uint32_t ld_u8(const uint8_t* p) { return *gptr(p); }
 __device__ __forceinline__ f4v ld_f4(const float* p) { return *gptr(reinterpret_cast<const f4v*>(p)); }
 __device__ __forceinline__ void st_f4(float* p, f4v v) { *gptr(reinterpret_cast<f4v*>(p)) = v; }
+
+// Copies each item's coverage box, widened to whole 16-column groups, into
+// its sheared plane, 8 bytes per thread (an 8-column group never straddles
+// two shear rows for S >= 4); columns at or past w are written as 0.
+// Grid: (blocks per item, items).
+template <int S>
+__global__ void __launch_bounds__(256) chamfer_shear_kernel(const ChamferWork* __restrict__ work) {
+    const ChamferWork wk = work[blockIdx.y];
+    const int x0 = wk.box[0], y0 = wk.box[1], x1 = wk.box[2], y1 = wk.box[3];
+    if (x1 < 0) return;
+    const int w = wk.w, cp = DistPitch(w);
+    const int ga = x0 & ~15, gb = min((x1 + 16) & ~15, cp);
+    const int ng = (gb - ga) >> 3;
+    const size_t total = static_cast<size_t>(ng) * (y1 - y0 + 1);
+    const bool aligned = (w & 7) == 0;
+    for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const int y = y0 + static_cast<int>(e / ng), x = ga + 8 * static_cast<int>(e % ng);
+        const uint8_t* c = wk.cov + static_cast<size_t>(y) * w + x;
+        uint32_t lo = 0, hi = 0;
+        if (aligned && x + 8 <= w) {
+            const u2v q = *gptr(reinterpret_cast<const u2v*>(c));
+            lo = q.x; hi = q.y;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (x + k < w) lo |= ld_u8(c + k) << (8 * k);
+                if (x + 4 + k < w) hi |= ld_u8(c + 4 + k) << (8 * k);
+            }
+        }
+        u2v v;
+        v.x = lo; v.y = hi;
+        *gptr(reinterpret_cast<u2v*>(wk.scov + ShearAt(x, y, ShearShift(S), cp))) = v;
+    }
+}
 
 // Block layout: the first half of the wavefronts owns the even strips, the
 // second half the odd ones.  Strips of one parity are busy on the same
@@ -67,8 +126,8 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
     const bool active = t < nact;
     const int xs = xa + t * S;
     const int dp = DistPitch(w);
-    const int xc = min(xs, dp - S);     // an in-bounds strip for inactive threads' loads
-    const int xb = min(xs, w - 1);
+    const int xl = xa + min(t, nact - 1) * S;  // load strip: inactive threads load the last active one
+    constexpr int SH = ShearShift(S);
     const float kInf = static_cast<float>(w + h);
     const int lag = FWD ? t : (nact - 1 - t);
     const int base = lag >> 1, par = lag & 1;
@@ -77,25 +136,32 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
 
     // raw prefetched row data; converted only when the row is processed so
     // that no instruction depends on a load until its row's step
-    //   FWD, vector path: packed coverage bytes in ring[k][0 .. S/4)
-    //   FWD, scalar path: one coverage byte per entry
+    //   FWD: the strip's S coverage bytes, packed four to a word
     //   BWD: the forward distance bits
-    uint32_t ring[P][S];  // rows of the current superstep
-    uint32_t next[P][S];  // rows of the next superstep, in flight
+    constexpr int RW = FWD ? S / 4 : S;
+    uint32_t ring[P][RW];  // rows of the current superstep
+    uint32_t next[P][RW];  // rows of the next superstep, in flight
     float prev[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) prev[s] = 0.0f;
 
     // loads are unconditional (rows and strips clamped into the box) so
     // that no register is merged across a branch while its load is in flight
-    auto load_row = [&](int r, uint32_t (&dst)[S]) {
+    auto load_row = [&](int r, uint32_t (&dst)[RW]) {
         const int y = FWD ? (y0 + r) : (y1 - r);
-        if (FWD) {
-            const uint8_t* c = wk.cov + static_cast<size_t>(y) * w;
-#pragma unroll
-            for (int s = 0; s < S; ++s) dst[s] = ld_u8(c + min(xb + s, w - 1));
+        if constexpr (FWD) {
+            const uint8_t* c = wk.scov + ShearAt(xl, y, SH, dp);
+            if constexpr (S == 4) {
+                dst[0] = *gptr(reinterpret_cast<const uint32_t*>(c));
+            } else if constexpr (S == 8) {
+                const u2v q = *gptr(reinterpret_cast<const u2v*>(c));
+                dst[0] = q.x; dst[1] = q.y;
+            } else {
+                const u4v q = *gptr(reinterpret_cast<const u4v*>(c));
+                dst[0] = q.x; dst[1] = q.y; dst[2] = q.z; dst[3] = q.w;
+            }
         } else {
-            const float* d = wk.dist + static_cast<size_t>(y) * dp + xc;
+            const float* d = wk.dist + ShearAt(xl, y, SH, dp);
 #pragma unroll
             for (int g = 0; g < S / 4; ++g) {
                 const f4v q = ld_f4(d + 4 * g);
@@ -106,16 +172,16 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
             }
         }
     };
-    auto init_of = [&](const uint32_t (&src)[S], int s) -> float {
+    auto init_of = [&](const uint32_t (&src)[RW], int s) -> float {
         if (xs + s >= w) return 0.0f;
-        if (!FWD) return __uint_as_float(src[s]);
-        return src[s] ? kInf : 0.0f;
+        if constexpr (FWD) return ((src[s >> 2] >> (8 * (s & 3))) & 0xffu) ? kInf : 0.0f;
+        else return __uint_as_float(src[s]);
     };
     // superstep j (steps 2Pj .. 2Pj+2P-1) processes rows Pj - base + k, k < P,
     // in ring slot k; the rows of superstep j+1 are loaded at the start of
     // superstep j and rotated in at its end, so each load has a whole
     // superstep to arrive and no step waits on memory
-    auto fetch = [&](int j, uint32_t (&dst)[P][S]) {
+    auto fetch = [&](int j, uint32_t (&dst)[P][RW]) {
 #pragma unroll
         for (int k = 0; k < P; ++k) load_row(min(max(P * j + k - base, 0), rows - 1), dst[k]);
     };
@@ -169,7 +235,7 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
                             cur[s] = (x < w) ? b : 0.0f;
                         }
                     }
-                    float* row = wk.dist + static_cast<size_t>(y) * dp + xs;
+                    float* row = wk.dist + ShearAt(xs, y, SH, dp);
 #pragma unroll
                     for (int g = 0; g < S / 4; ++g) {
                         f4v v;
@@ -187,7 +253,7 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
 #pragma unroll
         for (int k = 0; k < P; ++k)
 #pragma unroll
-            for (int s = 0; s < S; ++s) ring[k][s] = next[k][s];
+            for (int s = 0; s < RW; ++s) ring[k][s] = next[k][s];
     }
 }
 
@@ -212,18 +278,31 @@ inline int ChamferStrip(int w, int want = 0) {
     return s;
 }
 
-// Launches forward then backward passes over `n` work items (device table).
+// log2(S) + 1 for the strip width a launch over chunks up to max_w wide
+// picks: the shear of the distance planes it writes (ShearAt).
+inline int ChamferShear(int max_w, int want_strip = 0) { return ShearShift(ChamferStrip(max_w, want_strip)); }
+
+// Blocks per item of the shear pre-pass.
+constexpr int kShearBlocks = 32;
+
+// Shears the coverage boxes, then launches the forward and backward passes
+// over `n` work items (device table).  Every plane must hold
+// ShearRows(w, h) rows of DistPitch(w) elements.
 inline hipError_t launch_chamfer(const ChamferWork* d_work, int n, int max_w, hipStream_t s, int want_strip = 0) {
     if (n <= 0) return hipSuccess;
     const int S = ChamferStrip(max_w, want_strip);
     const int nt = ChamferThreads(max_w, S);
+    const dim3 sg(kShearBlocks, n);
     if (S == 4) {
+        chamfer_shear_kernel<4><<<sg, 256, 0, s>>>(d_work);
         chamfer_kernel<4, 8, true><<<n, nt, 0, s>>>(d_work);
         chamfer_kernel<4, 8, false><<<n, nt, 0, s>>>(d_work);
     } else if (S == 8) {
+        chamfer_shear_kernel<8><<<sg, 256, 0, s>>>(d_work);
         chamfer_kernel<8, 4, true><<<n, nt, 0, s>>>(d_work);
         chamfer_kernel<8, 4, false><<<n, nt, 0, s>>>(d_work);
     } else {
+        chamfer_shear_kernel<16><<<sg, 256, 0, s>>>(d_work);
         chamfer_kernel<16, 2, true><<<n, nt, 0, s>>>(d_work);
         chamfer_kernel<16, 2, false><<<n, nt, 0, s>>>(d_work);
     }

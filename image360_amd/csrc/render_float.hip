@@ -512,9 +512,11 @@ __global__ void __launch_bounds__(256) rmap_error_kernel(const RItem* __restrict
 // ---------------------------------------------------------- ownership ---
 // One thread per pixel of a 64x16 tile (1024 threads): a thread's work is
 // one row's pixel, so no row waits behind another row's store.
+// Distances are read from the chamfer's sheared planes (shift = the
+// launch's ChamferShear).
 __global__ void __launch_bounds__(1024) rowner_kernel(const RItem* __restrict__ items,
                                                       const RChunk* __restrict__ chunks,
-                                                      const RTile* __restrict__ tiles, int ntiles) {
+                                                      const RTile* __restrict__ tiles, int ntiles, int shift) {
     for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
         const RTile tl = tiles[e];
         const RChunk& ch = chunks[tl.id];
@@ -528,7 +530,7 @@ __global__ void __launch_bounds__(1024) rowner_kernel(const RItem* __restrict__ 
             for (int k = 0; k < ch.count; ++k) {
                 const RItem& it = items[ch.first + k];
                 if (!in_box(it.box, x, y)) continue;
-                const float v = gptr(it.dist)[static_cast<size_t>(y) * DistPitch(ch.w) + x];
+                const float v = gptr(it.dist)[ShearAt(x, y, shift, DistPitch(ch.w))];
                 if (v > bd) { bd = v; best = k; }
             }
             gptr(ch.owner)[p] = best;
@@ -1246,7 +1248,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     // layout offsets are relative; absolute pointers are patched after allocation
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
-    struct Pending { size_t warped, cov, dist, nd[kMaxLevels], wt[kMaxLevels]; };
+    struct Pending { size_t warped, cov, scov, dist, nd[kMaxLevels], wt[kMaxLevels]; };
     struct PendingC { size_t boxes, owner, recon[kMaxLevels], lohi, prevl, chosen, back, counters; };
     std::vector<Pending> pit;
     std::vector<PendingC> pch;
@@ -1333,7 +1335,9 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             Pending pd{};
             pd.warped = take(n * 4 * sS);
             pd.cov = take(n);
-            pd.dist = take(static_cast<size_t>(DistPitch(cc.width)) * cc.height * 4);
+            const size_t sheared = static_cast<size_t>(DistPitch(cc.width)) * ShearRows(cc.width, cc.height);
+            pd.scov = take(sheared);
+            pd.dist = take(sheared * 4);
             for (int l = 1; l < ch.levels; ++l) {
                 const size_t m = static_cast<size_t>(ch.lw[l]) * ch.lh[l];
                 pd.nd[l] = take(m * 4 * sS);
@@ -1403,6 +1407,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         RItem& it = P.items[i];
         it.warped = reinterpret_cast<void*>(pit[i].warped);
         it.cov = reinterpret_cast<uint8_t*>(pit[i].cov);
+        it.scov = reinterpret_cast<uint8_t*>(pit[i].scov);
         it.dist = reinterpret_cast<float*>(pit[i].dist);
         for (int l = 1; l < kMaxLevels; ++l) {
             it.nd[l] = reinterpret_cast<void*>(pit[i].nd[l]);
@@ -1490,7 +1495,7 @@ static void patch_plan(Plan& P, uint8_t* base) {
         fix(P.sbufs.tflag); fix(P.sbufs.cflag);
     }
     for (auto& it : P.items) {
-        fix(it.warped); fix(it.cov); fix(it.dist);
+        fix(it.warped); fix(it.cov); fix(it.scov); fix(it.dist);
         for (int l = 1; l < kMaxLevels; ++l) { fix(it.nd[l]); fix(it.wt[l]); }
     }
     for (auto& ch : P.chunks) {
@@ -1501,7 +1506,7 @@ static void patch_plan(Plan& P, uint8_t* base) {
     for (size_t i = 0; i < P.items.size(); ++i) {
         const RItem& it = P.items[i];
         const RChunk& ch = P.chunks[it.chunk];
-        P.chamfer[i] = ChamferWork{it.cov, it.dist, it.box, ch.w, ch.h};
+        P.chamfer[i] = ChamferWork{it.cov, it.scov, it.dist, it.box, ch.w, ch.h};
     }
 }
 
@@ -1647,7 +1652,8 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
         }
     }
     span(P.own, off, n);
-    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n, 2), 1024, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n, 2), 1024, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
+                                                                                   ChamferShear(P.max_w, ctx->tune_strip)));
     // seams: the k-th pair of every chunk in range, in pair order
     {
         for (size_t k = 0; k < T.round_chunks.size(); ++k) {
@@ -2063,7 +2069,7 @@ int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int 
         return st.code;
     }
     // only the item table and warp tiles are used; no workspace needed
-    for (auto& it : P.items) { it.warped = nullptr; it.cov = nullptr; it.dist = nullptr; }
+    for (auto& it : P.items) { it.warped = nullptr; it.cov = nullptr; it.scov = nullptr; it.dist = nullptr; }
     DevTables T;
     upload_tables(ctx, P, T, ctx->stream, st, kSlotRender + 3);
     auto* bits = static_cast<unsigned int*>(ctx_scratch(ctx, kSlotRender + 2, 256, st));
