@@ -411,6 +411,7 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
     const size_t oOwner = take(n * 4), oBox = take(4 * 4 * c);
     const size_t oLohi = take(8 * static_cast<size_t>(maxLines)), oPrev = take(4 * static_cast<size_t>(maxLines));
     const size_t oChosen = take(4 * static_cast<size_t>(maxLines)), oBack = take(64 * static_cast<size_t>(maxLines));
+    const size_t oCost = take(64 * sizeof(float) * static_cast<size_t>(maxLines));
     const size_t oWork = take(sizeof(ChamferWork) * I360HIP_MAX_CONTRIBUTORS);
     const size_t oCnt = take(sizeof(int) * I360HIP_MAX_CONTRIBUTORS * I360HIP_MAX_CONTRIBUTORS);
     size_t oNd[I360HIP_MAX_BANDS][I360HIP_MAX_CONTRIBUTORS] = {}, oWt[I360HIP_MAX_BANDS][I360HIP_MAX_CONTRIBUTORS] = {};
@@ -459,12 +460,16 @@ void blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int c, int
             int* lohi = reinterpret_cast<int*>(base + oLohi);
             int* chosen = reinterpret_cast<int*>(base + oChosen);
             int* cnt = counters + i * c + j;
+            auto* costs = reinterpret_cast<float*>(base + oCost);
+            const unsigned sb = (maxLines + kScanLinesPerBlock - 1) / kScanLinesPerBlock;
             BL_LAUNCH(st, "seam_scan", 0,
-                      seam_scan_kernel<<<(maxLines + kScanLinesPerBlock - 1) / kScanLinesPerBlock, 256, 0, s>>>(
-                          cov + n * i, cov + n * j, boxes, i, j, w, owner, lohi, chosen, cnt));
-            BL_LAUNCH(st, "seam_dp", 0, seam_dp_kernel<U16Img, float><<<1, 1024, 0, s>>>(
-                              U16Img{warped[i]}, U16Img{warped[j]}, cov + n * i, cov + n * j, boxes, i, j, w, owner,
-                              lohi, reinterpret_cast<int*>(base + oPrev), chosen, base + oBack, cnt));
+                      seam_scan_kernel<U16Img, float><<<sb, 256, 0, s>>>(U16Img{warped[i]}, U16Img{warped[j]},
+                                                                         cov + n * i, cov + n * j, boxes, i, j, w,
+                                                                         owner, lohi, chosen, cnt, costs));
+            BL_LAUNCH(st, "seam_dp", 0, seam_dp_kernel<U16Img, float><<<1, kDpThreads, 0, s>>>(
+                              boxes, i, j, costs, lohi, reinterpret_cast<int*>(base + oPrev), chosen, base + oBack, cnt));
+            BL_LAUNCH(st, "seam_reassign", 0,
+                      seam_reassign_kernel<><<<sb, 256, 0, s>>>(boxes, i, j, w, owner, lohi, chosen, cnt));
         }
     if (owner_out) BL_LAUNCH(st, "copy_owner", 8.0 * n, copy_owner_kernel<<<GridFor(n, 256), 256, 0, s>>>(owner, owner_out, n));
     // 5. pyramids of every contributor (contributors without owned pixels

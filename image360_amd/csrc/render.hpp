@@ -53,6 +53,7 @@ struct RChunk {
     int* prevl;
     int* chosen;
     uint8_t* back;
+    uint8_t* costs;  // seam band costs: 64 values of up to 8 bytes per line and slot
     int* counters;  // per-pair contested-line counts (count*count ints)
     // learned surrogate (SURROGATE form and its training-data export)
     int sk;           // pyramid level the surrogate predicts (levels-3); 0 = none for this chunk

@@ -1249,7 +1249,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
     struct Pending { size_t warped, cov, scov, dist, nd[kMaxLevels], wt[kMaxLevels]; };
-    struct PendingC { size_t boxes, owner, recon[kMaxLevels], lohi, prevl, chosen, back, counters; };
+    struct PendingC { size_t boxes, owner, recon[kMaxLevels], lohi, prevl, chosen, back, costs, counters; };
     std::vector<Pending> pit;
     std::vector<PendingC> pch;
     // the per-chunk forms reuse one chunk's worth of buffers
@@ -1366,6 +1366,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         const int nslots = schedule_pairs(pr);
         const size_t lines = static_cast<size_t>(std::max(cc.width, cc.height)) * std::max(1, nslots);
         pc.lohi = take(8 * lines); pc.prevl = take(4 * lines); pc.chosen = take(4 * lines); pc.back = take(64 * lines);
+        pc.costs = take(64 * 8 * lines);
         pc.counters = take(4 * static_cast<size_t>(std::max(1, ch.count)) * std::max(1, ch.count));
         pch.push_back(pc);
         P.chunks.push_back(ch);
@@ -1423,6 +1424,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         ch.prevl = reinterpret_cast<int*>(pch[c].prevl);
         ch.chosen = reinterpret_cast<int*>(pch[c].chosen);
         ch.back = reinterpret_cast<uint8_t*>(pch[c].back);
+        ch.costs = reinterpret_cast<uint8_t*>(pch[c].costs);
         ch.counters = reinterpret_cast<int*>(pch[c].counters);
     }
     // tiles, one stage after another so that each stage's tiles are
@@ -1499,7 +1501,7 @@ static void patch_plan(Plan& P, uint8_t* base) {
         for (int l = 1; l < kMaxLevels; ++l) { fix(it.nd[l]); fix(it.wt[l]); }
     }
     for (auto& ch : P.chunks) {
-        fix(ch.boxes); fix(ch.owner); fix(ch.lohi); fix(ch.prevl); fix(ch.chosen); fix(ch.back); fix(ch.counters);
+        fix(ch.boxes); fix(ch.owner); fix(ch.lohi); fix(ch.prevl); fix(ch.chosen); fix(ch.back); fix(ch.costs); fix(ch.counters);
         for (int l = 1; l < kMaxLevels; ++l) fix(ch.recon[l]);
         for (int k = 0; k < ch.count; ++k) P.items[ch.first + k].box = ch.boxes + 4 * k;
     }
@@ -1574,6 +1576,7 @@ static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables
                 s.covI = a.cov; s.covJ = b.cov; s.boxes = ch.boxes; s.ci = p.i; s.cj = p.j; s.w = ch.w;
                 s.owner = ch.owner;
                 s.lohi = ch.lohi + 2 * sl; s.prevl = ch.prevl + sl; s.chosen = ch.chosen + sl; s.back = ch.back + 64 * sl;
+                s.costs = ch.costs + 64 * 8 * sl;
                 s.contested = ch.counters + p.i * ch.count + p.j;
                 out.push_back(s);
                 T.round_chunks[k].push_back(static_cast<int>(c));
@@ -1670,18 +1673,30 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
             const dim3 sg((maxLines + kScanLinesPerBlock - 1) / kScanLinesPerBlock, cnt);
             {
                 ProfScope ps_(ctx, s, "seam_scan", 0.0);
-                if (MM == kModeFP64) seam_scan_table_kernel<RgbaImg<double, double>><<<sg, 256, 0, s>>>(T.seam64 + first);
-                else if (MM == kModePlain) seam_scan_table_kernel<RgbaImg<float, float>><<<sg, 256, 0, s>>>(T.seam32 + first);
-                else seam_scan_table_kernel<MapImg><<<sg, 256, 0, s>>>(T.seamMap + first);
+                if (MM == kModeFP64)
+                    seam_scan_table_kernel<RgbaImg<double, double>, double><<<sg, 256, 0, s>>>(T.seam64 + first);
+                else if (MM == kModePlain)
+                    seam_scan_table_kernel<RgbaImg<float, float>, float><<<sg, 256, 0, s>>>(T.seam32 + first);
+                else
+                    seam_scan_table_kernel<MapImg, double><<<sg, 256, 0, s>>>(T.seamMap + first);
             }
             {
                 ProfScope ps_(ctx, s, "seam_dp", 0.0);
                 if (MM == kModeFP64)
-                    seam_dp_table_kernel<RgbaImg<double, double>, double><<<cnt, 1024, 0, s>>>(T.seam64 + first);
+                    seam_dp_table_kernel<RgbaImg<double, double>, double><<<cnt, kDpThreads, 0, s>>>(T.seam64 + first);
                 else if (MM == kModePlain)
-                    seam_dp_table_kernel<RgbaImg<float, float>, float><<<cnt, 1024, 0, s>>>(T.seam32 + first);
+                    seam_dp_table_kernel<RgbaImg<float, float>, float><<<cnt, kDpThreads, 0, s>>>(T.seam32 + first);
                 else
-                    seam_dp_table_kernel<MapImg, double><<<cnt, 1024, 0, s>>>(T.seamMap + first);
+                    seam_dp_table_kernel<MapImg, double><<<cnt, kDpThreads, 0, s>>>(T.seamMap + first);
+            }
+            {
+                ProfScope ps_(ctx, s, "seam_reassign", 0.0);
+                if (MM == kModeFP64)
+                    seam_reassign_table_kernel<RgbaImg<double, double>><<<sg, 256, 0, s>>>(T.seam64 + first);
+                else if (MM == kModePlain)
+                    seam_reassign_table_kernel<RgbaImg<float, float>><<<sg, 256, 0, s>>>(T.seam32 + first);
+                else
+                    seam_reassign_table_kernel<MapImg><<<sg, 256, 0, s>>>(T.seamMap + first);
             }
             if (hipGetLastError() != hipSuccess) {
                 st.set(I360HIP_KERNEL_LAUNCH_FAILED, "seam launch failed");

@@ -1,14 +1,18 @@
 // Pairwise DP seam refinement (WindowsApp.Core/SourceFiles/
 // SeamBlend.cpp:293-445), shared by the RGB48 and float renders.
 //
-// One block handles one contributor pair of one chunk: (1) the crossover
-// of the current owner map and the contested band of every line, one
-// wavefront per line with ballots; (2) the line-to-line DP on one
-// wavefront, lane k = candidate position lo+k, the previous line's values
-// broadcast lane by lane (readlane), candidates scanned in increasing order
-// with a strict < so the first minimum wins as in the reference; (3) the
-// backtrack through back-pointers staged in LDS from the last line down;
-// (4) the reassignment of I/J-owned pixels inside each contested band.
+// Three launches per pair (or per round of pairs in the batched form):
+//  scan     one wavefront per line: the crossover of the current owner map
+//           and the contested band around it (ballots), then the band's
+//           pixel costs |I - J|, written to a per-line cost row;
+//  dp       one block per pair: the line-to-line DP, lane k = candidate
+//           position lo+k; the previous line's candidates are split into
+//           kDpWaves ranges, one wavefront each (readlane broadcasts), and
+//           the partial minima are combined in range order with a strict <,
+//           so the first minimum wins as in the reference's serial scan;
+//           then the backtrack through back-pointers staged in LDS;
+//  reassign one wavefront per line: I/J-owned pixels inside each contested
+//           band go to the side of the chosen seam.
 // Img supplies rgb(p, M[3]) and the smoothness weight per pixel of seam
 // movement (2 in RGB48 units, SeamBlend.cpp:309; 2/65535 for [0,1] images).
 #pragma once
@@ -44,7 +48,9 @@ struct U16Img {
 };
 
 constexpr int kSeamRadius = 24;  // SeamBlend.cpp:319
-constexpr int kCostBatch = 64;
+constexpr int kDpWaves = 4;      // wavefronts of the DP kernel (one per SIMD)
+constexpr int kDpThreads = 64 * kDpWaves;
+constexpr int kDpLines = 16;     // cost rows staged in LDS per batch
 constexpr int kBackBatch = 256;
 
 __device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w, int line, int t) {
@@ -57,11 +63,14 @@ __device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w,
 // One wavefront per line, lanes along the line, four 64-pixel segments in
 // flight per lane; blocks cover lines, so all lines of a pair (and all
 // pairs of a round) scan in parallel.
-template <int U = 4>
-__device__ __forceinline__ void seam_scan_body(const uint8_t* __restrict__ covI, const uint8_t* __restrict__ covJ,
-                                               const int* __restrict__ boxes, int ci, int cj, int w,
-                                               const int32_t* __restrict__ owner, int* __restrict__ lohi,
-                                               int* __restrict__ chosen, int* __restrict__ contested, int line) {
+// The band's pixel costs sum_c |I_c - J_c| (SeamBlend.cpp:371-378) go to
+// costs[line * 64 + k] for band position lo + k.
+template <class Img, typename M, int U = 4>
+__device__ __forceinline__ void seam_scan_body(Img imgI, Img imgJ, const uint8_t* __restrict__ covI,
+                                               const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
+                                               int ci, int cj, int w, const int32_t* __restrict__ owner,
+                                               int* __restrict__ lohi, int* __restrict__ chosen,
+                                               int* __restrict__ contested, M* __restrict__ costs, int line) {
     const int* bI = boxes + 4 * ci;
     const int* bJ = boxes + 4 * cj;
     const auto* gbI = gptr(bI);
@@ -118,6 +127,16 @@ __device__ __forceinline__ void seam_scan_body(const uint8_t* __restrict__ covI,
             hi = a + 63 - __clzll(static_cast<long long>(m));
         }
     }
+    if (lane <= hi - lo) {
+        const size_t p = seam_at(rowScan, ox0, oy0, w, line, lo + lane);
+        M x[3], y[3];
+        imgI.rgb(p, x);
+        imgJ.rgb(p, y);
+        M cost = M(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cost = cost + fabs(x[c] - y[c]);
+        gptr(costs)[static_cast<size_t>(line) * 64 + lane] = cost;
+    }
     if (lane == 0) {
         gptr(lohi)[2 * line] = lo;
         gptr(lohi)[2 * line + 1] = hi;
@@ -129,106 +148,142 @@ __device__ __forceinline__ void seam_scan_body(const uint8_t* __restrict__ covI,
 // Lines per block of the scan kernel (one wavefront each).
 constexpr int kScanLinesPerBlock = 4;
 
-template <int Dummy = 0>
-__global__ void __launch_bounds__(256) seam_scan_kernel(const uint8_t* __restrict__ covI,
+template <class Img, typename M>
+__global__ void __launch_bounds__(256) seam_scan_kernel(Img imgI, Img imgJ, const uint8_t* __restrict__ covI,
                                                         const uint8_t* __restrict__ covJ,
                                                         const int* __restrict__ boxes, int ci, int cj, int w,
                                                         const int32_t* __restrict__ owner, int* __restrict__ lohi,
-                                                        int* __restrict__ chosen, int* __restrict__ contested) {
+                                                        int* __restrict__ chosen, int* __restrict__ contested,
+                                                        M* __restrict__ costs) {
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body(covI, covJ, boxes, ci, cj, w, owner, lohi, chosen, contested, line);
+    seam_scan_body<Img, M>(imgI, imgJ, covI, covJ, boxes, ci, cj, w, owner, lohi, chosen, contested, costs, line);
 }
 
-template <class Img, typename M>
-__device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, Img imgI, Img imgJ,
-                                                       const uint8_t* __restrict__ covI,
-                                                       const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
-                                                       int ci, int cj, int w, int32_t* __restrict__ owner,
-                                                       int* __restrict__ lohi, int* __restrict__ prevl,
-                                                       int* __restrict__ chosen, uint8_t* __restrict__ back) {
-    const int* bI = boxes + 4 * ci;
-    const int* bJ = boxes + 4 * cj;
-    if (bI[2] < 0 || bJ[2] < 0) return;  // an empty coverage box
-    const int ox0 = max(bI[0], bJ[0]), oy0 = max(bI[1], bJ[1]);
+// Overlap of the pair's coverage boxes and its scan direction (the
+// reference's rowScan, SeamBlend.cpp:313-323); false when the boxes do
+// not overlap.
+struct SeamGeom {
+    int ox0, oy0, lines;
+    bool rowScan, iLow;
+};
+__device__ __forceinline__ bool seam_geom(const int* __restrict__ boxes, int ci, int cj, SeamGeom& g) {
+    const auto* bI = gptr(boxes + 4 * ci);
+    const auto* bJ = gptr(boxes + 4 * cj);
+    if (bI[2] < 0 || bJ[2] < 0) return false;  // an empty coverage box
+    g.ox0 = max(bI[0], bJ[0]);
+    g.oy0 = max(bI[1], bJ[1]);
     const int ox1 = min(bI[2], bJ[2]), oy1 = min(bI[3], bJ[3]);
-    if (ox0 > ox1 || oy0 > oy1) return;
-    const int bw = ox1 - ox0 + 1, bh = oy1 - oy0 + 1;
-    const bool rowScan = bw >= bh;
-    const float cI = rowScan ? static_cast<float>(bI[0] + bI[2]) * 0.5f : static_cast<float>(bI[1] + bI[3]) * 0.5f;
-    const float cJ = rowScan ? static_cast<float>(bJ[0] + bJ[2]) * 0.5f : static_cast<float>(bJ[1] + bJ[3]) * 0.5f;
-    const bool iLow = cI <= cJ;
-    const int lines = rowScan ? bh : bw;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    if (g.ox0 > ox1 || g.oy0 > oy1) return false;
+    const int bw = ox1 - g.ox0 + 1, bh = oy1 - g.oy0 + 1;
+    g.rowScan = bw >= bh;
+    const float cI = g.rowScan ? static_cast<float>(bI[0] + bI[2]) * 0.5f : static_cast<float>(bI[1] + bI[3]) * 0.5f;
+    const float cJ = g.rowScan ? static_cast<float>(bJ[0] + bJ[2]) * 0.5f : static_cast<float>(bJ[1] + bJ[3]) * 0.5f;
+    g.iLow = cI <= cJ;
+    g.lines = g.rowScan ? bh : bw;
+    return true;
+}
 
-    // Phase 1 (crossover and band of every line) ran in seam_scan_kernel,
-    // which also counted the contested lines; none means no DP and no
-    // reassignment (SeamBlend.cpp:409).
+// DP and backtrack of one pair; blockDim.x == kDpThreads.  Writes the
+// chosen seam position of every contested line (SeamBlend.cpp:362-407).
+template <class Img, typename M>
+__device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, const int* __restrict__ boxes,
+                                             int ci, int cj, const M* __restrict__ costs,
+                                             const int* __restrict__ lohi, int* __restrict__ prevl,
+                                             int* __restrict__ chosen, uint8_t* __restrict__ back) {
+    SeamGeom g;
+    if (!seam_geom(boxes, ci, cj, g)) return;
+    const int lines = g.lines;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // none contested: no DP and no reassignment (SeamBlend.cpp:409)
     if (*gptr(contested) == 0) return;
-    __syncthreads();
 
-    // Phase 2: DP over contested lines (SeamBlend.cpp:362-407).
-    __shared__ M costs[kCostBatch][64];
-    __shared__ int slo[kCostBatch], shi[kCostBatch];
-    __shared__ int s_last;
+    // Phase 2: DP over contested lines.  Cost rows and bands of kDpLines
+    // lines are staged in LDS (double buffered: the next batch is loaded
+    // into registers while the current one runs).
+    constexpr int kSpan = 64 / kDpWaves;
+    constexpr int kPer = kDpLines * 64 / kDpThreads;
+    __shared__ M scost[2][kDpLines][64];
+    __shared__ int sband[2][kDpLines][2];
+    __shared__ M pbest[2][kDpWaves][64];
+    __shared__ int pbk[2][kDpWaves][64];
     __shared__ M s_dpfinal[64];
-    __shared__ int s_lastw;
-    int last = -1, lastLo = 0, lastW = 0;
-    M dpv = M(0);  // this lane's dp value on the last contested line
-    for (int b0 = 0; b0 < lines; b0 += kCostBatch) {
-        const int nb = min(kCostBatch, lines - b0);
-        for (int idx = threadIdx.x; idx < nb * 64; idx += blockDim.x) {
-            const int l = idx >> 6, k = idx & 63;
-            const int lo = gptr(lohi)[2 * (b0 + l)], hi = gptr(lohi)[2 * (b0 + l) + 1];
-            if (k == 0) { slo[l] = lo; shi[l] = hi; }
-            if (k <= hi - lo) {
-                const size_t p = seam_at(rowScan, ox0, oy0, w, b0 + l, lo + k);
-                M a[3], b[3];
-                imgI.rgb(p, a);
-                imgJ.rgb(p, b);
-                M cost = M(0);
+    M creg[kPer];
+    int breg = 0;
+    auto fetch = [&](int b0) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) cost = cost + fabs(a[c] - b[c]);
-                costs[l][k] = cost;
-            }
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = static_cast<int>(threadIdx.x) + u * kDpThreads;
+            const int l = min(b0 + (idx >> 6), lines - 1);
+            creg[u] = gptr(costs)[static_cast<size_t>(l) * 64 + (idx & 63)];
         }
-        __syncthreads();
-        if (wave == 0) {
-            for (int l = 0; l < nb; ++l) {
-                const int lo = slo[l], hi = shi[l];
-                if (hi < lo) continue;
-                const int lw = hi - lo + 1;
-                const int line = b0 + l;
-                M v = M(0);
-                int bp = -1;
-                if (last < 0) {
-                    if (lane < lw) v = costs[l][lane];
-                } else {
-                    const int t = lo + lane;
-                    M best = SeamMax<M>::value;
-                    for (int pk = 0; pk < lastW; ++pk) {
-                        const M dprev = lane_value(dpv, pk);
-                        const M cand = dprev + Img::smooth() * fabs(static_cast<M>(t - (lastLo + pk)));
-                        if (cand < best) { best = cand; bp = pk; }
-                    }
-                    if (lane < lw) v = costs[l][lane] + best;
+        const int tid = static_cast<int>(threadIdx.x);
+        if (tid < 2 * kDpLines) breg = gptr(lohi)[2 * min(b0 + (tid >> 1), lines - 1) + (tid & 1)];
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = static_cast<int>(threadIdx.x) + u * kDpThreads;
+            scost[buf][idx >> 6][idx & 63] = creg[u];
+        }
+        if (threadIdx.x < 2 * kDpLines) sband[buf][threadIdx.x >> 1][threadIdx.x & 1] = breg;
+    };
+    fetch(0);
+    commit(0);
+    __syncthreads();
+    int last = -1, lastLo = 0, lastW = 0, ord = 0;
+    M dpv = M(0);  // this lane's dp value on the last contested line (every wave holds the whole row)
+    for (int b0 = 0, bi = 0; b0 < lines; b0 += kDpLines, ++bi) {
+        const int buf = bi & 1;
+        const bool more = b0 + kDpLines < lines;
+        if (more) fetch(b0 + kDpLines);
+        const int nb = min(kDpLines, lines - b0);
+        for (int l = 0; l < nb; ++l) {
+            const int lo = sband[buf][l][0], hi = sband[buf][l][1];
+            if (hi < lo) continue;  // block-uniform
+            const int lw = hi - lo + 1;
+            const int line = b0 + l;
+            M v = M(0);
+            int bp = -1;
+            if (last < 0) {
+                if (lane < lw) v = scost[buf][l][lane];
+            } else {
+                const int t = lo + lane;
+                M best = SeamMax<M>::value;
+                const int p0 = wave * kSpan, p1 = min(p0 + kSpan, lastW);
+                for (int pk = p0; pk < p1; ++pk) {
+                    const M dprev = lane_value(dpv, pk);
+                    const M cand = dprev + Img::smooth() * fabs(static_cast<M>(t - (lastLo + pk)));
+                    if (cand < best) { best = cand; bp = pk; }
                 }
+                const int pb = ord & 1;
+                ++ord;
+                pbest[pb][wave][lane] = best;
+                pbk[pb][wave][lane] = bp;
+                lds_barrier();
+                best = pbest[pb][0][lane];
+                bp = pbk[pb][0][lane];
+#pragma unroll
+                for (int q = 1; q < kDpWaves; ++q) {
+                    const M bq = pbest[pb][q][lane];
+                    if (bq < best) { best = bq; bp = pbk[pb][q][lane]; }
+                }
+                if (lane < lw) v = scost[buf][l][lane] + best;
+            }
+            if (wave == 0) {
                 if (lane < lw) gptr(back)[static_cast<size_t>(line) * 64 + lane] = static_cast<uint8_t>(bp & 0xFF);
                 if (lane == 0) gptr(prevl)[line] = last;
-                dpv = v;
-                last = line;
-                lastLo = lo;
-                lastW = lw;
             }
+            dpv = v;
+            last = line;
+            lastLo = lo;
+            lastW = lw;
         }
+        if (more) commit(buf ^ 1);
         __syncthreads();
     }
-    if (wave == 0) {
-        if (lane == 0) { s_last = last; s_lastw = lastW; }
-        s_dpfinal[lane] = dpv;
-    }
+    if (last < 0) return;  // no contested pixel in this overlap (block-uniform)
+    if (wave == 0) s_dpfinal[lane] = dpv;
     __syncthreads();
-    last = s_last;
-    if (last < 0) return;  // no contested pixel in this overlap (uniform)
 
     // Phase 3: backtrack through back-pointers staged in LDS, from the end.
     __shared__ uint8_t sback[kBackBatch][64];
@@ -237,7 +292,7 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
     if (threadIdx.x == 0) {
         M best = SeamMax<M>::value;
         int bk = 0;
-        for (int k = 0; k < s_lastw; ++k)
+        for (int k = 0; k < lastW; ++k)
             if (s_dpfinal[k] < best) { best = s_dpfinal[k]; bk = k; }
         s_line = last;
         s_k = bk;
@@ -266,31 +321,45 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
         __syncthreads();
         if (s_line < 0) break;
     }
-    __syncthreads();
-
-    // Phase 4: reassign I/J-owned pixels inside each contested band.
-    for (int line = wave; line < lines; line += nwaves) {
-        const int ch = gptr(chosen)[line];
-        if (ch < 0) continue;
-        const int lo = gptr(lohi)[2 * line], hi = gptr(lohi)[2 * line + 1];
-        for (int t = lo + lane; t <= hi; t += 64) {
-            const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
-            const int o = gptr(owner)[p];
-            if (o != ci && o != cj) continue;
-            gptr(owner)[p] = ((t < ch) == iLow) ? ci : cj;
-        }
-    }
 }
 
+// Phase 4 for one line (one wavefront): reassign I/J-owned pixels inside
+// the contested band to the side of the chosen seam (SeamBlend.cpp:409-437).
+__device__ __forceinline__ void seam_reassign_body(const int* __restrict__ boxes, int ci, int cj, int w,
+                                                   int32_t* __restrict__ owner, const int* __restrict__ lohi,
+                                                   const int* __restrict__ chosen, const int* __restrict__ contested,
+                                                   int line) {
+    SeamGeom g;
+    if (!seam_geom(boxes, ci, cj, g)) return;
+    if (line >= g.lines || *gptr(contested) == 0) return;  // wave-uniform
+    const int ch = gptr(chosen)[line];
+    if (ch < 0) return;
+    const int lo = gptr(lohi)[2 * line], hi = gptr(lohi)[2 * line + 1];
+    const int t = lo + (threadIdx.x & 63);
+    if (t > hi) return;
+    const size_t p = seam_at(g.rowScan, g.ox0, g.oy0, w, line, t);
+    const int o = gptr(owner)[p];
+    if (o != ci && o != cj) return;
+    gptr(owner)[p] = ((t < ch) == g.iLow) ? ci : cj;
+}
 
 template <class Img, typename M>
-__global__ void __launch_bounds__(1024) seam_dp_kernel(Img imgI, Img imgJ, const uint8_t* __restrict__ covI,
-                                                       const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
-                                                       int ci, int cj, int w, int32_t* __restrict__ owner,
-                                                       int* __restrict__ lohi, int* __restrict__ prevl,
-                                                       int* __restrict__ chosen, uint8_t* __restrict__ back,
-                                                       const int* __restrict__ contested) {
-    seam_dp_body<Img, M>(contested, imgI, imgJ, covI, covJ, boxes, ci, cj, w, owner, lohi, prevl, chosen, back);
+__global__ void __launch_bounds__(kDpThreads) seam_dp_kernel(const int* __restrict__ boxes, int ci, int cj,
+                                                             const M* __restrict__ costs,
+                                                             const int* __restrict__ lohi, int* __restrict__ prevl,
+                                                             int* __restrict__ chosen, uint8_t* __restrict__ back,
+                                                             const int* __restrict__ contested) {
+    seam_dp_body<Img, M>(contested, boxes, ci, cj, costs, lohi, prevl, chosen, back);
+}
+
+template <int Dummy = 0>  // a template, so the header defines it once per program
+__global__ void __launch_bounds__(256) seam_reassign_kernel(const int* __restrict__ boxes, int ci, int cj, int w,
+                                                            int32_t* __restrict__ owner,
+                                                            const int* __restrict__ lohi,
+                                                            const int* __restrict__ chosen,
+                                                            const int* __restrict__ contested) {
+    const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
+    seam_reassign_body(boxes, ci, cj, w, owner, lohi, chosen, contested, line);
 }
 
 // One pair of one chunk per table entry (batched form: every chunk's k-th
@@ -307,21 +376,30 @@ struct SeamWork {
     int* prevl;
     int* chosen;
     uint8_t* back;
+    void* costs;     // 64 cost values (of the DP's arithmetic type) per line
     int* contested;  // contested-line count of this pair, cleared before the render
 };
 
-template <class Img>
+template <class Img, typename M>
 __global__ void __launch_bounds__(256) seam_scan_table_kernel(const SeamWork<Img>* __restrict__ work) {
     const SeamWork<Img> wk = work[blockIdx.y];
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body(wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi, wk.chosen, wk.contested, line);
+    seam_scan_body<Img, M>(wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi,
+                           wk.chosen, wk.contested, static_cast<M*>(wk.costs), line);
 }
 
 template <class Img, typename M>
-__global__ void __launch_bounds__(1024) seam_dp_table_kernel(const SeamWork<Img>* __restrict__ work) {
+__global__ void __launch_bounds__(kDpThreads) seam_dp_table_kernel(const SeamWork<Img>* __restrict__ work) {
     const SeamWork<Img> wk = work[blockIdx.x];
-    seam_dp_body<Img, M>(wk.contested, wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner,
-                         wk.lohi, wk.prevl, wk.chosen, wk.back);
+    seam_dp_body<Img, M>(wk.contested, wk.boxes, wk.ci, wk.cj, static_cast<const M*>(wk.costs), wk.lohi, wk.prevl,
+                         wk.chosen, wk.back);
+}
+
+template <class Img>
+__global__ void __launch_bounds__(256) seam_reassign_table_kernel(const SeamWork<Img>* __restrict__ work) {
+    const SeamWork<Img> wk = work[blockIdx.y];
+    const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
+    seam_reassign_body(wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi, wk.chosen, wk.contested, line);
 }
 
 }  // namespace i360

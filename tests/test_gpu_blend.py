@@ -72,3 +72,25 @@ def test_blend_random_vs_oracle(hip, seed, cw, ch, c):
     out, owner = _blend(hip, bufs)
     assert np.array_equal(owner, want_owner)
     assert np.array_equal(out, want)
+
+
+@pytest.mark.parametrize("strip", ["4", "8", "16"])
+@pytest.mark.parametrize("cw", [260, 268, 276])
+def test_blend_ragged_width_full_rows(hip, monkeypatch, strip, cw):
+    """A contributor covering whole rows of a chunk whose width is not a
+    multiple of the chamfer strip: its last strip ends past the last column
+    and must not touch the next row, and the backward pass must read that
+    strip's own columns.  A rival covering the left part only loses every
+    pixel the full contributor covers (its distance is smaller)."""
+    monkeypatch.setenv("I360HIP_CHAMFER_STRIP", strip)
+    ch = 37
+    rng = np.random.Generator(np.random.PCG64(cw))
+    full = np.clip(rng.normal(30000, 5000, size=(ch, cw, 3)), 1, 65535).astype(np.uint16)
+    left = np.clip(rng.normal(20000, 5000, size=(ch, cw, 3)), 1, 65535).astype(np.uint16)
+    left[:, 200:] = 0
+    bufs = [left, full]
+    note(f"blend ragged strip={strip} {cw}x{ch}")
+    want, want_owner = po.blend_chunk(bufs, 4, want_owner=True)
+    out, owner = _blend(hip, bufs, bands=4)
+    assert np.array_equal(owner, want_owner)
+    assert np.array_equal(out, want)

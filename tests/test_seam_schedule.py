@@ -73,3 +73,42 @@ def test_breaking_shared_order_changes_the_result():
         want = _run(warped, dists, boxes, pairs)
         changed |= not np.array_equal(_run(warped, dists, boxes, pairs[::-1]), want)
     assert changed
+
+
+def _line_min(dprev, last_lo, lo, smooth, dt, ranges):
+    """Per target lane: minimum over the previous line's candidates of
+    dprev + smooth * |t - q|, each range scanned in increasing order with a
+    strict <, the ranges' results combined in order with a strict < (the
+    DP kernel's split across wavefronts; one range is the serial scan)."""
+    out = []
+    for lane in range(64):
+        t = lo + lane
+        best, bp = None, -1
+        for p0, p1 in ranges:
+            rb, rk = dt(np.finfo(dt).max), -1
+            for pk in range(p0, min(p1, len(dprev))):
+                cand = dt(dprev[pk] + dt(dt(smooth) * dt(abs(t - (last_lo + pk)))))
+                if cand < rb:
+                    rb, rk = cand, pk
+            if best is None or rb < best:
+                best, bp = rb, rk
+        out.append((best, bp))
+    return out
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("seed", range(6))
+def test_split_candidate_scan_keeps_first_minimum(seed, dt):
+    """Splitting a line's candidates into 4 ranges of 16 and combining the
+    partial minima in range order selects the same value and the same
+    (first) back-pointer as the reference's single serial scan, ties
+    included (integer costs make ties frequent)."""
+    rng = np.random.default_rng(seed)
+    for _ in range(20):
+        last_w = int(rng.integers(1, 50))
+        dprev = rng.integers(0, 6, size=last_w).astype(dt) * dt(2)
+        last_lo, lo = int(rng.integers(0, 40)), int(rng.integers(0, 40))
+        smooth = 2.0 if seed % 2 else 2.0 / 65535.0
+        serial = _line_min(dprev, last_lo, lo, smooth, dt, [(0, 64)])
+        split = _line_min(dprev, last_lo, lo, smooth, dt, [(0, 16), (16, 32), (32, 48), (48, 64)])
+        assert serial == split
