@@ -1089,47 +1089,140 @@ __global__ void __launch_bounds__(256) rsexport_kernel(const RChunk* __restrict_
 }
 
 // Level 0: only the owner carries weight (0/1 owner mask); FP32 RGB out.
-// One thread per pixel of a 64x16 tile (1024 threads).
+// A block takes one 64x16 tile per iteration, each thread final_pix
+// pixels of one column (rows r, r + 16 / final_pix, ...).  The
+// tile's chunk items are staged in LDS once per tile (double buffered by
+// iteration, so one barrier per tile suffices), and every load of a
+// thread's pixels is issued before any of them is used.  So that no load
+// needs a branch, a pixel outside the chunk reads index 0 and a pixel
+// without an owner reads item 0: both are valid addresses (every item
+// plane is chunk-sized), and their values are discarded.
+struct FinalItem {
+    const void* warped;
+    const void* nd1;
+    int box[4];
+};
+// Pixels per thread: 4 for the FP32-maths forms; 1 for the FP64 form,
+// whose four-pixel register footprint leaves one wavefront per SIMD.
 template <int MODE>
-__global__ void __launch_bounds__(1024) rfinal_kernel(const RItem* __restrict__ items,
-                                                      const RChunk* __restrict__ chunks,
-                                                      const RTile* __restrict__ tiles, int ntiles) {
+constexpr int final_pix() { return MODE == kModeFP64 ? 1 : 4; }
+template <int MODE>
+constexpr int final_threads() { return kWarpTW * kWarpTH / final_pix<MODE>(); }
+
+template <int MODE>
+__global__ void __launch_bounds__(final_threads<MODE>()) rfinal_kernel(const RItem* __restrict__ items,
+                                                               const RChunk* __restrict__ chunks,
+                                                               const RTile* __restrict__ tiles, int ntiles) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
-    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+    constexpr int K = final_pix<MODE>();
+    constexpr int kRowStep = kWarpTH / K;
+    __shared__ FinalItem sit[2][I360HIP_MAX_CONTRIBUTORS];
+    const int lx = threadIdx.x & 63, ry = threadIdx.x >> 6;
+    int buf = 0;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x, buf ^= 1) {
         const RTile tl = tiles[e];
         const RChunk& ch = chunks[tl.id];
-        const int lx = threadIdx.x & 63;
-        const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6);
-        if (x >= ch.w || y >= ch.h) continue;  // the next tile (no barrier in this kernel)
-        const size_t p = static_cast<size_t>(y) * ch.w + x;
-        const int o = gptr(ch.owner)[p];
-        if (ch.owner_out) gptr(ch.owner_out)[p] = o;
-        auto* dst = gptr(ch.out + p * 3);
-        if (o < 0) {
-            dst[0] = 0.0f; dst[1] = 0.0f; dst[2] = 0.0f;
+        const int w = ch.w, h = ch.h, count = ch.count;
+        if (static_cast<int>(threadIdx.x) < count) {
+            const RItem& it = items[ch.first + threadIdx.x];
+            FinalItem f;
+            f.warped = it.warped;
+            f.nd1 = it.nd[1];
+            f.box[0] = it.lbox[1][0]; f.box[1] = it.lbox[1][1]; f.box[2] = it.lbox[1][2]; f.box[3] = it.lbox[1][3];
+            sit[buf][threadIdx.x] = f;
+        }
+        const int x = tl.tx * kWarpTW + lx;
+        bool ok[K];
+        size_t p[K];
+        int o[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int y = tl.ty * kWarpTH + ry + kRowStep * k;
+            ok[k] = x < w && y < h;
+            p[k] = ok[k] ? static_cast<size_t>(y) * w + x : 0;
+            const int v = gptr(ch.owner)[p[k]];
+            o[k] = ok[k] ? v : -1;
+        }
+        lds_barrier();  // the staged items are visible
+        if (ch.owner_out) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (ok[k]) gptr(ch.owner_out)[p[k]] = o[k];
+        }
+        M v[K][3];
+        if (count == 0) {  // tile-uniform: no contributor, every pixel is 0
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[k][0] = v[k][1] = v[k][2] = M(0);
         } else {
-            const RItem& it = items[ch.first + o];
-            M q[4];
-            ld4(it.warped, p, q, static_cast<S*>(nullptr));
-            const M inv = (q[3] > static_cast<M>(1e-6)) ? M(1) / q[3] : M(0);
-            M v[3];
+            const FinalItem* fi[K];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) v[k] = (q[k] * q[3]) * inv;
-            if (ch.levels > 1) {
-                const Coord<M> c = expand_at<M>(x, y, ch.lw[1], ch.lh[1], ch.w, ch.h);
-                M up[3], ur[3];
-                expand_gauss_m<M, S>(it.nd[1], it.lbox[1], ch.lw[1], c, up);
-                expand_recon_m<M>(ch.recon[1], ch.lw[1], c, ur);
+            for (int k = 0; k < K; ++k) fi[k] = &sit[buf][o[k] >= 0 ? o[k] : 0];
+            M q[K][4];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) v[k] = (v[k] - up[k]) + ur[k];
+            for (int k = 0; k < K; ++k) ld4(fi[k]->warped, p[k], q[k], static_cast<S*>(nullptr));
+            if (ch.levels > 1) {  // tile-uniform
+                const int lw1 = ch.lw[1], lh1 = ch.lh[1];
+                Coord<M> c[K];
+                M a[K][4][4], r[K][4][4];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int y = tl.ty * kWarpTH + ry + kRowStep * k;
+                    c[k] = expand_at<M>(x, y, lw1, lh1, w, h);
+                    const size_t t00 = static_cast<size_t>(c[k].y0) * lw1 + c[k].x0;
+                    const size_t t10 = static_cast<size_t>(c[k].y0) * lw1 + c[k].x1;
+                    const size_t t01 = static_cast<size_t>(c[k].y1) * lw1 + c[k].x0;
+                    const size_t t11 = static_cast<size_t>(c[k].y1) * lw1 + c[k].x1;
+                    ld4(fi[k]->nd1, t00, a[k][0], static_cast<S*>(nullptr));
+                    ld4(fi[k]->nd1, t10, a[k][1], static_cast<S*>(nullptr));
+                    ld4(fi[k]->nd1, t01, a[k][2], static_cast<S*>(nullptr));
+                    ld4(fi[k]->nd1, t11, a[k][3], static_cast<S*>(nullptr));
+                    ld4(ch.recon[1], t00, r[k][0], static_cast<M*>(nullptr));
+                    ld4(ch.recon[1], t10, r[k][1], static_cast<M*>(nullptr));
+                    ld4(ch.recon[1], t01, r[k][2], static_cast<M*>(nullptr));
+                    ld4(ch.recon[1], t11, r[k][3], static_cast<M*>(nullptr));
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int* bx = fi[k]->box;
+                    const int xs[4] = {c[k].x0, c[k].x1, c[k].x0, c[k].x1};
+                    const int ys[4] = {c[k].y0, c[k].y0, c[k].y1, c[k].y1};
+                    M g[4][3];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        // hole-filled Gaussian, zero outside the support box
+                        const bool in = xs[j] >= bx[0] && xs[j] <= bx[2] && ys[j] >= bx[1] && ys[j] <= bx[3];
+                        const M inv = (a[k][j][3] > static_cast<M>(1e-6)) ? M(1) / a[k][j][3] : M(0);
+#pragma unroll
+                        for (int ch3 = 0; ch3 < 3; ++ch3) g[j][ch3] = in ? a[k][j][ch3] * inv : M(0);
+                    }
+                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? M(1) / q[k][3] : M(0);
+#pragma unroll
+                    for (int ch3 = 0; ch3 < 3; ++ch3) {
+                        const M up = lerp2m<M>(c[k], g[0][ch3], g[1][ch3], g[2][ch3], g[3][ch3]);
+                        const M ur = lerp2m<M>(c[k], r[k][0][ch3], r[k][1][ch3], r[k][2][ch3], r[k][3][ch3]);
+                        v[k][ch3] = (((q[k][ch3] * q[k][3]) * inv0) - up) + ur;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? M(1) / q[k][3] : M(0);
+#pragma unroll
+                    for (int ch3 = 0; ch3 < 3; ++ch3) v[k][ch3] = (q[k][ch3] * q[k][3]) * inv0;
+                }
             }
+        }
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                M t = v[k] < M(1) ? v[k] : M(1);
+        for (int k = 0; k < K; ++k) {
+            if (!ok[k]) continue;
+            auto* dst = gptr(ch.out + p[k] * 3);
+#pragma unroll
+            for (int ch3 = 0; ch3 < 3; ++ch3) {
+                M t = v[k][ch3] < M(1) ? v[k][ch3] : M(1);
                 t = M(0) < t ? t : M(0);
-                dst[k] = static_cast<float>(t);
+                dst[ch3] = o[k] < 0 ? 0.0f : static_cast<float>(t);
             }
         }
     }
@@ -1780,7 +1873,8 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
     if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
     RL(st, "final_level0", area0 * (4.0 + 4 * sS + 12.0),
-       rfinal_kernel<MODE><<<persistent_grid(ctx, n, 2), 1024, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
+       rfinal_kernel<MODE><<<persistent_grid(ctx, n, 8 / final_pix<MODE>()), final_threads<MODE>(), 0, s>>>(
+           T.items, T.chunks, T.tiles + off, n));
 }
 
 struct RenderCache {
