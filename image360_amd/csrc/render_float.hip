@@ -510,31 +510,70 @@ __global__ void __launch_bounds__(256) rmap_error_kernel(const RItem* __restrict
 }
 
 // ---------------------------------------------------------- ownership ---
-// One thread per pixel of a 64x16 tile (1024 threads): a thread's work is
-// one row's pixel, so no row waits behind another row's store.
-// Distances are read from the chamfer's sheared planes (shift = the
-// launch's ChamferShear).
-__global__ void __launch_bounds__(1024) rowner_kernel(const RItem* __restrict__ items,
-                                                      const RChunk* __restrict__ chunks,
-                                                      const RTile* __restrict__ tiles, int ntiles, int shift) {
-    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+// Owner = the contributor with the largest distance, the first on ties
+// (distances are read from the chamfer's sheared planes, shift = the
+// launch's ChamferShear).  A 256-thread block takes one 64x16 tile per
+// iteration, each thread four pixels of one column; the chunk's distance
+// planes and coverage boxes are staged in LDS once per tile (double
+// buffered by iteration, one barrier per tile).  Every contributor's
+// distance load is issued unconditionally (a pixel outside the box or the
+// chunk reads its plane's first element, and the value is discarded), so
+// the loads of several contributors and pixels are in flight together.
+struct OwnerItem {
+    const float* dist;
+    int box[4];
+};
+constexpr int kOwnerPix = 4;
+
+__global__ void __launch_bounds__(kWarpTW * kWarpTH / kOwnerPix) rowner_kernel(const RItem* __restrict__ items,
+                                                                              const RChunk* __restrict__ chunks,
+                                                                              const RTile* __restrict__ tiles,
+                                                                              int ntiles, int shift) {
+    constexpr int K = kOwnerPix;
+    constexpr int kRowStep = kWarpTH / K;
+    __shared__ OwnerItem sit[2][I360HIP_MAX_CONTRIBUTORS];
+    const int lx = threadIdx.x & 63, ry = threadIdx.x >> 6;
+    int buf = 0;
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x, buf ^= 1) {
         const RTile tl = tiles[e];
         const RChunk& ch = chunks[tl.id];
-        const int lx = threadIdx.x & 63;
-        {
-            const int x = tl.tx * kWarpTW + lx, y = tl.ty * kWarpTH + (threadIdx.x >> 6);
-            if (x >= ch.w || y >= ch.h) continue;  // the next tile
-            const size_t p = static_cast<size_t>(y) * ch.w + x;
-            int best = -1;
-            float bd = 0.0f;
-            for (int k = 0; k < ch.count; ++k) {
-                const RItem& it = items[ch.first + k];
-                if (!in_box(it.box, x, y)) continue;
-                const float v = gptr(it.dist)[ShearAt(x, y, shift, DistPitch(ch.w))];
-                if (v > bd) { bd = v; best = k; }
-            }
-            gptr(ch.owner)[p] = best;
+        const int w = ch.w, h = ch.h, count = ch.count;
+        if (static_cast<int>(threadIdx.x) < count) {
+            const RItem& it = items[ch.first + threadIdx.x];
+            const auto* b = gptr(it.box);
+            OwnerItem f;
+            f.dist = it.dist;
+            f.box[0] = b[0]; f.box[1] = b[1]; f.box[2] = b[2]; f.box[3] = b[3];
+            sit[buf][threadIdx.x] = f;
         }
+        lds_barrier();
+        const int x = tl.tx * kWarpTW + lx;
+        const int dp = DistPitch(w);
+        int y[K], best[K];
+        float bd[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            y[k] = tl.ty * kWarpTH + ry + kRowStep * k;
+            best[k] = -1;
+            bd[k] = 0.0f;
+        }
+#pragma unroll 2
+        for (int i = 0; i < count; ++i) {
+            const OwnerItem& it = sit[buf][i];
+            bool in[K];
+            float v[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                in[k] = x < w && y[k] < h && x >= it.box[0] && x <= it.box[2] && y[k] >= it.box[1] && y[k] <= it.box[3];
+                v[k] = gptr(it.dist)[in[k] ? ShearAt(x, y[k], shift, dp) : 0];
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (in[k] && v[k] > bd[k]) { bd[k] = v[k]; best[k] = i; }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (x < w && y[k] < h) gptr(ch.owner)[static_cast<size_t>(y[k]) * w + x] = best[k];
     }
 }
 
@@ -686,8 +725,10 @@ __global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ 
 
 // Level 0 -> 1 of the FP16 forms: the tile's level-0 window (19 rows x 132
 // columns, replicate-clamped, zero outside the item's coverage box) is
-// staged once into LDS, every element with one load issued up front, and
-// both blur passes read LDS.  Same arithmetic as rreduce_kernel<MODE, true>.
+// staged into LDS, and both blur passes read LDS.  Same arithmetic as
+// rreduce_kernel<MODE, true>.  Software pipelined: the next tile's window
+// loads are issued as soon as the current window is in LDS, and stay in
+// flight across the blur passes (the barriers order LDS only).
 constexpr int kRed0Cols = 2 * kRedTW + 4;
 template <int MODE>
 __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__ items,
@@ -700,18 +741,27 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
     __shared__ h4 sq[NE];
     __shared__ uint8_t sw[NE];
     __shared__ M tmp[5][kRedRows][kRedTW];
-    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
-        const RTile tl = tiles[e];
+    // first tile at or after f (stride gridDim.x) that this phase processes
+    auto next_tile = [&](int f) {
+        for (; f < ntiles; f += gridDim.x) {
+            const RItem& it = items[tiles[f].id];
+            if (!phase_skip(chunks[it.chunk], it.chunk, 1, phase, cflag)) break;
+        }
+        return f;
+    };
+    h4 qv[PER];
+    int ov[PER];
+    bool inb[PER];
+    // issues tile f's window loads (every address valid: elements outside
+    // the box read element 0 and are zeroed at the commit)
+    auto fetch = [&](int f) {
+        const RTile tl = tiles[f];
         const RItem& it = items[tl.id];
         const RChunk& ch = chunks[it.chunk];
-        if (phase_skip(ch, it.chunk, 1, phase, cflag)) continue;  // block-uniform
-        const int w = ch.lw[0], h = ch.lh[0], ow = ch.lw[1], oh = ch.lh[1];
+        const int w = ch.lw[0], h = ch.lh[0];
         const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
         const int* sb = it.box;
         const h4* wp = static_cast<const h4*>(it.warped);
-        h4 qv[PER];
-        int ov[PER];
-        bool inb[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = threadIdx.x + 256 * k;
@@ -723,6 +773,16 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
             qv[k] = gptr(wp)[p];
             ov[k] = gptr(ch.owner)[p];
         }
+    };
+    int e = next_tile(blockIdx.x);
+    if (e < ntiles) fetch(e);
+    while (e < ntiles) {
+        const RTile tl = tiles[e];
+        const RItem& it = items[tl.id];
+        const RChunk& ch = chunks[it.chunk];
+        const int ow = ch.lw[1], oh = ch.lh[1];
+        const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
+        const int local = it.local;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = threadIdx.x + 256 * k;
@@ -730,9 +790,11 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
             h4 z;
             z.x = z.y = z.z = z.w = static_cast<_Float16>(0.0f);
             sq[i] = inb[k] ? qv[k] : z;
-            sw[i] = (inb[k] && ov[k] == it.local) ? 1 : 0;
+            sw[i] = (inb[k] && ov[k] == local) ? 1 : 0;
         }
-        __syncthreads();
+        lds_barrier();  // window staged; the previous tile's vertical pass is done with tmp
+        const int nx = next_tile(e + gridDim.x);
+        if (nx < ntiles) fetch(nx);
         for (int idx = threadIdx.x; idx < kRedRows * kRedTW; idx += 256) {
             const int rr = idx / kRedTW, xx = idx % kRedTW;
             if (ox0 + xx >= ow) continue;
@@ -751,7 +813,7 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
 #pragma unroll
             for (int c = 0; c < 5; ++c) tmp[c][rr][xx] = blur5m<M>(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c]);
         }
-        __syncthreads();
+        lds_barrier();  // tmp complete (and every read of sq / sw done)
         for (int idx = threadIdx.x; idx < kRedTH * kRedTW; idx += 256) {
             const int yy = idx / kRedTW, xx = idx % kRedTW;
             const int ox = ox0 + xx, oy = oy0 + yy;
@@ -766,7 +828,7 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
             st4(it.nd[1], q, nd, static_cast<_Float16*>(nullptr));
             st1<_Float16, M>(it.wt[1], q, o[4]);
         }
-        __syncthreads();
+        e = nx;
     }
 }
 
@@ -1748,7 +1810,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
         }
     }
     span(P.own, off, n);
-    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n, 2), 1024, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
+    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n, 8), kWarpTW * kWarpTH / kOwnerPix, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
                                                                                    ChamferShear(P.max_w, ctx->tune_strip)));
     // seams: the k-th pair of every chunk in range, in pair order
     {
