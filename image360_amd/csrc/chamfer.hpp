@@ -110,10 +110,25 @@ __global__ void __launch_bounds__(256) chamfer_shear_kernel(const ChamferWork* _
 // second half the odd ones.  Strips of one parity are busy on the same
 // steps (row r of strip j runs at step 2r + j), so every wavefront is
 // either fully busy or idle at each step.
+//
+// Neighbours beyond the processed area are constants: a pixel that exists
+// outside the coverage box is uncovered (0), a pixel beyond the chunk edge
+// does not exist (+inf, which no min picks).  They come from sentinel LDS
+// slots (left of the first strip, right of the last) and, for the row
+// before the box, from per-thread constants, so every LDS read is
+// unconditional and a pixel's update is branch-free.  Distances are
+// non-negative floats, whose order is their bit patterns' unsigned order:
+// min is one integer instruction.
+__device__ __forceinline__ uint32_t dmin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__host__ __device__ constexpr int RWmax(int S) { return S / 4; }
+__device__ __forceinline__ uint32_t dadd(uint32_t a, float c) { return __float_as_uint(__uint_as_float(a) + c); }
+
 template <int S, int P, bool FWD>
 __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __restrict__ work) {
-    __shared__ float firstv[2][1024];
-    __shared__ float lastv[2][1024];
+    // boundary columns of every strip for the last two rows: slot t + 1
+    // holds strip t; slots 0 and nact + 1 the sentinels
+    __shared__ uint32_t firstv[2][1024 + 2];
+    __shared__ uint32_t lastv[2][1024 + 2];
     const ChamferWork wk = work[blockIdx.x];
     const int x0 = wk.box[0], y0 = wk.box[1], x1 = wk.box[2], y1 = wk.box[3];
     if (x1 < 0) return;  // block-uniform
@@ -128,22 +143,53 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
     const int dp = DistPitch(w);
     const int xl = xa + min(t, nact - 1) * S;  // load strip: inactive threads load the last active one
     constexpr int SH = ShearShift(S);
+    constexpr uint32_t kNone = 0x7f800000u;  // +inf: no pixel there
     const float kInf = static_cast<float>(w + h);
     const int lag = FWD ? t : (nact - 1 - t);
     const int base = lag >> 1, par = lag & 1;
-    const bool lastT = (t == nact - 1);
-    const bool edgePrev = FWD ? (y0 > 0) : (y1 < h - 1);  // a row exists beyond the box (all zeros)
+    const bool edgePrev = FWD ? (y0 > 0) : (y1 < h - 1);  // a row exists before the box (all uncovered)
+
+    // per-column constants of this strip
+    uint32_t inBox = 0, exists = 0;  // bit s: column xs + s
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int x = xs + s;
+        inBox |= (x >= x0 && x <= x1) ? (1u << s) : 0u;
+        exists |= (x < w) ? (1u << s) : 0u;
+    }
+    // FWD: coverage bytes of the columns outside the box are masked to 0
+    uint32_t byteMask[RWmax(S)];
+#pragma unroll
+    for (int g = 0; g < S / 4; ++g) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) m |= ((inBox >> (4 * g + b)) & 1u) ? (0xffu << (8 * b)) : 0u;
+        byteMask[g] = m;
+    }
+    // the row before the box, and the columns left and right of the strip there
+    const uint32_t pL0 = (edgePrev && xs > 0) ? 0u : kNone;
+    const uint32_t pR0 = (edgePrev && xs + S < w) ? 0u : kNone;
+    if (threadIdx.x == 0) {
+        const uint32_t sl = xa > 0 ? 0u : kNone, sr = (xa + nact * S < w) ? 0u : kNone;
+        lastv[0][0] = sl; lastv[1][0] = sl;
+        firstv[0][nact + 1] = sr; firstv[1][nact + 1] = sr;
+    }
 
     // raw prefetched row data; converted only when the row is processed so
     // that no instruction depends on a load until its row's step
     //   FWD: the strip's S coverage bytes, packed four to a word
     //   BWD: the forward distance bits
+    // Three row buffers rotate (no register copies): superstep j reads
+    // buffer j mod 3 and issues the loads of superstep j + 2.  A copy, or a
+    // load one superstep ahead, would make the compiler wait for every
+    // older memory operation (loads and the conditional stores share one
+    // in-order counter) at each superstep; two ahead, the wait at a
+    // buffer's first use leaves the newest loads and stores in flight.
     constexpr int RW = FWD ? S / 4 : S;
-    uint32_t ring[P][RW];  // rows of the current superstep
-    uint32_t next[P][RW];  // rows of the next superstep, in flight
-    float prev[S];
+    uint32_t bufA[P][RW], bufB[P][RW], bufC[P][RW];
+    uint32_t prev[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) prev[s] = 0.0f;
+    for (int s = 0; s < S; ++s) prev[s] = (edgePrev && ((exists >> s) & 1u)) ? 0u : kNone;
 
     // loads are unconditional (rows and strips clamped into the box) so
     // that no register is merged across a branch while its load is in flight
@@ -172,25 +218,36 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
             }
         }
     };
-    auto init_of = [&](const uint32_t (&src)[RW], int s) -> float {
-        if (xs + s >= w) return 0.0f;
-        if constexpr (FWD) return ((src[s >> 2] >> (8 * (s & 3))) & 0xffu) ? kInf : 0.0f;
-        else return __uint_as_float(src[s]);
+    // initial value of column s: covered -> w + h (FWD) / the forward
+    // distance (BWD); outside the box 0
+    auto init_of = [&](const uint32_t (&src)[RW], int s) -> uint32_t {
+        if constexpr (FWD) {
+            // coverage byte (0 or 1; 0 outside the box) times w + h
+            const float c = static_cast<float>(((src[s >> 2] & byteMask[s >> 2]) >> (8 * (s & 3))) & 0xffu);
+            return __float_as_uint(c * kInf);
+        } else {
+            return ((inBox >> s) & 1u) ? src[s] : 0u;
+        }
     };
     // superstep j (steps 2Pj .. 2Pj+2P-1) processes rows Pj - base + k, k < P,
-    // in ring slot k; the rows of superstep j+1 are loaded at the start of
-    // superstep j and rotated in at its end, so each load has a whole
-    // superstep to arrive and no step waits on memory
+    // in slot k of its buffer
     auto fetch = [&](int j, uint32_t (&dst)[P][RW]) {
 #pragma unroll
         for (int k = 0; k < P; ++k) load_row(min(max(P * j + k - base, 0), rows - 1), dst[k]);
     };
-    fetch(0, ring);
+    fetch(0, bufA);
+    fetch(1, bufB);
+    // row r of this strip lives at pbase +/- r * dp (sheared plane; r * dp < 2^28)
+    float* const pbase = wk.dist + ShearAt(xs, FWD ? y0 : y1, SH, dp);
+    lds_barrier();  // sentinels visible
 
     const int tmax = 2 * (rows - 1) + nact - 1;
-    for (int T0 = 0; T0 <= tmax; T0 += 2 * P) {
+    // one superstep; false (block-uniform) once past the last step
+    auto superstep = [&](int j, const uint32_t (&ring)[P][RW], uint32_t (&ld)[P][RW]) -> bool {
+        const int T0 = 2 * P * j;
+        if (T0 > tmax) return false;
+        fetch(j + 2, ld);
         const int nsteps = min(2 * P, tmax - T0 + 1);  // uniform
-        fetch(T0 / (2 * P) + 1, next);
 #pragma unroll
         for (int s2 = 0; s2 < 2 * P; ++s2) {
             if (s2 >= nsteps) continue;  // uniform: all threads skip the barrier together
@@ -198,62 +255,57 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
                 const int k = s2 >> 1;
                 const int r = (T0 >> 1) + k - base;
                 if (r >= 0 && r < rows) {
-                    const int y = FWD ? (y0 + r) : (y1 - r);
                     const int pb = (r - 1) & 1, cb = r & 1;
-                    const bool hasPrev = (r > 0) || edgePrev;
-                    float cur[S];
-                    if (FWD) {
-                        const float carryL = (t > 0) ? lastv[cb][t - 1] : 0.0f;
-                        const float prevL = (t > 0 && r > 0) ? lastv[pb][t - 1] : 0.0f;
-                        const float prevR = (!lastT && r > 0) ? firstv[pb][t + 1] : 0.0f;
+                    // neighbours (slot = strip + 1), all read together
+                    const uint32_t lCur = lastv[cb][t], rCur = firstv[cb][t + 2];
+                    const uint32_t lPrev = lastv[pb][t], rPrev = firstv[pb][t + 2];
+                    const bool first = r == 0;
+                    const uint32_t prevL = first ? pL0 : lPrev;
+                    const uint32_t prevR = first ? pR0 : rPrev;
+                    uint32_t cur[S];
+                    if constexpr (FWD) {
 #pragma unroll
                         for (int s = 0; s < S; ++s) {
-                            const int x = xs + s;
-                            float b = init_of(ring[k], s);
-                            if (x > 0) b = fminf(b, (s == 0 ? carryL : cur[s - 1]) + 1.0f);
-                            if (hasPrev) {
-                                b = fminf(b, prev[s] + 1.0f);
-                                if (x > 0) b = fminf(b, (s == 0 ? prevL : prev[s - 1]) + kChamferDiag);
-                                if (x < w - 1) b = fminf(b, (s == S - 1 ? prevR : prev[s + 1]) + kChamferDiag);
-                            }
-                            cur[s] = (x < w) ? b : 0.0f;
+                            uint32_t b = init_of(ring[k], s);
+                            b = dmin(b, dadd(s == 0 ? lCur : cur[s - 1], 1.0f));
+                            b = dmin(b, dadd(prev[s], 1.0f));
+                            b = dmin(b, dadd(s == 0 ? prevL : prev[s - 1], kChamferDiag));
+                            b = dmin(b, dadd(s == S - 1 ? prevR : prev[s + 1], kChamferDiag));
+                            cur[s] = ((exists >> s) & 1u) ? b : kNone;
                         }
                     } else {
-                        const float carryR = (!lastT) ? firstv[cb][t + 1] : 0.0f;
-                        const float prevR = (!lastT && r > 0) ? firstv[pb][t + 1] : 0.0f;
-                        const float prevL = (t > 0 && r > 0) ? lastv[pb][t - 1] : 0.0f;
 #pragma unroll
                         for (int s = S - 1; s >= 0; --s) {
-                            const int x = xs + s;
-                            float b = init_of(ring[k], s);
-                            if (x < w - 1) b = fminf(b, (s == S - 1 ? carryR : cur[s + 1]) + 1.0f);
-                            if (hasPrev) {
-                                b = fminf(b, prev[s] + 1.0f);
-                                if (x < w - 1) b = fminf(b, (s == S - 1 ? prevR : prev[s + 1]) + kChamferDiag);
-                                if (x > 0) b = fminf(b, (s == 0 ? prevL : prev[s - 1]) + kChamferDiag);
-                            }
-                            cur[s] = (x < w) ? b : 0.0f;
+                            uint32_t b = init_of(ring[k], s);
+                            b = dmin(b, dadd(s == S - 1 ? rCur : cur[s + 1], 1.0f));
+                            b = dmin(b, dadd(prev[s], 1.0f));
+                            b = dmin(b, dadd(s == S - 1 ? prevR : prev[s + 1], kChamferDiag));
+                            b = dmin(b, dadd(s == 0 ? prevL : prev[s - 1], kChamferDiag));
+                            cur[s] = ((exists >> s) & 1u) ? b : kNone;
                         }
                     }
-                    float* row = wk.dist + ShearAt(xs, y, SH, dp);
+                    float* row = FWD ? pbase + r * dp : pbase - r * dp;
 #pragma unroll
                     for (int g = 0; g < S / 4; ++g) {
                         f4v v;
-                        v.x = cur[4 * g]; v.y = cur[4 * g + 1]; v.z = cur[4 * g + 2]; v.w = cur[4 * g + 3];
+                        v.x = __uint_as_float(cur[4 * g]); v.y = __uint_as_float(cur[4 * g + 1]);
+                        v.z = __uint_as_float(cur[4 * g + 2]); v.w = __uint_as_float(cur[4 * g + 3]);
                         st_f4(row + 4 * g, v);
                     }
 #pragma unroll
                     for (int s = 0; s < S; ++s) prev[s] = cur[s];
-                    firstv[cb][t] = cur[0];
-                    lastv[cb][t] = cur[S - 1];
+                    firstv[cb][t + 1] = cur[0];
+                    lastv[cb][t + 1] = cur[S - 1];
                 }
             }
             lds_barrier();  // neighbours exchange through LDS only
         }
-#pragma unroll
-        for (int k = 0; k < P; ++k)
-#pragma unroll
-            for (int s = 0; s < RW; ++s) ring[k][s] = next[k][s];
+        return true;
+    };
+    for (int j = 0;; j += 3) {
+        if (!superstep(j, bufA, bufC)) break;
+        if (!superstep(j + 1, bufB, bufA)) break;
+        if (!superstep(j + 2, bufC, bufB)) break;
     }
 }
 
@@ -285,26 +337,31 @@ inline int ChamferShear(int max_w, int want_strip = 0) { return ShearShift(Chamf
 // Blocks per item of the shear pre-pass.
 constexpr int kShearBlocks = 32;
 
-// Shears the coverage boxes, then launches the forward and backward passes
-// over `n` work items (device table).  Every plane must hold
-// ShearRows(w, h) rows of DistPitch(w) elements.
-inline hipError_t launch_chamfer(const ChamferWork* d_work, int n, int max_w, hipStream_t s, int want_strip = 0) {
+// Shears the coverage boxes (unless the producer of the coverage already
+// wrote the sheared copy, for this launch's ChamferShear, over the box),
+// then launches the forward and backward passes over `n` work items
+// (device table).  Every plane must hold ShearRows(w, h) rows of
+// DistPitch(w) elements.
+inline hipError_t launch_chamfer(const ChamferWork* d_work, int n, int max_w, hipStream_t s, int want_strip = 0,
+                                 bool cov_sheared = false) {
     if (n <= 0) return hipSuccess;
     const int S = ChamferStrip(max_w, want_strip);
     const int nt = ChamferThreads(max_w, S);
     const dim3 sg(kShearBlocks, n);
+    // rows per superstep: the backward pass buffers S distances per row in
+    // three rotating buffers, so it takes fewer rows at wider strips
     if (S == 4) {
-        chamfer_shear_kernel<4><<<sg, 256, 0, s>>>(d_work);
+        if (!cov_sheared) chamfer_shear_kernel<4><<<sg, 256, 0, s>>>(d_work);
         chamfer_kernel<4, 8, true><<<n, nt, 0, s>>>(d_work);
         chamfer_kernel<4, 8, false><<<n, nt, 0, s>>>(d_work);
     } else if (S == 8) {
-        chamfer_shear_kernel<8><<<sg, 256, 0, s>>>(d_work);
+        if (!cov_sheared) chamfer_shear_kernel<8><<<sg, 256, 0, s>>>(d_work);
         chamfer_kernel<8, 4, true><<<n, nt, 0, s>>>(d_work);
-        chamfer_kernel<8, 4, false><<<n, nt, 0, s>>>(d_work);
+        chamfer_kernel<8, 2, false><<<n, nt, 0, s>>>(d_work);
     } else {
-        chamfer_shear_kernel<16><<<sg, 256, 0, s>>>(d_work);
+        if (!cov_sheared) chamfer_shear_kernel<16><<<sg, 256, 0, s>>>(d_work);
         chamfer_kernel<16, 2, true><<<n, nt, 0, s>>>(d_work);
-        chamfer_kernel<16, 2, false><<<n, nt, 0, s>>>(d_work);
+        chamfer_kernel<16, 1, false><<<n, nt, 0, s>>>(d_work);
     }
     return hipGetLastError();
 }

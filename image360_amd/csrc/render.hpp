@@ -31,7 +31,7 @@ struct RItem {
     int wtile0, wtilen;  // this item's warp tiles (global tile indices)
     void* warped;     // chunk-sized RGBA (coverage in A), storage type of the form
     uint8_t* cov;     // chunk-sized coverage bits
-    uint8_t* scov;    // sheared copy of the coverage box (chamfer input)
+    uint8_t* scov;    // sheared coverage (chamfer input), written by the warp over its footprint
     float* dist;      // sheared chamfer distances (inside the coverage box)
     int* box;         // coverage box x0,y0,x1,y1 (chunk-local, inclusive)
     int lbox[kMaxLevels][4];  // conservative support box per level (footprint dilated)

@@ -23,6 +23,8 @@
 #include <vector>
 
 #include "blend.hpp"
+#include <type_traits>
+
 #include "chamfer.hpp"
 #include "render.hpp"
 #include "seam.hpp"
@@ -207,7 +209,7 @@ __device__ __forceinline__ bool map_pixel(const RItem& it, const RChunk& ch, con
         const float den = (1.0f + tm.g0 * static_cast<float>(dx)) + tm.g1 * static_cast<float>(dy);
         float lx, ly;
         if (den > 0.5f) {
-            const float r = 1.0f / den;
+            const float r = __builtin_amdgcn_rcpf(den);  // 1 ulp; inside the tile's error margin
             lx = tm.fx + (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) * r;
             ly = tm.fy + (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) * r;
             const float ux = static_cast<float>(tm.ix) + lx, uy = static_cast<float>(tm.iy) + ly;
@@ -332,7 +334,8 @@ template <int MODE>
 __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ items,
                                                     const RChunk* __restrict__ chunks,
                                                     const RTile* __restrict__ tiles, int ntiles,
-                                                    const TileMap* __restrict__ maps, int4* __restrict__ wbox) {
+                                                    const TileMap* __restrict__ maps, int4* __restrict__ wbox,
+                                                    int shift) {
     using M = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::M;
     using S = typename Form<MODE == kModeBatched ? kModePrecision : MODE>::S;
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
@@ -414,6 +417,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
             }
             st4(it.warped, p, v, static_cast<S*>(nullptr));
             gptr(it.cov)[p] = ok[r] ? 1 : 0;
+            gptr(it.scov)[ShearAt(x, y, shift, DistPitch(ch.w))] = ok[r] ? 1 : 0;  // the chamfer's input
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -493,7 +497,7 @@ __global__ void __launch_bounds__(256) rmap_error_kernel(const RItem* __restrict
             const float den = (1.0f + tm.g0 * static_cast<float>(lx)) + tm.g1 * static_cast<float>(ly);
             float ox, oy;  // as map_pixel computes them
             if (den > 0.5f) {
-                const float rd = 1.0f / den;
+                const float rd = __builtin_amdgcn_rcpf(den);  // as the warp evaluates it
                 ox = (tm.a00 * static_cast<float>(lx) + tm.a01 * static_cast<float>(ly)) * rd;
                 oy = (tm.a10 * static_cast<float>(lx) + tm.a11 * static_cast<float>(ly)) * rd;
             } else {
@@ -853,6 +857,35 @@ __device__ __forceinline__ Coord<M> expand_at(int x, int y, int sw, int sh, int 
     c.x1 = min(c.x0 + 1, sw - 1);
     c.wx = fx - static_cast<M>(c.x0);
     return c;
+}
+
+// expand_at with the level's scales (sw-1)/(dw-1), (sh-1)/(dh-1) computed
+// once by the caller (same values)
+template <typename M>
+__device__ __forceinline__ M expand_scale(int s, int d) {
+    return (s > 1) ? static_cast<M>(s - 1) / static_cast<M>(max(1, d - 1)) : M(0);
+}
+template <typename M>
+__device__ __forceinline__ Coord<M> expand_at_s(int x, int y, M sx, M sy, int sw, int sh) {
+    Coord<M> c;
+    const M fy = sy * static_cast<M>(y);
+    c.y0 = min(static_cast<int>(fy), sh - 1);
+    c.y1 = min(c.y0 + 1, sh - 1);
+    c.wy = fy - static_cast<M>(c.y0);
+    const M fx = sx * static_cast<M>(x);
+    c.x0 = min(static_cast<int>(fx), sw - 1);
+    c.x1 = min(c.x0 + 1, sw - 1);
+    c.wx = fx - static_cast<M>(c.x0);
+    return c;
+}
+
+// 1/d for the hole-filled Gaussians of the FP16-storage forms: the
+// hardware reciprocal (1 ulp), far inside the FP16 storage error; the
+// FP32 and FP64 forms divide exactly.
+template <typename M, typename S>
+__device__ __forceinline__ M recip(M d) {
+    if constexpr (std::is_same<S, _Float16>::value) return __builtin_amdgcn_rcpf(d);
+    else return M(1) / d;
 }
 
 template <typename M>
@@ -1226,12 +1259,13 @@ __global__ void __launch_bounds__(final_threads<MODE>()) rfinal_kernel(const RIt
             for (int k = 0; k < K; ++k) ld4(fi[k]->warped, p[k], q[k], static_cast<S*>(nullptr));
             if (ch.levels > 1) {  // tile-uniform
                 const int lw1 = ch.lw[1], lh1 = ch.lh[1];
+                const M sx = expand_scale<M>(lw1, w), sy = expand_scale<M>(lh1, h);
                 Coord<M> c[K];
                 M a[K][4][4], r[K][4][4];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const int y = tl.ty * kWarpTH + ry + kRowStep * k;
-                    c[k] = expand_at<M>(x, y, lw1, lh1, w, h);
+                    c[k] = expand_at_s<M>(x, y, sx, sy, lw1, lh1);
                     const size_t t00 = static_cast<size_t>(c[k].y0) * lw1 + c[k].x0;
                     const size_t t10 = static_cast<size_t>(c[k].y0) * lw1 + c[k].x1;
                     const size_t t01 = static_cast<size_t>(c[k].y1) * lw1 + c[k].x0;
@@ -1255,11 +1289,11 @@ __global__ void __launch_bounds__(final_threads<MODE>()) rfinal_kernel(const RIt
                     for (int j = 0; j < 4; ++j) {
                         // hole-filled Gaussian, zero outside the support box
                         const bool in = xs[j] >= bx[0] && xs[j] <= bx[2] && ys[j] >= bx[1] && ys[j] <= bx[3];
-                        const M inv = (a[k][j][3] > static_cast<M>(1e-6)) ? M(1) / a[k][j][3] : M(0);
+                        const M inv = (a[k][j][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][j][3]) : M(0);
 #pragma unroll
                         for (int ch3 = 0; ch3 < 3; ++ch3) g[j][ch3] = in ? a[k][j][ch3] * inv : M(0);
                     }
-                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? M(1) / q[k][3] : M(0);
+                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(q[k][3]) : M(0);
 #pragma unroll
                     for (int ch3 = 0; ch3 < 3; ++ch3) {
                         const M up = lerp2m<M>(c[k], g[0][ch3], g[1][ch3], g[2][ch3], g[3][ch3]);
@@ -1270,7 +1304,7 @@ __global__ void __launch_bounds__(final_threads<MODE>()) rfinal_kernel(const RIt
             } else {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? M(1) / q[k][3] : M(0);
+                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(q[k][3]) : M(0);
 #pragma unroll
                     for (int ch3 = 0; ch3 < 3; ++ch3) v[k][ch3] = (q[k][ch3] * q[k][3]) * inv0;
                 }
@@ -1792,7 +1826,8 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
     RL(st, MODE == kModeFP64 ? "warp_fp64" : (MODE == kModePlain ? "warp_f32" : "warp_f16"),
        static_cast<double>(n) * kWarpTW * kWarpTH * (6.0 + 4 * sS + 1),
        rwarp_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
-                                                                    T.tmaps + off, T.wbox + 4 * static_cast<size_t>(off)));
+                                                                    T.tmaps + off, T.wbox + 4 * static_cast<size_t>(off),
+                                                                    ChamferShear(P.max_w, ctx->tune_strip)));
     {
         const int i0 = P.chunks[c0].first;
         const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
@@ -1804,7 +1839,8 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
         const int i0 = P.chunks[c0].first;
         const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
         ProfScope ps_(ctx, s, "chamfer", 0.0);
-        if (i1 > i0 && launch_chamfer(T.chamfer + i0, i1 - i0, P.max_w, s, ctx->tune_strip) != hipSuccess) {
+        // the warp wrote the sheared coverage: no shear pre-pass
+        if (i1 > i0 && launch_chamfer(T.chamfer + i0, i1 - i0, P.max_w, s, ctx->tune_strip, true) != hipSuccess) {
             st.set(I360HIP_KERNEL_LAUNCH_FAILED, "chamfer launch failed");
             return;
         }

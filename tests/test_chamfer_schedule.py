@@ -43,12 +43,19 @@ class Flat:
         self.a[off:off + len(v)] = v
 
 
-def shear_cov(cov, scov, box, w, h, S):
+def shear_cov(cov, scov, box, w, h, S, box_only=False):
     """chamfer_shear_kernel: the box's rows, columns widened to 16-column
-    groups, copied 8 bytes at a time to the sheared plane."""
+    groups, copied 8 bytes at a time to the sheared plane.  box_only: only
+    the box's own pixels are written (the render's warp writes its
+    footprint, which contains the box), the rest holds garbage."""
     x0, y0, x1, y1 = box
     if x1 < 0: return
     cp = pitch(w)
+    if box_only:
+        for y in range(y0, y1 + 1):
+            for x in range(x0, x1 + 1):
+                scov.store(shear_at(x, y, S, cp), np.array([cov[y, x]], np.uint8))
+        return
     ga, gb = x0 & ~15, min((x1 + 16) & ~15, cp)
     for y in range(y0, y1 + 1):
         for x in range(ga, gb, 8):
@@ -58,7 +65,14 @@ def shear_cov(cov, scov, box, w, h, S):
             scov.store(shear_at(x, y, S, cp), v)
 
 
+INF = np.float32(np.inf)
+
+
 def kernel(scov, dist, box, w, h, S, P, FWD, nt, access_rows):
+    """chamfer_kernel: neighbours beyond the processed area are constants
+    (0 for an existing pixel outside the box, +inf for no pixel) taken from
+    sentinel LDS slots and per-thread constants; columns past the chunk
+    edge hold +inf."""
     x0, y0, x1, y1 = box
     if x1 < 0: return
     dp = pitch(w)
@@ -66,14 +80,23 @@ def kernel(scov, dist, box, w, h, S, P, FWD, nt, access_rows):
     nact = (x1 + 1 - xa + S - 1) // S
     rows = y1 - y0 + 1
     kInf = f32(w + h)
-    firstv = np.zeros((2, nt), np.float32); lastv = np.zeros((2, nt), np.float32)
+    # slot t + 1 holds strip t; slots 0 / nact + 1 the sentinels
+    firstv = np.full((2, nt + 2), -7, np.float32); lastv = np.full((2, nt + 2), -7, np.float32)
+    sl = f32(0) if xa > 0 else INF
+    sr = f32(0) if xa + nact * S < w else INF
+    lastv[:, 0] = sl; firstv[:, nact + 1] = sr
+    edgePrev = (y0 > 0) if FWD else (y1 < h - 1)
     th = []
     for t in range(nt):
         lag = t if FWD else nact - 1 - t
-        th.append(dict(t=t, active=t < nact, xs=xa + t * S, xl=xa + min(t, nact - 1) * S, lag=lag,
-                       base=lag >> 1, par=lag & 1, lastT=(t == nact - 1),
-                       ring=None, prev=np.zeros(S, np.float32)))
-    edgePrev = (y0 > 0) if FWD else (y1 < h - 1)
+        xs = xa + t * S
+        th.append(dict(t=t, active=t < nact, xs=xs, xl=xa + min(t, nact - 1) * S, lag=lag,
+                       base=lag >> 1, par=lag & 1,
+                       inbox=[x0 <= xs + s <= x1 for s in range(S)], exists=[xs + s < w for s in range(S)],
+                       pL0=f32(0) if (edgePrev and xs > 0) else INF,
+                       pR0=f32(0) if (edgePrev and xs + S < w) else INF,
+                       ring=None,
+                       prev=np.array([f32(0) if (edgePrev and xs + s < w) else INF for s in range(S)], np.float32)))
 
     def load_row(T, r):
         # unconditional, for active and inactive threads alike (strip clamped)
@@ -86,9 +109,9 @@ def kernel(scov, dist, box, w, h, S, P, FWD, nt, access_rows):
         return [load_row(T, min(max(P * j + k - T['base'], 0), rows - 1)) for k in range(P)]
 
     def init_of(T, raw, s):
-        if T['xs'] + s >= w: return f32(0)
+        if not T['inbox'][s]: return f32(0)
         if not FWD: return raw[s]
-        return kInf if raw[s] else f32(0)
+        return f32(f32(raw[s]) * kInf)  # coverage byte 0 or 1
 
     for T in th:
         T['ring'] = fetch(T, 0)
@@ -110,45 +133,34 @@ def kernel(scov, dist, box, w, h, S, P, FWD, nt, access_rows):
                 y = y0 + r if FWD else y1 - r
                 access_rows.setdefault((FWD, T0 + s2, t & 1), set()).add((y + (T['xs'] // S >> 1)))
                 pb, cb = (r - 1) & 1, r & 1
-                hasPrev = (r > 0) or edgePrev
+                lCur, rCur = snap_last[cb][t], snap_first[cb][t + 2]
+                used = [lCur if FWD else rCur] + ([snap_last[pb][t], snap_first[pb][t + 2]] if r > 0 else [])
+                assert all(v != -7 for v in used)  # every neighbour used was written at an earlier step
+                prevL = T['pL0'] if r == 0 else snap_last[pb][t]
+                prevR = T['pR0'] if r == 0 else snap_first[pb][t + 2]
                 cur = np.zeros(S, np.float32)
                 xs = T['xs']; prev = T['prev']; raw = T['ring'][k]
-                if FWD:
-                    carryL = snap_last[cb][t - 1] if t > 0 else f32(0)
-                    prevL = snap_last[pb][t - 1] if (t > 0 and r > 0) else f32(0)
-                    prevR = snap_first[pb][t + 1] if (not T['lastT'] and r > 0) else f32(0)
-                    for s in range(S):
-                        x = xs + s
-                        b = init_of(T, raw, s)
-                        if x > 0: b = min(b, f32((carryL if s == 0 else cur[s - 1]) + f32(1)))
-                        if hasPrev:
-                            b = min(b, f32(prev[s] + f32(1)))
-                            if x > 0: b = min(b, f32((prevL if s == 0 else prev[s - 1]) + D))
-                            if x < w - 1: b = min(b, f32((prevR if s == S - 1 else prev[s + 1]) + D))
-                        cur[s] = b if x < w else f32(0)
-                else:
-                    carryR = snap_first[cb][t + 1] if not T['lastT'] else f32(0)
-                    prevR = snap_first[pb][t + 1] if (not T['lastT'] and r > 0) else f32(0)
-                    prevL = snap_last[pb][t - 1] if (t > 0 and r > 0) else f32(0)
-                    for s in range(S - 1, -1, -1):
-                        x = xs + s
-                        b = init_of(T, raw, s)
-                        if x < w - 1: b = min(b, f32((carryR if s == S - 1 else cur[s + 1]) + f32(1)))
-                        if hasPrev:
-                            b = min(b, f32(prev[s] + f32(1)))
-                            if x < w - 1: b = min(b, f32((prevR if s == S - 1 else prev[s + 1]) + D))
-                            if x > 0: b = min(b, f32((prevL if s == 0 else prev[s - 1]) + D))
-                        cur[s] = b if x < w else f32(0)
+                order = range(S) if FWD else range(S - 1, -1, -1)
+                for s in order:
+                    b = init_of(T, raw, s)
+                    if FWD:
+                        b = min(b, f32((lCur if s == 0 else cur[s - 1]) + f32(1)))
+                    else:
+                        b = min(b, f32((rCur if s == S - 1 else cur[s + 1]) + f32(1)))
+                    b = min(b, f32(prev[s] + f32(1)))
+                    b = min(b, f32((prevL if s == 0 else prev[s - 1]) + D))
+                    b = min(b, f32((prevR if s == S - 1 else prev[s + 1]) + D))
+                    cur[s] = b if T['exists'][s] else INF
                 assert xs + S <= dp  # the strip stays inside its row
                 dist.store(shear_at(xs, y, S, dp), cur)  # the whole strip, as the kernel does
                 T['prev'] = cur.copy()
-                firstv[cb][t] = cur[0]; lastv[cb][t] = cur[S - 1]
+                firstv[cb][t + 1] = cur[0]; lastv[cb][t + 1] = cur[S - 1]
         for T in th:
             T['ring'] = T['next']
         T0 += 2 * P
 
 
-def run_chamfer(cov, box, S, P):
+def run_chamfer(cov, box, S, P, box_only=False):
     """Shear pre-pass, forward, backward; returns the distances read back
     at every (x, y) in the box as the ownership kernels read them."""
     h, w = cov.shape
@@ -156,7 +168,7 @@ def run_chamfer(cov, box, S, P):
     rows = shear_rows(w, h)
     scov = Flat(rows * pitch(w), np.uint8, 0x5A)
     dist = Flat(rows * pitch(w), np.float32, -5)
-    shear_cov(cov, scov, box, w, h, S)
+    shear_cov(cov, scov, box, w, h, S, box_only)
     acc = {}
     kernel(scov, dist, box, w, h, S, P, True, nt, acc)
     kernel(scov, dist, box, w, h, S, P, False, nt, acc)
@@ -179,12 +191,17 @@ def test_schedule_matches_serial_chamfer(seed, S, P):
         cov[:, : w // 3] = 0  # a box that does not start at the chunk edge
     if seed % 4 == 1:
         cov[:, -1] = 1  # a box that reaches the last column
+    if seed % 4 == 2 and h > 4:
+        cov[:2] = 0; cov[-1] = 0  # rows exist above and below the box
     ys, xs = np.nonzero(cov)
     if len(xs) == 0:
         return
     box = (xs.min(), ys.min(), xs.max(), ys.max())
+    want = po.coverage_distance(cov)
     got, acc = run_chamfer(cov, box, S, P)
-    assert np.array_equal(got, po.coverage_distance(cov))
+    assert np.array_equal(got, want)
+    got, _ = run_chamfer(cov, box, S, P, box_only=True)
+    assert np.array_equal(got, want)
     # at every step the strips of one parity (one wavefront) touch one sheared row
     assert all(len(rows) == 1 for rows in acc.values())
 
