@@ -1184,54 +1184,62 @@ __global__ void __launch_bounds__(256) rsexport_kernel(const RChunk* __restrict_
 }
 
 // Level 0: only the owner carries weight (0/1 owner mask); FP32 RGB out.
-// A block takes one 64x16 tile per iteration, each thread final_pix
-// pixels of one column (rows r, r + 16 / final_pix, ...).  The
-// tile's chunk items are staged in LDS once per tile (double buffered by
-// iteration, so one barrier per tile suffices), and every load of a
-// thread's pixels is issued before any of them is used.  So that no load
-// needs a branch, a pixel outside the chunk reads index 0 and a pixel
-// without an owner reads item 0: both are valid addresses (every item
-// plane is chunk-sized), and their values are discarded.
+// A 256-thread block takes one 64x16 tile per iteration, each thread four
+// pixels of one column (rows r, r + 4, r + 8, r + 12):
+//   A. the chunk's contributor fields go to LDS; each thread loads its
+//      owners and ORs them into the tile's owner mask;
+//   B. the level-1 region the tile interpolates from (at most 34 x 10
+//      texels: the level-1 scale is <= 1/2) is staged in LDS once: the
+//      reconstruction, and the hole-filled Gaussians of the first
+//      kFinSlots owners of the tile (a contributor's slot is its rank in
+//      the mask);
+//   C. every pixel interpolates from LDS (a pixel whose owner has no slot
+//      gathers that owner's Gaussians from memory).
+// Each texel's Gaussian is formed once instead of once per pixel; the
+// values are the ones the per-pixel evaluation gives.
 struct FinalItem {
     const void* warped;
     const void* nd1;
     int box[4];
 };
-// Pixels per thread: 4 for the FP32-maths forms; 1 for the FP64 form,
-// whose four-pixel register footprint leaves one wavefront per SIMD.
-template <int MODE>
-constexpr int final_pix() { return MODE == kModeFP64 ? 1 : 4; }
-template <int MODE>
-constexpr int final_threads() { return kWarpTW * kWarpTH / final_pix<MODE>(); }
+constexpr int kFinSlots = 4;
+constexpr int kFinW1 = 36, kFinH1 = 12;  // staged region bound (34 x 10 needed)
 
 template <int MODE>
-__global__ void __launch_bounds__(final_threads<MODE>()) rfinal_kernel(const RItem* __restrict__ items,
-                                                               const RChunk* __restrict__ chunks,
-                                                               const RTile* __restrict__ tiles, int ntiles) {
+__global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ items,
+                                                     const RChunk* __restrict__ chunks,
+                                                     const RTile* __restrict__ tiles, int ntiles) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
-    constexpr int K = final_pix<MODE>();
+    constexpr int K = 4;
     constexpr int kRowStep = kWarpTH / K;
-    __shared__ FinalItem sit[2][I360HIP_MAX_CONTRIBUTORS];
-    const int lx = threadIdx.x & 63, ry = threadIdx.x >> 6;
-    int buf = 0;
-    for (int e = blockIdx.x; e < ntiles; e += gridDim.x, buf ^= 1) {
+    __shared__ FinalItem sit[I360HIP_MAX_CONTRIBUTORS];
+    __shared__ M sR[kFinH1 * kFinW1][4];
+    __shared__ M sG[kFinSlots][kFinH1 * kFinW1][4];
+    __shared__ uint32_t smask;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lx = tid & 63, ry = tid >> 6;
+    if (tid == 0) smask = 0u;
+    lds_barrier();  // the mask is clear before any thread ORs into it
+    for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
         const RTile tl = tiles[e];
         const RChunk& ch = chunks[tl.id];
         const int w = ch.w, h = ch.h, count = ch.count;
-        if (static_cast<int>(threadIdx.x) < count) {
-            const RItem& it = items[ch.first + threadIdx.x];
+        // A
+        if (tid < count) {
+            const RItem& it = items[ch.first + tid];
             FinalItem f;
             f.warped = it.warped;
             f.nd1 = it.nd[1];
             f.box[0] = it.lbox[1][0]; f.box[1] = it.lbox[1][1]; f.box[2] = it.lbox[1][2]; f.box[3] = it.lbox[1][3];
-            sit[buf][threadIdx.x] = f;
+            sit[tid] = f;
         }
         const int x = tl.tx * kWarpTW + lx;
         bool ok[K];
         size_t p[K];
         int o[K];
+        uint32_t bits = 0u;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int y = tl.ty * kWarpTH + ry + kRowStep * k;
@@ -1239,88 +1247,130 @@ __global__ void __launch_bounds__(final_threads<MODE>()) rfinal_kernel(const RIt
             p[k] = ok[k] ? static_cast<size_t>(y) * w + x : 0;
             const int v = gptr(ch.owner)[p[k]];
             o[k] = ok[k] ? v : -1;
+            bits |= o[k] >= 0 ? (1u << o[k]) : 0u;
         }
-        lds_barrier();  // the staged items are visible
+        if (bits) atomicOr(&smask, bits);
+        lds_barrier();
+        const uint32_t mask = smask;
+        const bool deep = ch.levels > 1;  // tile-uniform
+        const int lw1 = deep ? ch.lw[1] : 1, lh1 = deep ? ch.lh[1] : 1;
+        const M sx = expand_scale<M>(lw1, w), sy = expand_scale<M>(lh1, h);
+        // the tile's level-1 region
+        const Coord<M> cA = expand_at_s<M>(tl.tx * kWarpTW, tl.ty * kWarpTH, sx, sy, lw1, lh1);
+        const Coord<M> cB = expand_at_s<M>(min(tl.tx * kWarpTW + kWarpTW - 1, w - 1),
+                                           min(tl.ty * kWarpTH + kWarpTH - 1, h - 1), sx, sy, lw1, lh1);
+        const int X0 = cA.x0, Y0 = cA.y0, W1 = cB.x1 - cA.x0 + 1, H1 = cB.y1 - cA.y0 + 1;
+        const bool staged = deep && count > 0 && W1 <= kFinW1 && H1 <= kFinH1;  // tile-uniform
+        const int nslots = min(__popc(mask), kFinSlots);
+        // B
+        if (staged) {
+            const int n1 = W1 * H1;
+            for (int idx = tid; idx < (nslots + 1) * n1; idx += 256) {
+                const int sl = idx / n1, q = idx - sl * n1;
+                const int ty1 = Y0 + q / W1, tx1 = X0 + q % W1;
+                const size_t t = static_cast<size_t>(ty1) * lw1 + tx1;
+                if (sl == nslots) {
+                    M r[4];
+                    ld4(ch.recon[1], t, r, static_cast<M*>(nullptr));
+                    sR[q][0] = r[0]; sR[q][1] = r[1]; sR[q][2] = r[2];
+                } else {
+                    // contributor of slot sl: the sl-th set bit of the mask
+                    uint32_t m = mask;
+                    for (int j = 0; j < sl; ++j) m &= m - 1u;
+                    const FinalItem& it = sit[__ffs(m) - 1];
+                    M a[4];
+                    ld4(it.nd1, t, a, static_cast<S*>(nullptr));
+                    const bool in = tx1 >= it.box[0] && tx1 <= it.box[2] && ty1 >= it.box[1] && ty1 <= it.box[3];
+                    const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
+                    sG[sl][q][0] = in ? a[0] * inv : M(0);
+                    sG[sl][q][1] = in ? a[1] * inv : M(0);
+                    sG[sl][q][2] = in ? a[2] * inv : M(0);
+                }
+            }
+        }
+        // the warped values (in flight across the barrier)
+        M qv[K][4];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const FinalItem& it = sit[o[k] >= 0 ? o[k] : 0];
+            if (count > 0) ld4(it.warped, p[k], qv[k], static_cast<S*>(nullptr));
+        }
+        lds_barrier();
+        // C
         if (ch.owner_out) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
                 if (ok[k]) gptr(ch.owner_out)[p[k]] = o[k];
         }
-        M v[K][3];
-        if (count == 0) {  // tile-uniform: no contributor, every pixel is 0
-#pragma unroll
-            for (int k = 0; k < K; ++k) v[k][0] = v[k][1] = v[k][2] = M(0);
-        } else {
-            const FinalItem* fi[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) fi[k] = &sit[buf][o[k] >= 0 ? o[k] : 0];
-            M q[K][4];
-#pragma unroll
-            for (int k = 0; k < K; ++k) ld4(fi[k]->warped, p[k], q[k], static_cast<S*>(nullptr));
-            if (ch.levels > 1) {  // tile-uniform
-                const int lw1 = ch.lw[1], lh1 = ch.lh[1];
-                const M sx = expand_scale<M>(lw1, w), sy = expand_scale<M>(lh1, h);
-                Coord<M> c[K];
-                M a[K][4][4], r[K][4][4];
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int y = tl.ty * kWarpTH + ry + kRowStep * k;
-                    c[k] = expand_at_s<M>(x, y, sx, sy, lw1, lh1);
-                    const size_t t00 = static_cast<size_t>(c[k].y0) * lw1 + c[k].x0;
-                    const size_t t10 = static_cast<size_t>(c[k].y0) * lw1 + c[k].x1;
-                    const size_t t01 = static_cast<size_t>(c[k].y1) * lw1 + c[k].x0;
-                    const size_t t11 = static_cast<size_t>(c[k].y1) * lw1 + c[k].x1;
-                    ld4(fi[k]->nd1, t00, a[k][0], static_cast<S*>(nullptr));
-                    ld4(fi[k]->nd1, t10, a[k][1], static_cast<S*>(nullptr));
-                    ld4(fi[k]->nd1, t01, a[k][2], static_cast<S*>(nullptr));
-                    ld4(fi[k]->nd1, t11, a[k][3], static_cast<S*>(nullptr));
-                    ld4(ch.recon[1], t00, r[k][0], static_cast<M*>(nullptr));
-                    ld4(ch.recon[1], t10, r[k][1], static_cast<M*>(nullptr));
-                    ld4(ch.recon[1], t01, r[k][2], static_cast<M*>(nullptr));
-                    ld4(ch.recon[1], t11, r[k][3], static_cast<M*>(nullptr));
-                }
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int* bx = fi[k]->box;
-                    const int xs[4] = {c[k].x0, c[k].x1, c[k].x0, c[k].x1};
-                    const int ys[4] = {c[k].y0, c[k].y0, c[k].y1, c[k].y1};
-                    M g[4][3];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        // hole-filled Gaussian, zero outside the support box
-                        const bool in = xs[j] >= bx[0] && xs[j] <= bx[2] && ys[j] >= bx[1] && ys[j] <= bx[3];
-                        const M inv = (a[k][j][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][j][3]) : M(0);
-#pragma unroll
-                        for (int ch3 = 0; ch3 < 3; ++ch3) g[j][ch3] = in ? a[k][j][ch3] * inv : M(0);
-                    }
-                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(q[k][3]) : M(0);
-#pragma unroll
-                    for (int ch3 = 0; ch3 < 3; ++ch3) {
-                        const M up = lerp2m<M>(c[k], g[0][ch3], g[1][ch3], g[2][ch3], g[3][ch3]);
-                        const M ur = lerp2m<M>(c[k], r[k][0][ch3], r[k][1][ch3], r[k][2][ch3], r[k][3][ch3]);
-                        v[k][ch3] = (((q[k][ch3] * q[k][3]) * inv0) - up) + ur;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const M inv0 = (q[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(q[k][3]) : M(0);
-#pragma unroll
-                    for (int ch3 = 0; ch3 < 3; ++ch3) v[k][ch3] = (q[k][ch3] * q[k][3]) * inv0;
-                }
-            }
-        }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (!ok[k]) continue;
-            auto* dst = gptr(ch.out + p[k] * 3);
+            M v[3] = {M(0), M(0), M(0)};
+            if (o[k] >= 0) {
+                const M inv0 = (qv[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(qv[k][3]) : M(0);
 #pragma unroll
-            for (int ch3 = 0; ch3 < 3; ++ch3) {
-                M t = v[k][ch3] < M(1) ? v[k][ch3] : M(1);
-                t = M(0) < t ? t : M(0);
-                dst[ch3] = o[k] < 0 ? 0.0f : static_cast<float>(t);
+                for (int c3 = 0; c3 < 3; ++c3) v[c3] = (qv[k][c3] * qv[k][3]) * inv0;
+                if (deep) {
+                    const int y = tl.ty * kWarpTH + ry + kRowStep * k;
+                    const Coord<M> c = expand_at_s<M>(x, y, sx, sy, lw1, lh1);
+                    const int slot = __popc(mask & ((1u << o[k]) - 1u));
+                    M g[4][3], r[4][3];
+                    const int xs[4] = {c.x0, c.x1, c.x0, c.x1};
+                    const int ys[4] = {c.y0, c.y0, c.y1, c.y1};
+                    if (staged) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int q = (ys[j] - Y0) * W1 + (xs[j] - X0);
+#pragma unroll
+                            for (int c3 = 0; c3 < 3; ++c3) r[j][c3] = sR[q][c3];
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            M rr[4];
+                            ld4(ch.recon[1], static_cast<size_t>(ys[j]) * lw1 + xs[j], rr, static_cast<M*>(nullptr));
+#pragma unroll
+                            for (int c3 = 0; c3 < 3; ++c3) r[j][c3] = rr[c3];
+                        }
+                    }
+                    if (staged && slot < kFinSlots) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int q = (ys[j] - Y0) * W1 + (xs[j] - X0);
+#pragma unroll
+                            for (int c3 = 0; c3 < 3; ++c3) g[j][c3] = sG[slot][q][c3];
+                        }
+                    } else {
+                        const FinalItem& it = sit[o[k]];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            M a[4];
+                            ld4(it.nd1, static_cast<size_t>(ys[j]) * lw1 + xs[j], a, static_cast<S*>(nullptr));
+                            const bool in = xs[j] >= it.box[0] && xs[j] <= it.box[2] && ys[j] >= it.box[1] && ys[j] <= it.box[3];
+                            const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
+#pragma unroll
+                            for (int c3 = 0; c3 < 3; ++c3) g[j][c3] = in ? a[c3] * inv : M(0);
+                        }
+                    }
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        const M up = lerp2m<M>(c, g[0][c3], g[1][c3], g[2][c3], g[3][c3]);
+                        const M ur = lerp2m<M>(c, r[0][c3], r[1][c3], r[2][c3], r[3][c3]);
+                        v[c3] = (v[c3] - up) + ur;
+                    }
+                }
+            }
+            if (ok[k]) {
+                auto* dst = gptr(ch.out + p[k] * 3);
+#pragma unroll
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    M t = v[c3] < M(1) ? v[c3] : M(1);
+                    t = M(0) < t ? t : M(0);
+                    dst[c3] = o[k] < 0 ? 0.0f : static_cast<float>(t);
+                }
             }
         }
+        if (tid == 0) smask = 0u;  // read by every thread before the barrier above
+        lds_barrier();             // the next tile reuses sit, sR, sG and the mask
     }
 }
 
@@ -1971,8 +2021,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
     if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
     RL(st, "final_level0", area0 * (4.0 + 4 * sS + 12.0),
-       rfinal_kernel<MODE><<<persistent_grid(ctx, n, 8 / final_pix<MODE>()), final_threads<MODE>(), 0, s>>>(
-           T.items, T.chunks, T.tiles + off, n));
+       rfinal_kernel<MODE><<<persistent_grid(ctx, n, 8), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
 }
 
 struct RenderCache {
