@@ -76,6 +76,15 @@ __device__ __forceinline__ void st4(void* p, size_t i, const double (&v)[4], dou
     q.x = v[0]; q.y = v[1]; q.z = v[2]; q.w = v[3];
     gptr(static_cast<d4v*>(p))[i] = q;
 }
+// Four storage-type elements at index i, unconverted (so that a batch of
+// loads can be issued before any conversion waits on one).
+template <typename S>
+__device__ __forceinline__ void ld4raw(const void* p, size_t i, S (&o)[4]) {
+    typedef S s4v __attribute__((ext_vector_type(4)));
+    const s4v q = gptr(static_cast<const s4v*>(p))[i];
+    o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+}
+
 template <typename M, typename S>
 __device__ __forceinline__ M ld1(const void* p, size_t i) {
     return static_cast<M>(gptr(static_cast<const S*>(p))[i]);
@@ -85,9 +94,17 @@ __device__ __forceinline__ void st1(void* p, size_t i, M v) {
     gptr(static_cast<S*>(p))[i] = static_cast<S>(v);
 }
 
-__device__ __forceinline__ bool in_box(const int* b, int x, int y) {
-    const auto* g = gptr(b);
-    return x >= g[0] && x <= g[2] && y >= g[1] && y <= g[3];
+// A box x0, y0, x1, y1 (inclusive) held in registers: read once per tile
+// or item, so that no per-pixel test issues (and waits for) memory loads.
+struct IBox {
+    int x0, y0, x1, y1;
+};
+__device__ __forceinline__ IBox box_of(const int* b) {
+    const auto* g = gptr(b);  // item boxes are 8-byte aligned: four dword loads
+    return IBox{g[0], g[1], g[2], g[3]};
+}
+__device__ __forceinline__ bool in_box(const IBox& b, int x, int y) {
+    return x >= b.x0 && x <= b.x1 && y >= b.y0 && y <= b.y1;
 }
 
 // ----------------------------------------------------------- FP64 map ---
@@ -348,7 +365,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
         const int tx0 = tl.tx * kWarpTW, ty0 = tl.ty * kWarpTH;
         TileMap tm{};
         if (MM == kModePrecision) tm = maps[e];
-        const int* fb = it.lbox[0];
+        const IBox fb = box_of(it.lbox[0]);
         const void* src = it.src;
         const int fw = it.fw, fmt = it.fmt;
         const size_t nbytes = static_cast<size_t>(fw) * it.fh * (fmt == I360HIP_PIX_F16 ? 6 : 12);
@@ -675,7 +692,7 @@ __global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ 
         if (phase_skip(ch, it.chunk, l + 1, phase, cflag)) continue;  // block-uniform
         const int w = ch.lw[l], h = ch.lh[l], ow = ch.lw[l + 1], oh = ch.lh[l + 1];
         const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
-        const int* sb = L0 ? it.box : it.lbox[l];
+        const IBox sb = box_of(L0 ? it.box : it.lbox[l]);
         for (int idx = threadIdx.x; idx < kRedRows * kRedTW; idx += 256) {
             const int rr = idx / kRedTW, xx = idx % kRedTW;
             const int ox = ox0 + xx;
@@ -764,7 +781,7 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
         const RChunk& ch = chunks[it.chunk];
         const int w = ch.lw[0], h = ch.lh[0];
         const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
-        const int* sb = it.box;
+        const IBox sb = box_of(it.box);
         const h4* wp = static_cast<const h4*>(it.warped);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -898,7 +915,7 @@ __device__ __forceinline__ M lerp2m(const Coord<M>& c, M v00, M v10, M v01, M v1
 
 // Hole-filled Gaussian at one level-l sample; zero outside the support box.
 template <typename M, typename S>
-__device__ __forceinline__ void gauss_m(const void* nd, const int* box, int w, int x, int y, M (&g)[3]) {
+__device__ __forceinline__ void gauss_m(const void* nd, const IBox& box, int w, int x, int y, M (&g)[3]) {
     if (!in_box(box, x, y)) { g[0] = g[1] = g[2] = M(0); return; }
     M q[4];
     ld4(nd, static_cast<size_t>(y) * w + x, q, static_cast<S*>(nullptr));
@@ -907,7 +924,7 @@ __device__ __forceinline__ void gauss_m(const void* nd, const int* box, int w, i
 }
 
 template <typename M, typename S>
-__device__ __forceinline__ void expand_gauss_m(const void* nd, const int* box, int sw, const Coord<M>& c,
+__device__ __forceinline__ void expand_gauss_m(const void* nd, const IBox& box, int sw, const Coord<M>& c,
                                                M (&o)[3]) {
     M a[3], b[3], d[3], e[3];
     gauss_m<M, S>(nd, box, sw, c.x0, c.y0, a);
@@ -958,14 +975,15 @@ __global__ void __launch_bounds__(256) rcompose_kernel(const RItem* __restrict__
         const size_t p = static_cast<size_t>(y) * w + x;
         for (int k = 0; k < ch.count; ++k) {
             const RItem& it = items[ch.first + k];
-            if (!in_box(it.lbox[l], x, y)) continue;
+            const IBox bl = box_of(it.lbox[l]);
+            if (!in_box(bl, x, y)) continue;
             const M wt = ld1<M, S>(it.wt[l], p);
             if (wt == M(0)) continue;
             M g[3];
-            gauss_m<M, S>(it.nd[l], it.lbox[l], w, x, y, g);
+            gauss_m<M, S>(it.nd[l], bl, w, x, y, g);
             if (!top) {
                 M up[3];
-                expand_gauss_m<M, S>(it.nd[l + 1], it.lbox[l + 1], nw, c, up);
+                expand_gauss_m<M, S>(it.nd[l + 1], box_of(it.lbox[l + 1]), nw, c, up);
 #pragma unroll
                 for (int q = 0; q < 3; ++q) g[q] = g[q] - up[q];
             }
@@ -1009,11 +1027,12 @@ __global__ void __launch_bounds__(256) rsfeat_kernel(const RItem* __restrict__ i
         M base[3] = {M(0), M(0), M(0)};
         for (int k = 0; k < ch.count; ++k) {
             const RItem& it = items[ch.first + k];
-            if (!in_box(it.lbox[l], x, y)) continue;
+            const IBox bl = box_of(it.lbox[l]);
+            if (!in_box(bl, x, y)) continue;
             const M wt = ld1<M, S>(it.wt[l], p);
             if (wt == M(0)) continue;
             M g[3];
-            gauss_m<M, S>(it.nd[l], it.lbox[l], w, x, y, g);
+            gauss_m<M, S>(it.nd[l], bl, w, x, y, g);
 #pragma unroll
             for (int q = 0; q < 3; ++q) base[q] = base[q] + wt * g[q];
             if (wt > wa) {
@@ -1262,30 +1281,52 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         const int X0 = cA.x0, Y0 = cA.y0, W1 = cB.x1 - cA.x0 + 1, H1 = cB.y1 - cA.y0 + 1;
         const bool staged = deep && count > 0 && W1 <= kFinW1 && H1 <= kFinH1;  // tile-uniform
         const int nslots = min(__popc(mask), kFinSlots);
-        // B
+        // B: every staging load of a thread is issued before any is used
+        // (indices past the region are clamped to its last texel and not
+        // written)
         if (staged) {
             const int n1 = W1 * H1;
-            for (int idx = tid; idx < (nslots + 1) * n1; idx += 256) {
+            constexpr int kRPer = (kFinW1 * kFinH1 + 255) / 256;
+            constexpr int kGPer = (kFinSlots * kFinW1 * kFinH1 + 255) / 256;
+            M rr[kRPer][4];
+            S ga[kGPer][4];
+            const FinalItem* gi[kGPer];
+#pragma unroll
+            for (int u = 0; u < kRPer; ++u) {
+                const int q = min(tid + 256 * u, n1 - 1);
+                ld4(ch.recon[1], static_cast<size_t>(Y0 + q / W1) * lw1 + X0 + q % W1, rr[u], static_cast<M*>(nullptr));
+            }
+#pragma unroll
+            for (int u = 0; u < kGPer; ++u) {
+                const int idx = min(tid + 256 * u, max(nslots * n1 - 1, 0));
+                const int sl = idx / n1, q = idx - sl * n1;
+                // contributor of slot sl: the sl-th set bit of the mask
+                uint32_t m = mask;
+                for (int j = 0; j < sl; ++j) m &= m - 1u;
+                gi[u] = &sit[m ? __ffs(m) - 1 : 0];
+                const size_t t = static_cast<size_t>(Y0 + q / W1) * lw1 + X0 + q % W1;
+                if (nslots > 0) ld4raw(gi[u]->nd1, t, ga[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kRPer; ++u) {
+                const int q = tid + 256 * u;
+                if (q < n1) { sR[q][0] = rr[u][0]; sR[q][1] = rr[u][1]; sR[q][2] = rr[u][2]; }
+            }
+#pragma unroll
+            for (int u = 0; u < kGPer; ++u) {
+                const int idx = tid + 256 * u;
+                if (idx >= nslots * n1) continue;
                 const int sl = idx / n1, q = idx - sl * n1;
                 const int ty1 = Y0 + q / W1, tx1 = X0 + q % W1;
-                const size_t t = static_cast<size_t>(ty1) * lw1 + tx1;
-                if (sl == nslots) {
-                    M r[4];
-                    ld4(ch.recon[1], t, r, static_cast<M*>(nullptr));
-                    sR[q][0] = r[0]; sR[q][1] = r[1]; sR[q][2] = r[2];
-                } else {
-                    // contributor of slot sl: the sl-th set bit of the mask
-                    uint32_t m = mask;
-                    for (int j = 0; j < sl; ++j) m &= m - 1u;
-                    const FinalItem& it = sit[__ffs(m) - 1];
-                    M a[4];
-                    ld4(it.nd1, t, a, static_cast<S*>(nullptr));
-                    const bool in = tx1 >= it.box[0] && tx1 <= it.box[2] && ty1 >= it.box[1] && ty1 <= it.box[3];
-                    const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
-                    sG[sl][q][0] = in ? a[0] * inv : M(0);
-                    sG[sl][q][1] = in ? a[1] * inv : M(0);
-                    sG[sl][q][2] = in ? a[2] * inv : M(0);
-                }
+                const FinalItem& it = *gi[u];
+                M a[4];
+#pragma unroll
+                for (int c4 = 0; c4 < 4; ++c4) a[c4] = static_cast<M>(ga[u][c4]);
+                const bool in = tx1 >= it.box[0] && tx1 <= it.box[2] && ty1 >= it.box[1] && ty1 <= it.box[3];
+                const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
+                sG[sl][q][0] = in ? a[0] * inv : M(0);
+                sG[sl][q][1] = in ? a[1] * inv : M(0);
+                sG[sl][q][2] = in ? a[2] * inv : M(0);
             }
         }
         // the warped values (in flight across the barrier)
