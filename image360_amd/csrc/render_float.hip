@@ -76,15 +76,6 @@ __device__ __forceinline__ void st4(void* p, size_t i, const double (&v)[4], dou
     q.x = v[0]; q.y = v[1]; q.z = v[2]; q.w = v[3];
     gptr(static_cast<d4v*>(p))[i] = q;
 }
-// Four storage-type elements at index i, unconverted (so that a batch of
-// loads can be issued before any conversion waits on one).
-template <typename S>
-__device__ __forceinline__ void ld4raw(const void* p, size_t i, S (&o)[4]) {
-    typedef S s4v __attribute__((ext_vector_type(4)));
-    const s4v q = gptr(static_cast<const s4v*>(p))[i];
-    o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
-}
-
 template <typename M, typename S>
 __device__ __forceinline__ M ld1(const void* p, size_t i) {
     return static_cast<M>(gptr(static_cast<const S*>(p))[i]);
@@ -744,45 +735,49 @@ __global__ void __launch_bounds__(256) rreduce_kernel(const RItem* __restrict__ 
     }
 }
 
-// Level 0 -> 1 of the FP16 forms: the tile's level-0 window (19 rows x 132
-// columns, replicate-clamped, zero outside the item's coverage box) is
+// Level l -> l+1 of the FP16 forms: the tile's level-l window (19 rows x
+// 132 columns, replicate-clamped, zero outside the item's support) is
 // staged into LDS, and both blur passes read LDS.  Same arithmetic as
-// rreduce_kernel<MODE, true>.  Software pipelined: the next tile's window
-// loads are issued as soon as the current window is in LDS, and stay in
-// flight across the blur passes (the barriers order LDS only).
+// rreduce_kernel<MODE, L0>.  Level 0 reads the warped plane (premultiplied
+// by its coverage) and the owner map (weight = owned by this item); the
+// levels above read the numerator/denominator and weight planes.
+// Software pipelined: the next tile's window loads are issued as soon as
+// the current window is in LDS, and stay in flight across the blur passes
+// (the barriers order LDS only).
 constexpr int kRed0Cols = 2 * kRedTW + 4;
-template <int MODE>
-__global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__ items,
-                                                       const RChunk* __restrict__ chunks,
-                                                       const RTile* __restrict__ tiles, int ntiles,
-                                                       int phase, const int* __restrict__ cflag) {
+template <int MODE, bool L0>
+__global__ void __launch_bounds__(256) rreduce_s_kernel(const RItem* __restrict__ items,
+                                                        const RChunk* __restrict__ chunks,
+                                                        const RTile* __restrict__ tiles, int ntiles, int l,
+                                                        int phase, const int* __restrict__ cflag) {
     using M = float;
     constexpr int NE = kRedRows * kRed0Cols;  // staged elements
     constexpr int PER = (NE + 255) / 256;
     __shared__ h4 sq[NE];
-    __shared__ uint8_t sw[NE];
+    __shared__ _Float16 sw[NE];
     __shared__ M tmp[5][kRedRows][kRedTW];
     // first tile at or after f (stride gridDim.x) that this phase processes
     auto next_tile = [&](int f) {
         for (; f < ntiles; f += gridDim.x) {
             const RItem& it = items[tiles[f].id];
-            if (!phase_skip(chunks[it.chunk], it.chunk, 1, phase, cflag)) break;
+            if (!phase_skip(chunks[it.chunk], it.chunk, l + 1, phase, cflag)) break;
         }
         return f;
     };
     h4 qv[PER];
-    int ov[PER];
+    int ov[PER];       // level 0: owner
+    _Float16 wv[PER];  // levels above: weight
     bool inb[PER];
     // issues tile f's window loads (every address valid: elements outside
-    // the box read element 0 and are zeroed at the commit)
+    // the support read element 0 and are zeroed at the commit)
     auto fetch = [&](int f) {
         const RTile tl = tiles[f];
         const RItem& it = items[tl.id];
         const RChunk& ch = chunks[it.chunk];
-        const int w = ch.lw[0], h = ch.lh[0];
+        const int w = ch.lw[l], h = ch.lh[l];
         const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
-        const IBox sb = box_of(it.box);
-        const h4* wp = static_cast<const h4*>(it.warped);
+        const IBox sb = box_of(L0 ? it.box : it.lbox[l]);
+        const h4* qp = static_cast<const h4*>(L0 ? it.warped : it.nd[l]);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = threadIdx.x + 256 * k;
@@ -791,8 +786,9 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
             const int ry = min(max(2 * oy0 - 2 + rr, 0), h - 1);
             inb[k] = i < NE && in_box(sb, cx, ry);
             const size_t p = inb[k] ? static_cast<size_t>(ry) * w + cx : 0;
-            qv[k] = gptr(wp)[p];
-            ov[k] = gptr(ch.owner)[p];
+            qv[k] = gptr(qp)[p];
+            if (L0) ov[k] = gptr(ch.owner)[p];
+            else wv[k] = gptr(static_cast<const _Float16*>(it.wt[l]))[p];
         }
     };
     int e = next_tile(blockIdx.x);
@@ -801,7 +797,7 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
         const RTile tl = tiles[e];
         const RItem& it = items[tl.id];
         const RChunk& ch = chunks[it.chunk];
-        const int ow = ch.lw[1], oh = ch.lh[1];
+        const int ow = ch.lw[l + 1], oh = ch.lh[l + 1];
         const int ox0 = tl.tx * kRedTW, oy0 = tl.ty * kRedTH;
         const int local = it.local;
 #pragma unroll
@@ -811,7 +807,8 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
             h4 z;
             z.x = z.y = z.z = z.w = static_cast<_Float16>(0.0f);
             sq[i] = inb[k] ? qv[k] : z;
-            sw[i] = (inb[k] && ov[k] == local) ? 1 : 0;
+            if (L0) sw[i] = static_cast<_Float16>((inb[k] && ov[k] == local) ? 1.0f : 0.0f);
+            else sw[i] = inb[k] ? wv[k] : static_cast<_Float16>(0.0f);
         }
         lds_barrier();  // window staged; the previous tile's vertical pass is done with tmp
         const int nx = next_tile(e + gridDim.x);
@@ -825,11 +822,17 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
                 const int j = 2 * xx + k;
                 const h4 q = sq[rr * kRed0Cols + j];
                 const M q0 = static_cast<M>(q.x), q1 = static_cast<M>(q.y), q2 = static_cast<M>(q.z), q3 = static_cast<M>(q.w);
-                v[k][0] = q0 * q3;
-                v[k][1] = q1 * q3;
-                v[k][2] = q2 * q3;
+                if (L0) {
+                    v[k][0] = q0 * q3;
+                    v[k][1] = q1 * q3;
+                    v[k][2] = q2 * q3;
+                } else {
+                    v[k][0] = q0;
+                    v[k][1] = q1;
+                    v[k][2] = q2;
+                }
                 v[k][3] = q3;
-                v[k][4] = sw[rr * kRed0Cols + j] ? M(1) : M(0);
+                v[k][4] = static_cast<M>(sw[rr * kRed0Cols + j]);
             }
 #pragma unroll
             for (int c = 0; c < 5; ++c) tmp[c][rr][xx] = blur5m<M>(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c]);
@@ -846,8 +849,8 @@ __global__ void __launch_bounds__(256) rreduce0_kernel(const RItem* __restrict__
                 o[c] = blur5m<M>(tmp[c][r - 2][xx], tmp[c][r - 1][xx], tmp[c][r][xx], tmp[c][r + 1][xx], tmp[c][r + 2][xx]);
             const size_t q = static_cast<size_t>(oy) * ow + ox;
             const M nd[4] = {o[0], o[1], o[2], o[3]};
-            st4(it.nd[1], q, nd, static_cast<_Float16*>(nullptr));
-            st1<_Float16, M>(it.wt[1], q, o[4]);
+            st4(it.nd[l + 1], q, nd, static_cast<_Float16*>(nullptr));
+            st1<_Float16, M>(it.wt[l + 1], q, o[4]);
         }
         e = nx;
     }
@@ -1281,52 +1284,30 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         const int X0 = cA.x0, Y0 = cA.y0, W1 = cB.x1 - cA.x0 + 1, H1 = cB.y1 - cA.y0 + 1;
         const bool staged = deep && count > 0 && W1 <= kFinW1 && H1 <= kFinH1;  // tile-uniform
         const int nslots = min(__popc(mask), kFinSlots);
-        // B: every staging load of a thread is issued before any is used
-        // (indices past the region are clamped to its last texel and not
-        // written)
+        // B
         if (staged) {
             const int n1 = W1 * H1;
-            constexpr int kRPer = (kFinW1 * kFinH1 + 255) / 256;
-            constexpr int kGPer = (kFinSlots * kFinW1 * kFinH1 + 255) / 256;
-            M rr[kRPer][4];
-            S ga[kGPer][4];
-            const FinalItem* gi[kGPer];
-#pragma unroll
-            for (int u = 0; u < kRPer; ++u) {
-                const int q = min(tid + 256 * u, n1 - 1);
-                ld4(ch.recon[1], static_cast<size_t>(Y0 + q / W1) * lw1 + X0 + q % W1, rr[u], static_cast<M*>(nullptr));
-            }
-#pragma unroll
-            for (int u = 0; u < kGPer; ++u) {
-                const int idx = min(tid + 256 * u, max(nslots * n1 - 1, 0));
-                const int sl = idx / n1, q = idx - sl * n1;
-                // contributor of slot sl: the sl-th set bit of the mask
-                uint32_t m = mask;
-                for (int j = 0; j < sl; ++j) m &= m - 1u;
-                gi[u] = &sit[m ? __ffs(m) - 1 : 0];
-                const size_t t = static_cast<size_t>(Y0 + q / W1) * lw1 + X0 + q % W1;
-                if (nslots > 0) ld4raw(gi[u]->nd1, t, ga[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < kRPer; ++u) {
-                const int q = tid + 256 * u;
-                if (q < n1) { sR[q][0] = rr[u][0]; sR[q][1] = rr[u][1]; sR[q][2] = rr[u][2]; }
-            }
-#pragma unroll
-            for (int u = 0; u < kGPer; ++u) {
-                const int idx = tid + 256 * u;
-                if (idx >= nslots * n1) continue;
+            for (int idx = tid; idx < (nslots + 1) * n1; idx += 256) {
                 const int sl = idx / n1, q = idx - sl * n1;
                 const int ty1 = Y0 + q / W1, tx1 = X0 + q % W1;
-                const FinalItem& it = *gi[u];
-                M a[4];
-#pragma unroll
-                for (int c4 = 0; c4 < 4; ++c4) a[c4] = static_cast<M>(ga[u][c4]);
-                const bool in = tx1 >= it.box[0] && tx1 <= it.box[2] && ty1 >= it.box[1] && ty1 <= it.box[3];
-                const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
-                sG[sl][q][0] = in ? a[0] * inv : M(0);
-                sG[sl][q][1] = in ? a[1] * inv : M(0);
-                sG[sl][q][2] = in ? a[2] * inv : M(0);
+                const size_t t = static_cast<size_t>(ty1) * lw1 + tx1;
+                if (sl == nslots) {
+                    M r[4];
+                    ld4(ch.recon[1], t, r, static_cast<M*>(nullptr));
+                    sR[q][0] = r[0]; sR[q][1] = r[1]; sR[q][2] = r[2];
+                } else {
+                    // contributor of slot sl: the sl-th set bit of the mask
+                    uint32_t m = mask;
+                    for (int j = 0; j < sl; ++j) m &= m - 1u;
+                    const FinalItem& it = sit[__ffs(m) - 1];
+                    M a[4];
+                    ld4(it.nd1, t, a, static_cast<S*>(nullptr));
+                    const bool in = tx1 >= it.box[0] && tx1 <= it.box[2] && ty1 >= it.box[1] && ty1 <= it.box[3];
+                    const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
+                    sG[sl][q][0] = in ? a[0] * inv : M(0);
+                    sG[sl][q][1] = in ? a[1] * inv : M(0);
+                    sG[sl][q][2] = in ? a[2] * inv : M(0);
+                }
             }
         }
         // the warped values (in flight across the barrier)
@@ -2001,7 +1982,9 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
             if (n == 0) continue;
             const double bytes = static_cast<double>(n) * kRedTW * kRedTH * (4.0 * (4 * sS + sS) + (4 * sS + sS));
             if (l == 0 && MM == kModePrecision)
-                RL(st, "reduce_l0", bytes, rreduce0_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, phase, cflag));
+                RL(st, "reduce_l0", bytes, rreduce_s_kernel<MODE, true><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, 0, phase, cflag));
+            else if (MM == kModePrecision)
+                RL(st, "reduce", bytes, rreduce_s_kernel<MODE, false><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
             else if (l == 0)
                 RL(st, "reduce_l0", bytes, rreduce_kernel<MODE, true><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
             else
