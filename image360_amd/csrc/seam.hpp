@@ -193,7 +193,9 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
     SeamGeom g;
     if (!seam_geom(boxes, ci, cj, g)) return;
     const int lines = g.lines;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wave-uniform values are made scalar (readfirstlane) so that the
+    // candidate loop below is a scalar loop with scalar lane selects
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     // none contested: no DP and no reassignment (SeamBlend.cpp:409)
     if (*gptr(contested) == 0) return;
 
@@ -238,7 +240,8 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
         if (more) fetch(b0 + kDpLines);
         const int nb = min(kDpLines, lines - b0);
         for (int l = 0; l < nb; ++l) {
-            const int lo = sband[buf][l][0], hi = sband[buf][l][1];
+            const int lo = __builtin_amdgcn_readfirstlane(sband[buf][l][0]);
+            const int hi = __builtin_amdgcn_readfirstlane(sband[buf][l][1]);
             if (hi < lo) continue;  // block-uniform
             const int lw = hi - lo + 1;
             const int line = b0 + l;
@@ -249,11 +252,15 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
             } else {
                 const int t = lo + lane;
                 M best = SeamMax<M>::value;
-                const int p0 = wave * kSpan, p1 = min(p0 + kSpan, lastW);
-                for (int pk = p0; pk < p1; ++pk) {
-                    const M dprev = lane_value(dpv, pk);
-                    const M cand = dprev + Img::smooth() * fabs(static_cast<M>(t - (lastLo + pk)));
-                    if (cand < best) { best = cand; bp = pk; }
+                const int p0 = wave * kSpan;
+#pragma unroll
+                for (int i = 0; i < kSpan; ++i) {
+                    const int pk = p0 + i;
+                    if (pk < lastW) {  // uniform
+                        const M dprev = lane_value(dpv, pk);
+                        const M cand = dprev + Img::smooth() * fabs(static_cast<M>(t - (lastLo + pk)));
+                        if (cand < best) { best = cand; bp = pk; }
+                    }
                 }
                 const int pb = ord & 1;
                 ++ord;
