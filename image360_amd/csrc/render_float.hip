@@ -317,6 +317,26 @@ __device__ __forceinline__ void decode_pair_f16(const u4a& q, uint32_t sh, const
     b[2] = static_cast<M>(static_cast<float>(d2.y));
 }
 
+// As decode_pair_f16, keeping the FP16 values (the precision forms blend
+// them with mixed-precision FMAs, FP32 accumulation).
+__device__ __forceinline__ void decode_pair_h(const u4a& q, uint32_t sh, const void* src, size_t idx,
+                                              _Float16 (&a)[3], _Float16 (&b)[3]) {
+    if (sh > 4) {  // last pair of an odd-sized frame: texel by texel
+        const auto* p = gptr(static_cast<const _Float16*>(src) + idx * 3);
+        a[0] = p[0]; a[1] = p[1]; a[2] = p[2];
+        b[0] = p[3]; b[1] = p[4]; b[2] = p[5];
+        return;
+    }
+    const bool s = sh >= 4;
+    const uint32_t bits = (sh & 2u) * 8u;
+    const uint32_t w0 = s ? q.y : q.x, w1 = s ? q.z : q.y, w2 = s ? q.w : q.z, w3 = s ? 0u : q.w;
+    const h2v d0 = __builtin_bit_cast(h2v, __builtin_amdgcn_alignbit(w1, w0, bits));
+    const h2v d1 = __builtin_bit_cast(h2v, __builtin_amdgcn_alignbit(w2, w1, bits));
+    const h2v d2 = __builtin_bit_cast(h2v, __builtin_amdgcn_alignbit(w3, w2, bits));
+    a[0] = d0.x; a[1] = d0.y; a[2] = d1.x;
+    b[0] = d1.y; b[1] = d2.x; b[2] = d2.y;
+}
+
 // Per-tile FP32 maps of the precision forms, one thread per warp tile
 // (the FP64 reduction runs fully parallel here instead of serially inside
 // the warp's tile loop).
@@ -379,8 +399,12 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
             ok[r] = in[r] && map_pixel<MM, M>(it, ch, tm, x, y, lx, ly, sx0, sy0, fx[r], fy[r]);
             i0[r] = ok[r] ? static_cast<size_t>(sy0) * fw + sx0 : 0;
         }
-        // phase 2: loads
-        M a[R][3], b[R][3], c[R][3], d[R][3];
+        // phase 2: loads and the bilinear blend of every row
+        M val[R][3];
+        auto weights = [&](int r, M (&wq)[4]) {
+            const M omx = M(1) - fx[r], omy = M(1) - fy[r];
+            wq[0] = omx * omy; wq[1] = fx[r] * omy; wq[2] = omx * fy[r]; wq[3] = fx[r] * fy[r];
+        };
         if (fmt == I360HIP_PIX_F16) {
             u4a q0[R], q1[R];
             uint32_t s0[R], s1[R];
@@ -391,19 +415,44 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                decode_pair_f16<M>(q0[r], s0[r], src, i0[r], a[r], b[r]);
-                decode_pair_f16<M>(q1[r], s1[r], src, i0[r] + fw, c[r], d[r]);
+                M wq[4];
+                weights(r, wq);
+                if constexpr (MM == kModePrecision) {
+                    // precision forms: the FP16 texels go straight into FP32
+                    // FMAs (mixed-precision multiply-add), same order of terms
+                    _Float16 a[3], b[3], c[3], d[3];
+                    decode_pair_h(q0[r], s0[r], src, i0[r], a, b);
+                    decode_pair_h(q1[r], s1[r], src, i0[r] + fw, c, d);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        float acc = __builtin_fmaf(wq[0], static_cast<float>(a[k]), 0.0f);
+                        acc = __builtin_fmaf(wq[1], static_cast<float>(b[k]), acc);
+                        acc = __builtin_fmaf(wq[2], static_cast<float>(c[k]), acc);
+                        val[r][k] = __builtin_fmaf(wq[3], static_cast<float>(d[k]), acc);
+                    }
+                } else {
+                    M a[3], b[3], c[3], d[3];
+                    decode_pair_f16<M>(q0[r], s0[r], src, i0[r], a, b);
+                    decode_pair_f16<M>(q1[r], s1[r], src, i0[r] + fw, c, d);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) val[r][k] = ((wq[0] * a[k] + wq[1] * b[k]) + wq[2] * c[k]) + wq[3] * d[k];
+                }
             }
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                fetch_rgb<M>(src, fmt, i0[r], a[r]);
-                fetch_rgb<M>(src, fmt, i0[r] + 1, b[r]);
-                fetch_rgb<M>(src, fmt, i0[r] + fw, c[r]);
-                fetch_rgb<M>(src, fmt, i0[r] + fw + 1, d[r]);
+                M a[3], b[3], c[3], d[3];
+                fetch_rgb<M>(src, fmt, i0[r], a);
+                fetch_rgb<M>(src, fmt, i0[r] + 1, b);
+                fetch_rgb<M>(src, fmt, i0[r] + fw, c);
+                fetch_rgb<M>(src, fmt, i0[r] + fw + 1, d);
+                M wq[4];
+                weights(r, wq);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) val[r][k] = ((wq[0] * a[k] + wq[1] * b[k]) + wq[2] * c[k]) + wq[3] * d[k];
             }
         }
-        // phase 3: blend, store
+        // phase 3: gain, clamp, store
         int bx0 = INT_MAX, by0 = INT_MAX, bx1 = -1, by1 = -1;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -412,13 +461,11 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
             const size_t p = static_cast<size_t>(y) * ch.w + x;
             M v[4] = {M(0), M(0), M(0), M(0)};
             if (ok[r]) {
-                const M omx = M(1) - fx[r], omy = M(1) - fy[r];
-                const M w00 = omx * omy, w10 = fx[r] * omy, w01 = omx * fy[r], w11 = fx[r] * fy[r];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    M val = ((w00 * a[r][k] + w10 * b[r][k]) + w01 * c[r][k]) + w11 * d[r][k];
-                    if (useGain) val = val * static_cast<M>(gain);
-                    v[k] = val < M(0) ? M(0) : (M(1) < val ? M(1) : val);
+                    M vv = val[r][k];
+                    if (useGain) vv = vv * static_cast<M>(gain);
+                    v[k] = vv < M(0) ? M(0) : (M(1) < vv ? M(1) : vv);
                 }
                 v[3] = M(1);
                 bx0 = min(bx0, x); by0 = min(by0, y); bx1 = max(bx1, x); by1 = max(by1, y);
