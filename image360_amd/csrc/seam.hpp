@@ -6,11 +6,12 @@
 //           and the contested band around it (ballots), then the band's
 //           pixel costs |I - J|, written to a per-line cost row;
 //  dp       one block per pair: the line-to-line DP, lane k = candidate
-//           position lo+k; the previous line's candidates are split into
-//           kDpWaves ranges, one wavefront each (readlane broadcasts), and
-//           the partial minima are combined in range order with a strict <,
-//           so the first minimum wins as in the reference's serial scan;
-//           then the backtrack through back-pointers staged in LDS;
+//           position lo+k; the previous line's candidates (at most 49) are
+//           split into kDpWaves ranges, one wavefront each (readlane
+//           broadcasts, a tree minimum), and the partial minima are
+//           combined in range order with a strict <, so the first minimum
+//           wins as in the reference's serial scan; then the backtrack
+//           through back-pointers staged in LDS;
 //  reassign one wavefront per line: I/J-owned pixels inside each contested
 //           band go to the side of the chosen seam.
 // Img supplies rgb(p, M[3]) and the smoothness weight per pixel of seam
@@ -52,6 +53,29 @@ constexpr int kDpWaves = 4;      // wavefronts of the DP kernel (one per SIMD)
 constexpr int kDpThreads = 64 * kDpWaves;
 constexpr int kDpLines = 16;     // cost rows staged in LDS per batch
 constexpr int kBackBatch = 256;
+
+// A wavefront's partial minimum for one lane, value bits and candidate
+// index packed in one vector, written and read as one LDS access (separate
+// scalar loads let the compiler load the index from a selected address,
+// after the compares).
+template <typename M> struct SeamBest;
+template <> struct SeamBest<double> {
+    typedef int T __attribute__((ext_vector_type(4)));
+    __device__ static T pack(double v, int k) {
+        const long long b = __double_as_longlong(v);
+        return T{static_cast<int>(b), static_cast<int>(b >> 32), k, 0};
+    }
+    __device__ static double value(T t) {
+        return __longlong_as_double((static_cast<long long>(t.y) << 32) | static_cast<unsigned int>(t.x));
+    }
+    __device__ static int index(T t) { return t.z; }
+};
+template <> struct SeamBest<float> {
+    typedef int T __attribute__((ext_vector_type(2)));
+    __device__ static T pack(float v, int k) { return T{__float_as_int(v), k}; }
+    __device__ static float value(T t) { return __int_as_float(t.x); }
+    __device__ static int index(T t) { return t.y; }
+};
 
 __device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w, int line, int t) {
     return rowScan ? static_cast<size_t>(oy0 + line) * w + (ox0 + t) : static_cast<size_t>(oy0 + t) * w + (ox0 + line);
@@ -183,6 +207,163 @@ __device__ __forceinline__ bool seam_geom(const int* __restrict__ boxes, int ci,
     return true;
 }
 
+// LDS of the DP kernel.
+template <typename M>
+struct SeamDpShared {
+    static constexpr int kSpan = (2 * kSeamRadius + 1 + kDpWaves - 1) / kDpWaves;
+    M cost[2][kDpLines][64];                               // cost rows of a batch (double buffered)
+    typename SeamBest<M>::T part[2][kDpWaves][64];          // per-wave partial minima of a line
+    __align__(16) uint8_t bk[2][kDpLines][64];              // back-pointer rows of a batch
+    int prv[2][kDpLines];                                   // predecessor line of each line of a batch
+    M dpfinal[64];                                          // dp row of the last contested line
+};
+
+// Phase 2 for wavefront W: the line-to-line DP (SeamBlend.cpp:362-399).
+//
+// A band is at most 2 * kSeamRadius + 1 wide, so the previous line's
+// candidates split into kDpWaves ranges of kSpan; wavefront W evaluates
+// candidates W * kSpan + i for every target lane t = lo + lane, with no
+// branch: a candidate outside the last line's band holds +inf, which never
+// wins a strict <.  The penalty smooth * |t - (lastLo + pk)| is formed from
+// (t - lastLo - W * kSpan) - i, exact in M for these small integers, so
+// each candidate's value is the reference's bit for bit.  The kSpan values
+// are reduced as a tree whose right operand (higher candidates) wins only
+// on a strict <, and the wavefronts' partial minima are combined in
+// wavefront order the same way, so the first minimum wins as in the
+// reference's serial scan.  Wavefront W combines only the lanes of its own
+// candidate range: it needs the new dp values there (readlane broadcasts
+// on the next line) and writes those lanes' back-pointers.
+//
+// Cost rows of kDpLines lines are staged in LDS and their bands held in
+// registers (lane 2l / 2l+1: lo / hi of line l), double buffered: the next
+// batch is loaded while the current one runs.  Back-pointer rows are
+// collected in LDS and written out at the batch's end.
+template <class Img, typename M, int W>
+__device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, const M* __restrict__ costs,
+                                                const int* __restrict__ lohi, int* __restrict__ prevl,
+                                                uint8_t* __restrict__ back, int& last_out, int& lastW_out) {
+    constexpr int kSpan = SeamDpShared<M>::kSpan;
+    constexpr int P0 = W * kSpan;
+    static_assert(kSpan * kDpWaves >= 2 * kSeamRadius + 1 && kSpan * kDpWaves <= 64, "candidate split");
+    constexpr int kPer = kDpLines * 64 / kDpThreads;
+    static_assert(2 * kDpLines <= 64, "bands of a batch fit one wavefront");
+    static_assert(kDpLines * 16 <= kDpThreads, "one dword of back-pointers per thread");
+    using Best = SeamBest<M>;
+    const int lane = threadIdx.x & 63;
+    const bool own = lane >= P0 && lane < P0 + kSpan;  // this wavefront's candidate lanes
+    const M kOut = __builtin_huge_val();  // dp value of a lane outside the band
+    M creg[kPer];
+    int bnext = 0;
+    auto fetch = [&](int b0) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = static_cast<int>(threadIdx.x) + u * kDpThreads;
+            const int l = min(b0 + (idx >> 6), lines - 1);
+            creg[u] = gptr(costs)[static_cast<size_t>(l) * 64 + (idx & 63)];
+        }
+        if (lane < 2 * kDpLines) bnext = gptr(lohi)[2 * min(b0 + (lane >> 1), lines - 1) + (lane & 1)];
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = static_cast<int>(threadIdx.x) + u * kDpThreads;
+            sh.cost[buf][idx >> 6][idx & 63] = creg[u];
+        }
+    };
+    // rows of uncontested lines carry stale bytes: the backtrack never
+    // visits them (it follows the predecessor chain of contested lines)
+    auto flush = [&](int buf, int b0, int nb) {
+        const int tid = static_cast<int>(threadIdx.x);
+        if (tid < nb * 16)
+            reinterpret_cast<I360_GLOBAL uint32_t*>(gptr(back) + static_cast<size_t>(b0) * 64)[tid] =
+                reinterpret_cast<const uint32_t*>(&sh.bk[buf][0][0])[tid];
+        if (tid < nb) gptr(prevl)[b0 + tid] = sh.prv[buf][tid];
+    };
+    fetch(0);
+    commit(0);
+    int bcur = bnext;
+    __syncthreads();
+    int last = -1, lastLo = 0, lastW = 0, ord = 0;
+    M dpv = kOut;  // dp value of the last contested line at this lane (kept on own lanes)
+    // penalties smooth * |t - (lastLo + P0 + i)| of this lane's candidates,
+    // which depend on the line only through the band shift lo - lastLo
+    M pen[kSpan] = {};
+    int pshift = INT_MIN;
+    for (int b0 = 0, bi = 0; b0 < lines; b0 += kDpLines, ++bi) {
+        const int buf = bi & 1;
+        const bool more = b0 + kDpLines < lines;
+        if (more) fetch(b0 + kDpLines);
+        const int nb = min(kDpLines, lines - b0);
+        for (int l = 0; l < nb; ++l) {
+            const int lo = __builtin_amdgcn_readlane(bcur, 2 * l);
+            const int hi = __builtin_amdgcn_readlane(bcur, 2 * l + 1);
+            if (hi < lo) continue;  // block-uniform
+            const int lw = hi - lo + 1;
+            const M c = sh.cost[buf][l][lane];
+            M v = kOut;
+            int bp = -1;
+            if (last < 0) {
+                v = (lane < lw) ? c : kOut;
+            } else {
+                if (lo - lastLo != pshift) {  // uniform; the seam band mostly keeps its offset
+                    pshift = lo - lastLo;
+                    const M d0 = static_cast<M>(pshift + lane - P0);
+#pragma unroll
+                    for (int i = 0; i < kSpan; ++i) pen[i] = Img::smooth() * fabs(d0 - static_cast<M>(i));
+                }
+                M cv[kSpan];
+                int ck[kSpan];
+#pragma unroll
+                for (int i = 0; i < kSpan; ++i) {
+                    cv[i] = lane_value(dpv, P0 + i) + pen[i];
+                    ck[i] = i;
+                }
+#pragma unroll
+                for (int st = 1; st < kSpan; st *= 2)
+#pragma unroll
+                    for (int i = 0; i + st < kSpan; i += 2 * st) {
+                        ck[i] = (cv[i + st] < cv[i]) ? ck[i + st] : ck[i];
+                        cv[i] = __builtin_fmin(cv[i], cv[i + st]);  // no NaN; a tie keeps the same bits
+                    }
+                const int pb = ord & 1;
+                ++ord;
+                sh.part[pb][W][lane] = Best::pack(cv[0], P0 + ck[0]);
+                lds_barrier();
+                if (own) {
+                    typename Best::T r[kDpWaves];  // all read before any is used
+#pragma unroll
+                    for (int q = 0; q < kDpWaves; ++q) r[q] = sh.part[pb][q][lane];
+                    M best = Best::value(r[0]);
+                    bp = Best::index(r[0]);
+#pragma unroll
+                    for (int q = 1; q < kDpWaves; ++q) {
+                        const M bq = Best::value(r[q]);
+                        const bool lt = bq < best;
+                        bp = lt ? Best::index(r[q]) : bp;
+                        best = lt ? bq : best;
+                    }
+                    v = (lane < lw) ? c + best : kOut;
+                }
+            }
+            if (own) {
+                sh.bk[buf][l][lane] = static_cast<uint8_t>(bp & 0xFF);
+                dpv = v;
+            }
+            if (W == 0 && lane == 0) sh.prv[buf][l] = last;
+            last = b0 + l;
+            lastLo = lo;
+            lastW = lw;
+        }
+        if (more) commit(buf ^ 1);
+        bcur = bnext;
+        __syncthreads();
+        flush(buf, b0, nb);
+    }
+    if (own) sh.dpfinal[lane] = dpv;
+    last_out = last;
+    lastW_out = lastW;
+}
+
 // DP and backtrack of one pair; blockDim.x == kDpThreads.  Writes the
 // chosen seam position of every contested line (SeamBlend.cpp:362-407).
 template <class Img, typename M>
@@ -193,104 +374,23 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
     SeamGeom g;
     if (!seam_geom(boxes, ci, cj, g)) return;
     const int lines = g.lines;
-    // wave-uniform values are made scalar (readfirstlane) so that the
-    // candidate loop below is a scalar loop with scalar lane selects
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     // none contested: no DP and no reassignment (SeamBlend.cpp:409)
     if (*gptr(contested) == 0) return;
-
-    // Phase 2: DP over contested lines.  Cost rows and bands of kDpLines
-    // lines are staged in LDS (double buffered: the next batch is loaded
-    // into registers while the current one runs).
-    constexpr int kSpan = 64 / kDpWaves;
-    constexpr int kPer = kDpLines * 64 / kDpThreads;
-    __shared__ M scost[2][kDpLines][64];
-    __shared__ int sband[2][kDpLines][2];
-    __shared__ M pbest[2][kDpWaves][64];
-    __shared__ int pbk[2][kDpWaves][64];
-    __shared__ M s_dpfinal[64];
-    M creg[kPer];
-    int breg = 0;
-    auto fetch = [&](int b0) {
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int idx = static_cast<int>(threadIdx.x) + u * kDpThreads;
-            const int l = min(b0 + (idx >> 6), lines - 1);
-            creg[u] = gptr(costs)[static_cast<size_t>(l) * 64 + (idx & 63)];
-        }
-        const int tid = static_cast<int>(threadIdx.x);
-        if (tid < 2 * kDpLines) breg = gptr(lohi)[2 * min(b0 + (tid >> 1), lines - 1) + (tid & 1)];
-    };
-    auto commit = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int idx = static_cast<int>(threadIdx.x) + u * kDpThreads;
-            scost[buf][idx >> 6][idx & 63] = creg[u];
-        }
-        if (threadIdx.x < 2 * kDpLines) sband[buf][threadIdx.x >> 1][threadIdx.x & 1] = breg;
-    };
-    fetch(0);
-    commit(0);
-    __syncthreads();
-    int last = -1, lastLo = 0, lastW = 0, ord = 0;
-    M dpv = M(0);  // this lane's dp value on the last contested line (every wave holds the whole row)
-    for (int b0 = 0, bi = 0; b0 < lines; b0 += kDpLines, ++bi) {
-        const int buf = bi & 1;
-        const bool more = b0 + kDpLines < lines;
-        if (more) fetch(b0 + kDpLines);
-        const int nb = min(kDpLines, lines - b0);
-        for (int l = 0; l < nb; ++l) {
-            const int lo = __builtin_amdgcn_readfirstlane(sband[buf][l][0]);
-            const int hi = __builtin_amdgcn_readfirstlane(sband[buf][l][1]);
-            if (hi < lo) continue;  // block-uniform
-            const int lw = hi - lo + 1;
-            const int line = b0 + l;
-            M v = M(0);
-            int bp = -1;
-            if (last < 0) {
-                if (lane < lw) v = scost[buf][l][lane];
-            } else {
-                const int t = lo + lane;
-                M best = SeamMax<M>::value;
-                const int p0 = wave * kSpan;
-#pragma unroll
-                for (int i = 0; i < kSpan; ++i) {
-                    const int pk = p0 + i;
-                    if (pk < lastW) {  // uniform
-                        const M dprev = lane_value(dpv, pk);
-                        const M cand = dprev + Img::smooth() * fabs(static_cast<M>(t - (lastLo + pk)));
-                        if (cand < best) { best = cand; bp = pk; }
-                    }
-                }
-                const int pb = ord & 1;
-                ++ord;
-                pbest[pb][wave][lane] = best;
-                pbk[pb][wave][lane] = bp;
-                lds_barrier();
-                best = pbest[pb][0][lane];
-                bp = pbk[pb][0][lane];
-#pragma unroll
-                for (int q = 1; q < kDpWaves; ++q) {
-                    const M bq = pbest[pb][q][lane];
-                    if (bq < best) { best = bq; bp = pbk[pb][q][lane]; }
-                }
-                if (lane < lw) v = scost[buf][l][lane] + best;
-            }
-            if (wave == 0) {
-                if (lane < lw) gptr(back)[static_cast<size_t>(line) * 64 + lane] = static_cast<uint8_t>(bp & 0xFF);
-                if (lane == 0) gptr(prevl)[line] = last;
-            }
-            dpv = v;
-            last = line;
-            lastLo = lo;
-            lastW = lw;
-        }
-        if (more) commit(buf ^ 1);
-        __syncthreads();
+    __shared__ SeamDpShared<M> sh;
+    static_assert(kDpWaves == 4, "one forward instance per wavefront");
+    int last = -1, lastW = 0;
+    // each wavefront runs its own instance (candidate lanes as constants);
+    // all execute the same barriers in the same order
+    switch (wave) {
+        case 0: seam_dp_forward<Img, M, 0>(sh, lines, costs, lohi, prevl, back, last, lastW); break;
+        case 1: seam_dp_forward<Img, M, 1>(sh, lines, costs, lohi, prevl, back, last, lastW); break;
+        case 2: seam_dp_forward<Img, M, 2>(sh, lines, costs, lohi, prevl, back, last, lastW); break;
+        default: seam_dp_forward<Img, M, 3>(sh, lines, costs, lohi, prevl, back, last, lastW); break;
     }
     if (last < 0) return;  // no contested pixel in this overlap (block-uniform)
-    if (wave == 0) s_dpfinal[lane] = dpv;
     __syncthreads();
+    M* s_dpfinal = sh.dpfinal;
 
     // Phase 3: backtrack through back-pointers staged in LDS, from the end.
     __shared__ uint8_t sback[kBackBatch][64];

@@ -96,19 +96,67 @@ def _line_min(dprev, last_lo, lo, smooth, dt, ranges):
     return out
 
 
+SPAN, WAVES = 13, 4  # candidates per wavefront of the DP kernel, wavefronts
+
+
+def _line_min_kernel(dprev, last_lo, lo, smooth, dt, combine_le=False):
+    """The DP kernel's candidate scan, step for step: the previous line's
+    row held in 64 lanes with +inf outside its band, WAVES ranges of SPAN
+    candidates each evaluated with no guard, the penalty formed from the
+    band shift (lo - last_lo + lane - p0) - i, each range reduced as a tree
+    (node i absorbs node i + st; the right node's index on a strict <, the
+    value by fmin), then the ranges combined in order with a strict <
+    (combine_le plants a <= there)."""
+    row = np.full(64, np.inf, dtype=dt)
+    row[: len(dprev)] = dprev
+    out = []
+    for lane in range(64):
+        best, bp = None, -1
+        for w in range(WAVES):
+            p0 = w * SPAN
+            d0 = dt(lo - last_lo + lane - p0)
+            cv = [dt(row[p0 + i] + dt(dt(smooth) * abs(dt(d0 - dt(i))))) for i in range(SPAN)]
+            ck = list(range(SPAN))
+            st = 1
+            while st < SPAN:
+                for i in range(0, SPAN - st, 2 * st):
+                    ck[i] = ck[i + st] if cv[i + st] < cv[i] else ck[i]
+                    cv[i] = np.fmin(cv[i], cv[i + st])
+                st *= 2
+            if best is None or cv[0] < best or (combine_le and cv[0] == best):
+                best, bp = cv[0], p0 + ck[0]
+        out.append((best, bp))
+    return out
+
+
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("seed", range(6))
 def test_split_candidate_scan_keeps_first_minimum(seed, dt):
-    """Splitting a line's candidates into 4 ranges of 16 and combining the
-    partial minima in range order selects the same value and the same
-    (first) back-pointer as the reference's single serial scan, ties
-    included (integer costs make ties frequent)."""
+    """Splitting a line's candidates into 4 ranges of 13 (a band is at most
+    49 wide) and combining the partial minima in range order selects the
+    same value and the same (first) back-pointer as the reference's single
+    serial scan, ties included (integer costs make ties frequent)."""
+    assert SPAN * WAVES >= 2 * 24 + 1
     rng = np.random.default_rng(seed)
-    for _ in range(20):
-        last_w = int(rng.integers(1, 50))
+    for k in range(40):
+        last_w = 49 if k < 4 else int(rng.integers(1, 50))
         dprev = rng.integers(0, 6, size=last_w).astype(dt) * dt(2)
         last_lo, lo = int(rng.integers(0, 40)), int(rng.integers(0, 40))
+        if k % 3 == 0:
+            lo = last_lo  # the common case: the band keeps its offset
         smooth = 2.0 if seed % 2 else 2.0 / 65535.0
         serial = _line_min(dprev, last_lo, lo, smooth, dt, [(0, 64)])
-        split = _line_min(dprev, last_lo, lo, smooth, dt, [(0, 16), (16, 32), (32, 48), (48, 64)])
-        assert serial == split
+        split = _line_min_kernel(dprev, last_lo, lo, smooth, dt)
+        assert serial == split, (k, last_w, last_lo, lo)
+
+
+def test_kernel_scan_emulation_catches_a_wrong_tie_rule():
+    """The emulation is sharp: with a <= planted in the range combine, a row
+    of tied candidates takes its back-pointer from the last range instead of
+    candidate 0."""
+    dt = np.float64
+    dprev = np.full(40, 4.0, dtype=dt)  # every candidate ties
+    serial = _line_min(dprev, 0, 0, 0.0, dt, [(0, 64)])
+    assert all(bp == 0 for _, bp in serial)
+    assert _line_min_kernel(dprev, 0, 0, 0.0, dt) == serial
+    assert _line_min_kernel(dprev, 0, 0, 0.0, dt, combine_le=True) != serial
