@@ -49,6 +49,8 @@ def parse():
                     help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--forms", default="plain,precision,batched,surrogate")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch configuration")
+    ap.add_argument("--profile-forms", default="batched",
+                    help="forms whose per-kernel event timing is printed (the roofline uses the batched form's)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the burst median+demosaic and uneven-chunk measurements")
     ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "image360_amd", "surrogate_weights.npz"))
@@ -340,13 +342,26 @@ def main():
         if name in ("batched", "surrogate"):
             outputs[name] = out.clone()
 
-    # per-kernel event timing of one eager step of the batched launch sequence
-    hip.profile_enable(True)
-    with torch.cuda.stream(stream):
-        render(_abi.RENDER_BATCHED)
-    torch.cuda.synchronize()
-    stats = hip.profile_read()
-    hip.profile_enable(False)
+    # per-kernel event timing of one eager step of a form's launch sequence
+    def stage_profile(mode):
+        hip.profile_enable(True)
+        with torch.cuda.stream(stream):
+            render(mode)
+        torch.cuda.synchronize()
+        st_ = hip.profile_read()
+        hip.profile_enable(False)
+        return st_
+
+    for pname in [f for f in args.profile_forms.split(",") if f and f != "batched"]:
+        if pname == "surrogate" and surr_src is None:
+            continue
+        pst = stage_profile({"plain": _abi.RENDER_PLAIN, "precision": _abi.RENDER_PRECISION,
+                             "surrogate": _abi.RENDER_SURROGATE}[pname])
+        ptot = sum(v["total_ms"] for v in pst.values())
+        log(f"stage profile of the {pname} form:")
+        for name, v in sorted(pst.items(), key=lambda kv: -kv[1]["total_ms"]):
+            log(f"  {name:18s} {v['launches']:5d} launches {v['total_ms']:9.2f} ms ({100 * v['total_ms'] / ptot:5.1f} %)")
+    stats = stage_profile(_abi.RENDER_BATCHED)
     tot = sum(v["total_ms"] for v in stats.values())
     for name, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"]):
         gbs = v["alg_bytes"] / (v["total_ms"] / 1e3) / 1e9 if v["total_ms"] > 0 else 0

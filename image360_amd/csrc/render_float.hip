@@ -1271,7 +1271,7 @@ struct FinalItem {
     const void* nd1;
     int box[4];
 };
-constexpr int kFinSlots = 4;
+constexpr int kFinSlots = 3;  // (3 slots of RGB keep the block's LDS at ~22 KB: 6 blocks per CU)
 constexpr int kFinW1 = 36, kFinH1 = 12;  // staged region bound (34 x 10 needed)
 
 template <int MODE>
@@ -1284,8 +1284,8 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
     constexpr int K = 4;
     constexpr int kRowStep = kWarpTH / K;
     __shared__ FinalItem sit[I360HIP_MAX_CONTRIBUTORS];
-    __shared__ M sR[kFinH1 * kFinW1][4];
-    __shared__ M sG[kFinSlots][kFinH1 * kFinW1][4];
+    __shared__ M sR[kFinH1 * kFinW1][3];
+    __shared__ M sG[kFinSlots][kFinH1 * kFinW1][3];
     __shared__ uint32_t smask;
     const int tid = static_cast<int>(threadIdx.x);
     const int lx = tid & 63, ry = tid >> 6;
