@@ -298,6 +298,8 @@ def main():
     def render(mode, descs=cdesc, cont=contrib):
         hip.check(hip.render_chunks(fdesc, descs, cont, args.bands, mode, sh), f"render mode {mode}")
 
+    host_ms = {}
+
     def timed(mode, steps, warmup):
         with torch.cuda.stream(stream):
             for _ in range(warmup):
@@ -305,8 +307,10 @@ def main():
             torch.cuda.synchronize()
             ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             ev0.record(stream)
+            h0 = time.perf_counter()
             for _ in range(steps):
                 render(mode)
+            host_ms[mode] = (time.perf_counter() - h0) * 1e3 / steps  # host time per call (graph replays are asynchronous)
             ev1.record(stream)
             torch.cuda.synchronize()
         return ev0.elapsed_time(ev1)
@@ -338,7 +342,8 @@ def main():
             tiles, fb = hip.surrogate_stats()
             forms[name].update({"tiles": tiles, "fallback_tiles": fb, "weights": surr_src})
             log(f"surrogate : {fb}/{tiles} level-K tiles fell back to the plain blend")
-        log(f"{name:10s}: {ms / steps:9.2f} ms/step  {forms[name]['mp_s']:9.1f} MP/s")
+        log(f"{name:10s}: {ms / steps:9.2f} ms/step  {forms[name]['mp_s']:9.1f} MP/s"
+            f"  (host {host_ms.get(mode, 0.0):.2f} ms per call)")
         if name in ("batched", "surrogate"):
             outputs[name] = out.clone()
 

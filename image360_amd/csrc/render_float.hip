@@ -2097,6 +2097,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
 
 struct RenderCache {
     uint64_t key = 0;
+    uint64_t call = 0;  // call_key of the call that captured the graph
     int mode = -1;
     Plan plan;
     DevTables tables;
@@ -2257,6 +2258,44 @@ static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, 
         }
 }
 
+// Hash of everything a render call's plan depends on, read straight from
+// the call's arguments: frames (pixels, size, format, gain, homography),
+// chunks (rectangle, contributor range, outputs), the contributor lists
+// (as far as the planner reads them), bands, form and launch tuning.  A
+// repeated call with an equal hash replays its captured graph without
+// planning again (planning a gigapixel canvas costs ~20 ms of host time).
+static uint64_t call_key(const i360hip_ctx* ctx, const i360hip_frame* frames, int nf, const i360hip_chunk* chunks,
+                         int nc, const int* contrib, int bands, int api_mode) {
+    uint64_t k = mix(0x1a2b3c4d5e6f7788ull, static_cast<uint64_t>(api_mode));
+    k = mix(k, static_cast<uint64_t>(bands));
+    k = mix(k, static_cast<uint64_t>(nf) * 1000003ull + static_cast<uint64_t>(nc));
+    k = mix(k, static_cast<uint64_t>(ctx->tune_groups * 1000003 + ctx->tune_strip * 101 + ctx->tune_share));
+    auto bits = [](auto v) {
+        uint64_t b = 0;
+        std::memcpy(&b, &v, sizeof(v));
+        return b;
+    };
+    for (int f = 0; f < nf; ++f) {
+        const i360hip_frame& fr = frames[f];
+        k = mix(k, reinterpret_cast<uintptr_t>(fr.pixels));
+        k = mix(k, static_cast<uint64_t>(fr.width) * 65537ull + static_cast<uint64_t>(fr.height) * 3ull + fr.format);
+        k = mix(k, bits(fr.gain));
+        for (int q = 0; q < 9; ++q) k = mix(k, bits(fr.h[q]));
+    }
+    for (int c = 0; c < nc; ++c) {
+        const i360hip_chunk& cc = chunks[c];
+        k = mix(k, static_cast<uint64_t>(cc.x) * 1000003ull + static_cast<uint64_t>(cc.y));
+        k = mix(k, static_cast<uint64_t>(cc.width) * 1000003ull + static_cast<uint64_t>(cc.height));
+        k = mix(k, static_cast<uint64_t>(cc.first) * 1000003ull + static_cast<uint64_t>(cc.count));
+        k = mix(k, reinterpret_cast<uintptr_t>(cc.out));
+        k = mix(k, reinterpret_cast<uintptr_t>(cc.owner));
+        if (cc.first < 0 || cc.count < 0) continue;  // the planner rejects it before reading
+        const int cap = std::min(cc.count, I360HIP_MAX_CONTRIBUTORS);
+        for (int q = 0; q < cap; ++q) k = mix(k, static_cast<uint64_t>(static_cast<uint32_t>(contrib[cc.first + q])));
+    }
+    return k;
+}
+
 static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc,
                        const int* contrib, int bands, int api_mode, hipStream_t s, Status& st,
                        float* export_feat = nullptr, float* export_target = nullptr) {
@@ -2267,6 +2306,12 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
         st.set(I360HIP_INVALID_PARAM, "the SURROGATE form needs parameters (i360hip_surrogate_set)");
         return;
     }
+    const bool graphable = mode == kModeBatched && surr != kSurrExport;
+    const uint64_t ck = graphable ? call_key(ctx, frames, nf, chunks, nc, contrib, bands, api_mode) : 0;
+    if (graphable && !ctx->profiling && rc.graph && rc.mode == api_mode && rc.call == ck) {
+        if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
+        return;
+    }
     Plan P;
     if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st, surr)) return;
     SurrRun sr;
@@ -2274,10 +2319,10 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     sr.weights = ctx->surr_weights;
     sr.biases = ctx->surr_biases;
     sr.threshold = ctx->surr_threshold;
-    const bool graphable = mode == kModeBatched && surr != kSurrExport;
     P.key = mix(P.key, static_cast<uint64_t>(ctx->tune_groups * 1000003 + ctx->tune_strip * 101 + ctx->tune_share));
     const bool reuse = !ctx->profiling && rc.graph && rc.key == P.key && rc.mode == api_mode && graphable;
-    if (reuse) {
+    if (reuse) {  // other arguments, the same plan
+        rc.call = ck;
         if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
         return;
     }
@@ -2352,6 +2397,7 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     }
     (void)hipGraphDestroy(g);
     rc.graph = ge;
+    rc.call = ck;
     if (hipGraphLaunch(ge, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
 }
 

@@ -197,3 +197,26 @@ def test_autotune_and_launch_configurations_agree(hip):
         for k in range(len(g.chunks)):
             assert np.array_equal(b[k], ref[k]), (cfg, k)
             assert np.array_equal(bo[k], ref_own[k]), (cfg, k)
+
+
+def test_batched_replay_follows_changed_arguments(hip, case):
+    # a repeated batched call replays its graph; any changed argument
+    # (outputs, a gain) renders afresh and matches the precision form
+    g, frames, dev = case
+    note("render batched: changed outputs and gains")
+    ref, ref_own = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION)
+    a, _ = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED)
+    b, _ = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED)  # new output buffers
+    for k in range(len(g.chunks)):
+        assert np.array_equal(a[k], ref[k]) and np.array_equal(b[k], ref[k]), k
+    keep = g.gains.copy()
+    try:
+        g.gains[0] = 1.25
+        want, _ = _gpu_render(hip, g, dev, _abi.RENDER_PRECISION)
+        for rep in range(2):
+            got, _ = _gpu_render(hip, g, dev, _abi.RENDER_BATCHED)
+            for k in range(len(g.chunks)):
+                assert np.array_equal(got[k], want[k]), (rep, k)
+        assert any(not np.array_equal(want[k], ref[k]) for k in range(len(g.chunks)))
+    finally:
+        g.gains[:] = keep
