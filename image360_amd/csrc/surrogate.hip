@@ -2,15 +2,17 @@
 // 6-layer 3x3 convolutional network declared in surrogate.hpp, and the C
 // ABI entry points that train and run it.
 //
-// Convolution: one block per 16x16 output tile, one thread per output
-// pixel holding all output channels in registers.  The input tile (18x18
-// with halo, zero outside the image) is staged in LDS; the weights are
-// block-uniform, so they are read through the scalar cache as BF16 pairs
-// and widened to FP32 (a 16-bit shift) before each FMA.  The backward data
-// pass is the same kernel on the ReLU-masked output gradient with the
-// weights flipped and transposed; the weight gradient is a per-tile partial
-// sum (LDS-staged input and output-gradient tiles) reduced across tiles in
-// a fixed order, so training is deterministic.
+// Convolution: an implicit GEMM on the matrix cores, one block per 16x16
+// output tile.  M = the tile's pixels, N = output channels, K = (tap,
+// input channel); FP32 MFMA (v_mfma_f32_16x16x4f32), so activations and
+// products stay FP32.  The input tile (18x18 with halo, zero outside the
+// image) and the BF16 weights (widened to FP32 by a 16-bit shift) are
+// staged in LDS; each wavefront accumulates four output rows (four 16-pixel
+// M blocks) over all output channels.  The backward data pass is the same
+// kernel on the ReLU-masked output gradient with the weights flipped and
+// transposed; the weight gradient is a per-tile partial sum (LDS-staged
+// input and output-gradient tiles) reduced across tiles in a fixed order,
+// so training is deterministic.
 
 #include <cmath>
 #include <vector>
@@ -71,52 +73,97 @@ __device__ __forceinline__ void stage_tile(float* lds, const float* __restrict__
 //   RELU: max(0, .) on the output;  BIAS: add the bias.
 //   mask: when non-null the input is multiplied by (mask > 0) (backward
 //   data through a ReLU layer whose output is `mask`).
+// MFMA 16x16x4 (FP32) lane layout: A[i][k] in lane i + 16 k, B[k][j] in lane
+// j + 16 k, D[i][j] in lane j + 16 (i / 4), element i % 4.  Here i is a
+// pixel of one output row, j an output channel of a 16-channel block and k
+// one of four consecutive input channels of one tap.
+typedef float surr_f4 __attribute__((ext_vector_type(4)));
 template <int CIN, int COUT, bool BIAS, bool RELU>
 __global__ void __launch_bounds__(256) surr_conv_kernel(const SImg* __restrict__ imgs, const STile* __restrict__ tiles,
                                                         const float* __restrict__ in, const float* __restrict__ mask,
                                                         float* __restrict__ out, const uint32_t* __restrict__ wpk,
                                                         const float* __restrict__ bias) {
-    __shared__ float lds[kTH * kTH * CIN];
+    static_assert(CIN % 4 == 0, "four input channels per MFMA step");
+    constexpr int CP = CIN + 2;             // LDS channel pitch: the 16 pixels of a read spread over the banks
+    constexpr int NB = (COUT + 15) / 16;    // 16-channel output blocks
+    constexpr int NP = NB * 16;             // output channels, padded
+    __shared__ float xs[kTH * kTH * CP];
+    __shared__ uint16_t ws[9 * CIN * NP];   // BF16 weights [tap * CIN + ci][co], zero for co >= COUT
     const STile tl = tiles[blockIdx.x];
     const SImg im = imgs[tl.img];
     const int x0 = tl.tx * kT, y0 = tl.ty * kT;
-    stage_tile<CIN>(lds, in, mask, im, x0, y0, kT);
+    // input rows [y0-1, y0+17) x cols [x0-1, x0+17); 0 outside the image and the mask
+    for (int i = threadIdx.x; i < kTH * kTH * (CIN / 4); i += blockDim.x) {
+        const int c4 = i % (CIN / 4), pc = i / (CIN / 4);
+        const int col = pc % kTH, row = pc / kTH;
+        const int x = x0 - 1 + col, y = y0 - 1 + row;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (x >= 0 && y >= 0 && x < im.w && y < im.h) {
+            const size_t p = static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x;
+            v = *reinterpret_cast<const float4*>(in + p * CIN + 4 * c4);
+            if (mask) {
+                const float4 m = *reinterpret_cast<const float4*>(mask + p * CIN + 4 * c4);
+                v.x = m.x > 0.f ? v.x : 0.f;
+                v.y = m.y > 0.f ? v.y : 0.f;
+                v.z = m.z > 0.f ? v.z : 0.f;
+                v.w = m.w > 0.f ? v.w : 0.f;
+            }
+        }
+        float* d = xs + pc * CP + 4 * c4;  // 8-byte aligned (CP even)
+        *reinterpret_cast<float2*>(d) = make_float2(v.x, v.y);
+        *reinterpret_cast<float2*>(d + 2) = make_float2(v.z, v.w);
+    }
+    // weights: BF16 pairs wpk[(tap * CIN + ci) * COUT / 2 + co / 2], low half = even co
+    const auto* w16 = reinterpret_cast<const uint16_t*>(wpk);
+    for (int i = threadIdx.x; i < 9 * CIN * NP; i += blockDim.x) {
+        const int kk = i / NP, n = i % NP;
+        ws[i] = n < COUT ? w16[kk * COUT + n] : uint16_t(0);
+    }
     __syncthreads();
-    const int tx = threadIdx.x & (kT - 1), ty = threadIdx.x / kT;
-    float acc[COUT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    surr_f4 acc[4][NB];
 #pragma unroll
-    for (int co = 0; co < COUT; ++co) acc[co] = BIAS ? bias[co] : 0.0f;
-    // weights as BF16 pairs: wpk[(k*CIN + ci)*COUT/2 + co/2]
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[m][nb] = surr_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
-    for (int k = 0; k < 9; ++k) {
-        const float* xin = lds + ((ty + k / 3) * kTH + (tx + k % 3)) * CIN;
-#pragma unroll 2
-        for (int ci = 0; ci < CIN; ci += 4) {
-            const float4 x4 = *reinterpret_cast<const float4*>(xin + ci);
-            const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int ky = tap / 3, kx = tap % 3;
+        // this lane's A column: pixel li of row (4 wave + m), tap (ky, kx)
+        const float* xa = xs + ((4 * wave + ky) * kTH + li + kx) * CP + lk;
+        const uint16_t* wb = ws + (tap * CIN + lk) * NP + li;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t* wr = wpk + ((k * CIN + ci + j) * COUT) / 2;
+        for (int c0 = 0; c0 < CIN; c0 += 4) {
+            float bv[NB];
 #pragma unroll
-                for (int c2 = 0; c2 < COUT / 2; ++c2) {
-                    const uint32_t u = wr[c2];
-                    acc[2 * c2] = fmaf(bf16_lo(u), xs[j], acc[2 * c2]);
-                    acc[2 * c2 + 1] = fmaf(bf16_hi(u), xs[j], acc[2 * c2 + 1]);
-                }
+            for (int nb = 0; nb < NB; ++nb) bv[nb] = __uint_as_float(static_cast<uint32_t>(wb[c0 * NP + nb * 16]) << 16);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const float av = xa[m * kTH * CP + c0];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nb], acc[m][nb], 0, 0, 0);
             }
         }
     }
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= im.w || y >= im.h) return;
-    const size_t p = static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x;
-    float* o = out + p * COUT;
 #pragma unroll
-    for (int c4 = 0; c4 < COUT / 4; ++c4) {
-        float4 v = make_float4(acc[4 * c4], acc[4 * c4 + 1], acc[4 * c4 + 2], acc[4 * c4 + 3]);
-        if (RELU) {
-            v.x = fmaxf(v.x, 0.f); v.y = fmaxf(v.y, 0.f); v.z = fmaxf(v.z, 0.f); v.w = fmaxf(v.w, 0.f);
+    for (int m = 0; m < 4; ++m) {
+        const int y = y0 + 4 * wave + m;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int co = nb * 16 + li;
+            if (co >= COUT || y >= im.h) continue;
+            const float bc = BIAS ? bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + 4 * lk + r;
+                if (x >= im.w) continue;
+                float v = acc[m][nb][r] + bc;
+                if (RELU) v = fmaxf(v, 0.f);
+                out[(static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x) * COUT + co] = v;
+            }
         }
-        *reinterpret_cast<float4*>(o + 4 * c4) = v;
     }
 }
 
