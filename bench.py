@@ -120,6 +120,24 @@ def cpu_baseline(geom, frames_host_fn, chunks, threads, bands):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_gigapixel_batched_hbm_traffic.json")
 
 
+def cpu_export(geom, chunks, rgb48, offs, log, max_dim=4096):
+    """The export composite of the oracle (a port of PanoramaExporter.cpp:53-98,
+    one thread) over the same RGB48 blobs, at the application's preview
+    size; the blobs are copied to the host first (not timed)."""
+    from oracle import pyoracle as po
+
+    host = rgb48.cpu().numpy().view(np.uint16)
+    bufs = [(c.x, c.y, c.w, c.h, host[int(offs[k]): int(offs[k + 1])].reshape(c.h, c.w, 3))
+            for k, c in enumerate(chunks)]
+    t0 = time.perf_counter()
+    img = po.export_composite(bufs, geom.canvas_w, geom.canvas_h, max_dim)
+    dt = time.perf_counter() - t0
+    r = {"max_dim": max_dim, "out": [img.shape[1], img.shape[0]], "ms": round(dt * 1e3, 1),
+         "mp_s": round(img.shape[0] * img.shape[1] / dt / 1e6, 1), "cores": 1, "kind": "port"}
+    log(f"CPU export: {r}")
+    return r
+
+
 def measured_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated against a
@@ -246,6 +264,51 @@ def load_surrogate(hip, args, log):
     torch.cuda.synchronize()
     hip.check(hip.surrogate_set(wt.data_ptr(), bt.data_ptr(), args.surrogate_threshold), "surrogate_set")
     return src
+
+
+def bench_export(hip, torch, geom, chunks, out, offs, log, max_dims=(8192, 4096)):
+    """Preview export of the rendered canvas (PanoramaExporter.cpp:53-98):
+    the FP32 render output rounded to RGB48 blobs (SeamBlend.cpp:529-534),
+    then the nearest-neighbour RGB8 composite at the preview sizes, from
+    the RGB48 blobs and straight from the FP32 output.  Returns the
+    results and the RGB48 buffer (kept for the CPU leg)."""
+    from image360_amd import _abi
+
+    n = out.numel()
+    rgb48 = torch.empty(n, dtype=torch.int16, device="cuda")
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn, reps=3):
+        with torch.cuda.stream(stream):
+            fn()
+            torch.cuda.synchronize()
+            ev[0].record(stream)
+            for _ in range(reps):
+                fn()
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps
+
+    t48 = timed(lambda: hip.check(hip.rgb48_from_render_dev(out.data_ptr(), n, rgb48.data_ptr(), sh), "rgb48"))
+    res = {"rgb48_from_render": {"values": n, "ms": round(t48, 3), "gb_s": round(n * 6 / (t48 / 1e3) / 1e9, 1)}}
+    for md in max_dims:
+        ow, oh = hip.export_size(geom.canvas_w, geom.canvas_h, md)
+        comp = torch.empty(ow * oh * 3, dtype=torch.uint8, device="cuda")
+        r = {"out": [ow, oh]}
+        for fmt, base, esz in ((_abi.PIX_U16, rgb48, 2), (_abi.PIX_F32, out, 4)):
+            args_ = [(c.x, c.y, c.w, c.h, base.data_ptr() + esz * int(offs[k]), c.w, c.h)
+                     for k, c in enumerate(chunks)]
+            t = timed(lambda: hip.check(hip.export_composite_dev(args_, geom.canvas_w, geom.canvas_h, md, fmt,
+                                                                 comp.data_ptr(), sh), "export"))
+            alg = ow * oh * (3 * esz + 3)
+            r["from_rgb48" if fmt == _abi.PIX_U16 else "from_fp32"] = {
+                "ms": round(t, 3), "mp_s": round(ow * oh / (t / 1e3) / 1e6, 1),
+                "gb_s_alg": round(alg / (t / 1e3) / 1e9, 1)}
+        res[f"max_dim_{md}"] = r
+    log(f"export: {res}")
+    return res, rgb48
 
 
 def main():
@@ -417,9 +480,12 @@ def main():
     psnr, maxerr = head.get("psnr_db_vs_fp64"), head.get("max_abs_err_vs_fp64")
 
     extra = None
+    rgb48 = None
     if not args.no_extra:
+        exp, rgb48 = bench_export(hip, torch, geom, chunks, out, offs, log)
         extra = {"burst_median_demosaic": bench_burst(hip, torch, log),
-                 "uneven_chunks": bench_uneven(hip, torch, geom, fdesc, args, log)}
+                 "uneven_chunks": bench_uneven(hip, torch, geom, fdesc, args, log),
+                 "preview_export": exp}
 
     cpu = None
     if not args.no_cpu:
@@ -435,6 +501,8 @@ def main():
         log(f"CPU baseline: {len(csample)} chunks on {threads} threads")
         cpu = cpu_baseline(geom, host_frame, csample, threads, args.bands)
         log(f"CPU baseline: {cpu['value']:.2f} MP/s")
+        if rgb48 is not None:
+            cpu["preview_export"] = cpu_export(geom, chunks, rgb48, offs, log)
 
     line = {
         "metric": METRIC, "value": head["mp_s"], "unit": "MP/s", "n_gpus": 1, "steps": args.steps,

@@ -51,7 +51,12 @@ class ChunkDesc(C.Structure):
                 ("count", C.c_int), ("out", C.c_void_p), ("owner", C.c_void_p)]
 
 
-PIX_F16, PIX_F32 = 0, 1
+class ExportChunk(C.Structure):
+    _fields_ = [("x_offset", C.c_int), ("y_offset", C.c_int), ("width", C.c_int), ("height", C.c_int),
+                ("pixels", C.c_void_p), ("buf_width", C.c_int), ("buf_height", C.c_int)]
+
+
+PIX_F16, PIX_F32, PIX_U16 = 0, 1, 2
 RENDER_FP64, RENDER_PLAIN, RENDER_PRECISION, RENDER_BATCHED, RENDER_SURROGATE = 0, 1, 2, 3, 4
 RENDER_NAMES = {0: "fp64", 1: "plain", 2: "precision", 3: "batched", 4: "surrogate"}
 
@@ -97,6 +102,11 @@ SIGNATURES = {
     "i360hip_render_autotune": (i32, [vp, C.POINTER(Frame), i32, C.POINTER(ChunkDesc), i32, C.POINTER(i32), i32,
                                       C.POINTER(Tuning)]),
     "i360hip_render_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
+    "i360hip_export_scale": (C.c_float, [i32, i32, i32]),
+    "i360hip_export_placement": (None, [i32, i32, i32, i32, C.c_float, i32p]),
+    "i360hip_export_size": (i32, [i32, i32, i32, i32p, i32p]),
+    "i360hip_export_composite_dev": (i32, [vp, C.POINTER(ExportChunk), i32, i32, i32, i32, i32, vp, vp]),
+    "i360hip_rgb48_from_render_dev": (i32, [vp, vp, sz, vp, vp]),
 }
 
 

@@ -283,6 +283,41 @@ class HipComputeBackend:
         self.check(lib().i360hip_surrogate_stats(self._ctx, C.byref(t), C.byref(f)), "surrogate_stats")
         return t.value, f.value
 
+    # ---- preview export (PanoramaExporter.cpp:9-24, 34-98) ----
+    @staticmethod
+    def export_scale(total_w, total_h, max_dim) -> float:
+        """ComputeExportScale (PanoramaExporter.cpp:9-14)."""
+        return float(lib().i360hip_export_scale(total_w, total_h, max_dim))
+
+    @staticmethod
+    def export_placement(x_offset, y_offset, width, height, scale):
+        """ComputeChunkPlacement (PanoramaExporter.cpp:16-24): (destX, destY, destW, destH)."""
+        out = (C.c_int32 * 4)()
+        lib().i360hip_export_placement(x_offset, y_offset, width, height, scale, out)
+        return tuple(out)
+
+    @staticmethod
+    def export_size(total_w, total_h, max_dim):
+        """(out_w, out_h) of the composite, or None where ExportPreviewJpeg returns false."""
+        w, h = C.c_int32(0), C.c_int32(0)
+        if lib().i360hip_export_size(total_w, total_h, max_dim, C.byref(w), C.byref(h)) != SUCCESS:
+            return None
+        return w.value, h.value
+
+    @_ordered
+    def export_composite_dev(self, chunks, total_w, total_h, max_dim, fmt, out_ptr, stream=None) -> int:
+        """chunks: sequence of (x_offset, y_offset, width, height, pixels_ptr or None, buf_w, buf_h);
+        fmt: _abi.PIX_U16 (RGB48) or _abi.PIX_F32 (render output)."""
+        arr = (_abi.ExportChunk * max(1, len(chunks)))()
+        for i, (x, y, w, h, p, bw, bh) in enumerate(chunks):
+            arr[i] = _abi.ExportChunk(x, y, w, h, p, bw, bh)
+        return lib().i360hip_export_composite_dev(self._ctx, arr, len(chunks), total_w, total_h, max_dim, fmt,
+                                                  out_ptr, stream)
+
+    @_ordered
+    def rgb48_from_render_dev(self, rgb_ptr, num_values, out_ptr, stream=None) -> int:
+        return lib().i360hip_rgb48_from_render_dev(self._ctx, rgb_ptr, num_values, out_ptr, stream)
+
     def profile_enable(self, on=True) -> None:
         lib().i360hip_profile_enable(self._ctx, 1 if on else 0)
 

@@ -174,7 +174,8 @@ int i360hip_blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int
  * No counterpart in IComputeBackend (the reference blends on the host). */
 enum {
     I360HIP_PIX_F16 = 0,
-    I360HIP_PIX_F32 = 1
+    I360HIP_PIX_F32 = 1,
+    I360HIP_PIX_U16 = 2 /* RGB48 (export composite input only) */
 };
 enum {
     I360HIP_RENDER_FP64 = 0,
@@ -320,6 +321,52 @@ int i360hip_apply_gain_u16_dev(i360hip_ctx* ctx, uint16_t* data, size_t num_valu
 /* `warped` is a HOST array of DEVICE pointers. */
 int i360hip_blend_chunk_u16_dev(i360hip_ctx* ctx, const uint16_t* const* warped, int num_contributors, int width,
                                 int height, int num_bands, uint16_t* out, int32_t* owner_out, void* stream);
+
+/* ---- preview export ---------------------------------------------------------
+ * PanoramaExporter (PanoramaExporter.cpp:9-24, 34-98) without the project
+ * database, the codec and the file write: the composite of rendered chunks
+ * into one RGB8 canvas whose longest edge is max_dimension. */
+
+/* ComputeExportScale (PanoramaExporter.cpp:9-14): max_dimension / longest
+ * edge in binary32, 1 when the longest edge is not positive. */
+float i360hip_export_scale(int total_width, int total_height, int max_dimension);
+/* ComputeChunkPlacement (PanoramaExporter.cpp:16-24): out[0..3] = destX,
+ * destY, destW, destH, each the truncated binary32 product with `scale`;
+ * destW and destH are at least 1. */
+void i360hip_export_placement(int x_offset, int y_offset, int width, int height, float scale, int* out);
+/* Composite size (PanoramaExporter.cpp:49-51); INVALID_PARAM where
+ * ExportPreviewJpeg returns false for the sizes (max_dimension <= 0, empty
+ * panorama). */
+int i360hip_export_size(int total_width, int total_height, int max_dimension, int* out_width, int* out_height);
+
+typedef struct i360hip_export_chunk {
+    int x_offset, y_offset, width, height; /* ChunkModel's canvas rectangle (Types.h) */
+    const void* pixels;                    /* device: the chunk's rendered buffer, or NULL when it has no
+                                              render output (skipped, PanoramaExporter.cpp:55-68) */
+    int buf_width, buf_height;             /* that buffer's PixelBuffer width and height */
+} i360hip_export_chunk;
+
+/* The composite loop of ExportPreviewJpeg (PanoramaExporter.cpp:53-98):
+ * chunks in list order, nearest-neighbour source pixel
+ * ((int)(dx / scale), (int)(dy / scale)), 16 -> 8 bits by >> 8, later
+ * chunks over earlier ones, a pixel whose source lies outside the buffer
+ * left as it was, 0 where nothing lands.  `format`: I360HIP_PIX_U16 for
+ * RGB48 buffers (the reference's rendered_chunk_rgb48 blobs), or
+ * I360HIP_PIX_F32 for this library's FP32 render output, which is first
+ * rounded to RGB48 as i360hip_rgb48_from_render_dev does.  rgb8_out:
+ * device, out_width * out_height * 3 bytes (i360hip_export_size), every
+ * byte written.  Uploads its tables synchronously, then enqueues. */
+int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks,
+                                 int total_width, int total_height, int max_dimension, int format,
+                                 uint8_t* rgb8_out, void* stream);
+
+/* FP32 RGB render output in [0,1] -> the RGB48 values the reference's blend
+ * stores (SeamBlend.cpp:529-534: clamp to [0,65535], then u16(v + 0.5)),
+ * applied to v * 65535.  This is the pixel payload of a
+ * rendered_chunk_rgb48 blob (RenderExecutor.cpp:138-142).  rgb: 16-byte
+ * aligned; rgb48: 8-byte aligned; num_values counts floats. */
+int i360hip_rgb48_from_render_dev(i360hip_ctx* ctx, const float* rgb, size_t num_values, uint16_t* rgb48,
+                                  void* stream);
 
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop

@@ -195,6 +195,60 @@ def render_chunk(srcs, hinv, gains, x0, y0, w, h, num_bands, fp64=True, owner_in
     return (out, owner) if want_owner else out
 
 
+# ---------------------------------------------------------------- export --
+def export_scale(total_w, total_h, max_dim):
+    f = lib().orc_export_scale
+    f.restype = C.c_float
+    f.argtypes = [C.c_int, C.c_int, C.c_int]
+    return f(total_w, total_h, max_dim)
+
+
+def export_placement(x, y, w, h, scale):
+    out = (C.c_int * 4)()
+    f = lib().orc_export_placement
+    f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int)]
+    f(x, y, w, h, scale, out)
+    return tuple(out)
+
+
+def export_size(total_w, total_h, max_dim):
+    w, h = C.c_int(0), C.c_int(0)
+    ok = lib().orc_export_size(total_w, total_h, max_dim, C.byref(w), C.byref(h))
+    return (w.value, h.value) if ok else None
+
+
+def rgb48_from_render(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, np.uint16)
+    lib().orc_rgb48_from_render(_p(x, f32p), C.c_longlong(x.size), _p(out, u16p))
+    return out
+
+
+def export_composite(chunks, total_w, total_h, max_dim):
+    """chunks: list of (x_offset, y_offset, width, height, buf) with buf an
+    (h, w, 3) uint16 or float32 array, or None.  All buffers of one call
+    share one dtype.  Returns the (out_h, out_w, 3) uint8 composite, or None
+    where the reference returns false."""
+    size = export_size(total_w, total_h, max_dim)
+    if size is None:
+        return None
+    n = len(chunks)
+    bufs = [None if c[4] is None else np.ascontiguousarray(c[4]) for c in chunks]
+    kinds = {b.dtype for b in bufs if b is not None}
+    assert len(kinds) <= 1
+    is_f32 = int(kinds == {np.dtype(np.float32)})
+    rect = np.array([c[:4] for c in chunks] or [[0, 0, 0, 0]], np.int32).reshape(-1)
+    bw = np.array([0 if b is None else b.shape[1] for b in bufs] or [0], np.int32)
+    bh = np.array([0 if b is None else b.shape[0] for b in bufs] or [0], np.int32)
+    ptrs = (C.c_void_p * max(1, n))()
+    for i, b in enumerate(bufs):
+        ptrs[i] = None if b is None else b.ctypes.data
+    out = np.empty((size[1], size[0], 3), np.uint8)
+    lib().orc_export_composite(_p(rect, i32p), ptrs, _p(bw, i32p), _p(bh, i32p), n, total_w, total_h, max_dim,
+                               is_f32, _p(out, u8p))
+    return out
+
+
 # -------------------------------------------------------------- reference --
 def ref_tier():
     r = ref()

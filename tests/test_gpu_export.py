@@ -1,0 +1,157 @@
+"""GPU parity of the preview export composite and of the RGB48 rounding of
+the render output, through the C ABI, against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+from image360_amd import _abi
+from image360_amd.synth import chunk_grid
+from oracle import pyoracle as po
+from tests.gpu_util import note
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _composite(hip, chunks, tw, th, max_dim, fmt, offset=0):
+    """chunks: (x, y, w, h, host buffer or None).  Returns the GPU composite."""
+    import torch
+
+    size = hip.export_size(tw, th, max_dim)
+    assert size == po.export_size(tw, th, max_dim)
+    ow, oh = size
+    keep, args = [], []
+    for (x, y, w, h, b) in chunks:
+        if b is None:
+            args.append((x, y, w, h, None, 0, 0))
+            continue
+        d = _dev(b)
+        keep.append(d)
+        args.append((x, y, w, h, d.data_ptr(), b.shape[1], b.shape[0]))
+    n = ow * oh * 3
+    out = torch.full((n + 8,), 0xCD, dtype=torch.uint8, device="cuda")
+    hip.check(hip.export_composite_dev(args, tw, th, max_dim, fmt, out.data_ptr() + offset), "export_composite_dev")
+    host = out.cpu().numpy()
+    # bytes around the composite stay untouched
+    assert (host[:offset] == 0xCD).all() and (host[offset + n:] == 0xCD).all()
+    return host[offset:offset + n].reshape(oh, ow, 3)
+
+
+def _random_layout(rng, tw, th, cell, f32):
+    chunks = []
+    for (x, y, w, h) in chunk_grid(tw, th, cell):
+        r = rng.random()
+        if r < 0.1:
+            chunks.append((x, y, w, h, None))
+            continue
+        # jittered rectangles overlap their neighbours; some buffers are smaller than their chunk
+        x += int(rng.integers(-cell // 8, cell // 8 + 1))
+        y += int(rng.integers(-cell // 8, cell // 8 + 1))
+        w += int(rng.integers(0, cell // 4))
+        h += int(rng.integers(0, cell // 4))
+        bw = w if r > 0.3 else max(1, int(w * rng.uniform(0.5, 1.0)))
+        bh = h if r > 0.3 else max(1, int(h * rng.uniform(0.5, 1.0)))
+        if f32:
+            b = rng.uniform(-0.05, 1.05, (bh, bw, 3)).astype(np.float32)
+        else:
+            b = rng.integers(0, 65536, (bh, bw, 3), dtype=np.uint16)
+        chunks.append((x, y, w, h, b))
+    order = rng.permutation(len(chunks))
+    return [chunks[i] for i in order]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("f32", [False, True])
+def test_random_layouts_match_oracle(hip, seed, f32):
+    rng = np.random.default_rng(seed)
+    tw, th = int(rng.integers(300, 900)), int(rng.integers(200, 700))
+    chunks = _random_layout(rng, tw, th, int(rng.integers(96, 200)), f32)
+    for max_dim in (max(tw, th), 64, 91, 333, int(rng.integers(17, max(tw, th)))):
+        note(f"export seed={seed} f32={f32} canvas={tw}x{th} chunks={len(chunks)} max_dim={max_dim}")
+        want = po.export_composite(chunks, tw, th, max_dim)
+        got = _composite(hip, chunks, tw, th, max_dim, _abi.PIX_F32 if f32 else _abi.PIX_U16)
+        assert np.array_equal(got, want), (seed, f32, max_dim, np.argwhere(got != want)[:5])
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_unaligned_output(hip, offset):
+    rng = np.random.default_rng(7)
+    chunks = _random_layout(rng, 517, 301, 128, False)
+    note(f"export unaligned offset={offset}")
+    want = po.export_composite(chunks, 517, 301, 205)
+    got = _composite(hip, chunks, 517, 301, 205, _abi.PIX_U16, offset=offset)
+    assert np.array_equal(got, want)
+
+
+def test_edge_cases(hip):
+    a = np.full((20, 20, 3), 0x1234, np.uint16)
+    note("export edge cases")
+    # nothing lands: every byte is written as 0
+    got = _composite(hip, [(0, 0, 20, 20, None), (100, 100, 20, 20, a)], 40, 40, 40, _abi.PIX_U16)
+    assert not got.any()
+    # no chunks at all
+    assert not _composite(hip, [], 40, 30, 17, _abi.PIX_U16).any()
+    # one-pixel output
+    got = _composite(hip, [(0, 0, 20, 20, a)], 1000, 1, 1, _abi.PIX_U16)
+    assert got.shape == (1, 1, 3) and got.tolist() == [[[0x12, 0x12, 0x12]]]
+    # the reference's rejections
+    import torch
+
+    out = torch.zeros(16, dtype=torch.uint8, device="cuda")
+    assert hip.export_composite_dev([], 10, 10, 0, _abi.PIX_U16, out.data_ptr()) == _abi.INVALID_PARAM
+    assert hip.export_composite_dev([], 0, 10, 5, _abi.PIX_U16, out.data_ptr()) == _abi.INVALID_PARAM
+    assert hip.export_composite_dev([], 10, 10, 5, _abi.PIX_F16, out.data_ptr()) == _abi.INVALID_PARAM
+
+
+def test_repeated_calls_follow_changed_chunks(hip):
+    rng = np.random.default_rng(11)
+    c1 = _random_layout(rng, 400, 300, 100, False)
+    c2 = _random_layout(rng, 400, 300, 64, False)
+    for k, chunks in enumerate((c1, c2, c1)):
+        note(f"export repeat call {k}")
+        assert np.array_equal(_composite(hip, chunks, 400, 300, 150, _abi.PIX_U16),
+                              po.export_composite(chunks, 400, 300, 150))
+
+
+def test_rgb48_from_render_matches_oracle(hip):
+    import torch
+
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 4, 5, 1023, 4096 * 3 + 2):
+        v = rng.uniform(-0.2, 1.2, n).astype(np.float32)
+        v[:: 7] = rng.random(len(v[:: 7])).astype(np.float32)
+        if n > 8:
+            v[:8] = [0.0, 1.0, 0.5, -0.0, np.nan, np.inf, -np.inf, 1.0 / 65535]
+        note(f"rgb48 n={n}")
+        src = _dev(v)
+        out = torch.full((n + 4,), 0xBEEF, dtype=torch.int32, device="cuda").to(torch.int16)
+        hip.check(hip.rgb48_from_render_dev(src.data_ptr(), n, out.data_ptr()), "rgb48_from_render_dev")
+        got = out.cpu().numpy().view(np.uint16)
+        assert np.array_equal(got[:n], po.rgb48_from_render(v)), n
+        assert (got[n:] == np.uint16(0xBEEF)).all()
+
+
+def test_gigapixel_canvas_export(hip):
+    """The gigapixel canvas (37977 x 19405, 50 chunks of up to 4096^2) at
+    the preview sizes the application uses (4096) and twice that."""
+    import torch
+
+    tw, th = 37977, 19405
+    rects = chunk_grid(tw, th, 4096)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    dev = [torch.randint(-32768, 32768, (h, w, 3), dtype=torch.int16, device="cuda", generator=gen)
+           for (x, y, w, h) in rects]
+    host = [d.cpu().numpy().view(np.uint16) for d in dev]
+    args = [(x, y, w, h, d.data_ptr(), w, h) for (x, y, w, h), d in zip(rects, dev)]
+    for max_dim in (4096, 8192):
+        ow, oh = hip.export_size(tw, th, max_dim)
+        note(f"export gigapixel max_dim={max_dim} -> {ow}x{oh}")
+        out = torch.empty(ow * oh * 3, dtype=torch.uint8, device="cuda")
+        hip.check(hip.export_composite_dev(args, tw, th, max_dim, _abi.PIX_U16, out.data_ptr()), "export")
+        want = po.export_composite([r + (b,) for r, b in zip(rects, host)], tw, th, max_dim)
+        assert np.array_equal(out.cpu().numpy().reshape(oh, ow, 3), want), max_dim
