@@ -151,8 +151,40 @@ class HipComputeBackend:
         own = None if ownerOut is None else _ptr(_c_contig(ownerOut, np.int32))
         return lib().i360hip_blend_chunk_u16(self._ctx, arr, len(bufs), width, height, numBands, _ptr(out), own)
 
-    # Align / Optimize / burst operations are not on this backend's path: the
-    # reference GPU backends answer NOT_SUPPORTED for the burst family
+    # ---- optimize-stage colour operations (GainColorOps.cpp:103-157)
+    def ComputeLabStats(self, rgb, width, height, outMean, outStd) -> int:
+        """outMean / outStd: float64 arrays of 3, written in place."""
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        rgb = _c_contig(rgb, np.uint16)
+        m = _c_contig(outMean, np.float64)
+        sd = _c_contig(outStd, np.float64)
+        return lib().i360hip_compute_lab_stats(self._ctx, _ptr(rgb), width, height, _ptr(m), _ptr(sd))
+
+    def ApplyReinhardColorTransfer(self, rgbInOut, width, height, srcMean, srcStd, refMean, refStd) -> int:
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        rgb = _c_contig(rgbInOut, np.uint16)
+        v = [np.ascontiguousarray(x, dtype=np.float64).reshape(3) for x in (srcMean, srcStd, refMean, refStd)]
+        return lib().i360hip_apply_reinhard_u16(self._ctx, _ptr(rgb), width, height, *[_ptr(x) for x in v])
+
+    def lab_stats_dev(self, rgb_ptr, width, height, stream=None):
+        """(rc, mean[3], std[3]) of a device RGB48 image; returns once the result is on the host."""
+        m, sd = np.zeros(3), np.zeros(3)
+        rc = self._lab_stats_dev(rgb_ptr, width, height, m, sd, stream=stream)
+        return rc, m, sd
+
+    @_ordered
+    def _lab_stats_dev(self, rgb_ptr, width, height, m, sd, stream=None) -> int:
+        return lib().i360hip_compute_lab_stats_dev(self._ctx, rgb_ptr, width, height, _ptr(m), _ptr(sd), stream)
+
+    @_ordered
+    def reinhard_dev(self, rgb_ptr, width, height, src_mean, src_std, ref_mean, ref_std, stream=None) -> int:
+        v = [np.ascontiguousarray(x, dtype=np.float64).reshape(3) for x in (src_mean, src_std, ref_mean, ref_std)]
+        return lib().i360hip_apply_reinhard_u16_dev(self._ctx, rgb_ptr, width, height, *[_ptr(x) for x in v], stream)
+
+    # Align / burst operations are not on this backend's path: the reference
+    # GPU backends answer NOT_SUPPORTED for the burst family
     # (CudaPipeline.cpp:794-839); hosts delegate the rest to their CPU backend.
     def BlockMatchAlign(self, *a, **k) -> int:
         self._error = "BlockMatchAlign is not supported by the HIP backend."

@@ -45,7 +45,7 @@ namespace WindowsApp { namespace Hip
     // fills `sel` (two tasks in flight, chunk size from device memory,
     // status "Using GPU (HIP): <name>") and returns true; otherwise stores
     // the backend's error text in lastError and returns false.
-    // hostDelegate serves the align/optimize operations (may be null).
+    // hostDelegate serves the align operations (may be null).
     bool TryHip(BackendSelection& sel, std::wstring& lastError,
                 std::shared_ptr<Compute::IComputeBackend> hostDelegate = nullptr, int device = 0);
 

@@ -145,7 +145,7 @@ namespace WindowsApp { namespace Compute
                                                   filters, rgbOut));
     }
 
-    // ---- align / optimize operations: served by the host delegate
+    // ---- align operations: served by the host delegate
 
     ComputeResult HipComputeBackend::Delegated(const char* op)
     {
@@ -180,14 +180,15 @@ namespace WindowsApp { namespace Compute
         return r;
     }
 
+    // ---- optimize-stage colour operations: on the GPU (GainColorOps.cpp:103-157)
+
     ComputeResult HipComputeBackend::ComputeLabStats(
         const unsigned short* rgb, int width, int height, double outMean[3], double outStd[3])
     {
         if (!m_ctx) return Fail(ComputeResult::CUDA_ERROR, "Not initialized.");
-        if (!m_host) return Delegated("ComputeLabStats");
-        const ComputeResult r = m_host->ComputeLabStats(rgb, width, height, outMean, outStd);
-        if (r != ComputeResult::SUCCESS) Fail(r, m_host->GetLastError());
-        return r;
+        if (!rgb || !outMean || !outStd) return Fail(ComputeResult::INVALID_PARAM, "Null pointer argument.");
+        if (width <= 0 || height <= 0) return Fail(ComputeResult::INVALID_PARAM, "Invalid dimensions.");
+        return FromAbi(i360hip_compute_lab_stats(m_ctx, rgb, width, height, outMean, outStd));
     }
 
     ComputeResult HipComputeBackend::ApplyReinhardColorTransfer(
@@ -196,11 +197,10 @@ namespace WindowsApp { namespace Compute
         const double refMean[3], const double refStd[3])
     {
         if (!m_ctx) return Fail(ComputeResult::CUDA_ERROR, "Not initialized.");
-        if (!m_host) return Delegated("ApplyReinhardColorTransfer");
-        const ComputeResult r = m_host->ApplyReinhardColorTransfer(rgbInOut, width, height, srcMean, srcStd, refMean,
-                                                                   refStd);
-        if (r != ComputeResult::SUCCESS) Fail(r, m_host->GetLastError());
-        return r;
+        if (!rgbInOut || !srcMean || !srcStd || !refMean || !refStd)
+            return Fail(ComputeResult::INVALID_PARAM, "Null pointer argument.");
+        if (width <= 0 || height <= 0) return Fail(ComputeResult::INVALID_PARAM, "Invalid dimensions.");
+        return FromAbi(i360hip_apply_reinhard_u16(m_ctx, rgbInOut, width, height, srcMean, srcStd, refMean, refStd));
     }
 
     // ---- burst-mode operations: not on this backend

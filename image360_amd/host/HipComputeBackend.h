@@ -16,12 +16,12 @@
 //    runs two GPU tasks in flight, TaskScheduler.cpp:67-71): the C ABI
 //    context serialises its calls, this class guards its own state.
 //
-// Warp / median / gain / demosaic run on the GPU.  The align and optimize
-// operations (features, matching, Lab statistics, Reinhard transfer) are
-// not on the render path: like VulkanPipeline (VulkanPipeline.h:18-23) this
-// backend hands them to a host backend supplied at construction (in the
-// reference build, Core's CpuComputeBackend) and reports NOT_SUPPORTED when
-// none was given.  The burst-mode operations report NOT_SUPPORTED, as
+// Warp / median / gain / demosaic and the optimize stage's colour
+// operations (Lab statistics, Reinhard transfer) run on the GPU.  The align
+// operations (features, matching) are not on the render path: like
+// VulkanPipeline (VulkanPipeline.h:18-23) this backend hands them to a host
+// backend supplied at construction (in the reference build, Core's
+// CpuComputeBackend) and reports NOT_SUPPORTED when none was given.  The burst-mode operations report NOT_SUPPORTED, as
 // CudaPipeline does (CudaPipeline.cpp:794-839).
 //
 // Beyond IComputeBackend it implements IChunkBlendBackend, the GPU form of
@@ -60,7 +60,7 @@ namespace WindowsApp { namespace Compute
     {
     public:
         // device: HIP device index.  hostDelegate: backend that serves the
-        // align/optimize operations (may be null).
+        // align operations (may be null).
         explicit HipComputeBackend(int device = 0, std::shared_ptr<IComputeBackend> hostDelegate = nullptr);
         ~HipComputeBackend() override;
 

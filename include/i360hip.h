@@ -368,6 +368,28 @@ int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* c
 int i360hip_rgb48_from_render_dev(i360hip_ctx* ctx, const float* rgb, size_t num_values, uint16_t* rgb48,
                                   void* stream);
 
+/* ---- optimize-stage colour operations --------------------------------------
+ * IComputeBackend::ComputeLabStats and ::ApplyReinhardColorTransfer
+ * (IComputeBackend.h:65-73; CpuComputeBackend.cpp:234-257 for the
+ * validation order and messages; GainColorOps.cpp:28-157 for the maths).
+ * pow and cbrt are correctly rounded here; the reference's C library
+ * versions are not, so results match the reference to within 1e-4 (Lab
+ * mean and deviation) and 2 LSB per unit of transfer gain (transferred
+ * RGB48, over 99 % identical) rather than bit for bit.  The statistics are deterministic on a device. */
+int i360hip_compute_lab_stats(i360hip_ctx* ctx, const uint16_t* rgb, int width, int height, double* out_mean,
+                              double* out_std);
+int i360hip_apply_reinhard_u16(i360hip_ctx* ctx, uint16_t* rgb_in_out, int width, int height,
+                               const double* src_mean, const double* src_std, const double* ref_mean,
+                               const double* ref_std);
+/* Device-image forms: rgb is device memory; the statistics and transfer
+ * parameters are host arrays of 3 doubles.  The statistics call returns
+ * after its result has reached the host; the transfer is enqueued. */
+int i360hip_compute_lab_stats_dev(i360hip_ctx* ctx, const uint16_t* rgb, int width, int height, double* out_mean,
+                                  double* out_std, void* stream);
+int i360hip_apply_reinhard_u16_dev(i360hip_ctx* ctx, uint16_t* rgb_in_out, int width, int height,
+                                   const double* src_mean, const double* src_std, const double* ref_mean,
+                                   const double* ref_std, void* stream);
+
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop
 #endif

@@ -249,6 +249,47 @@ def export_composite(chunks, total_w, total_h, max_dim):
     return out
 
 
+# ----------------------------------------------------------------- colour --
+def rgb_to_lab(rgb):
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint16)
+    out = np.empty(rgb.shape, np.float32)
+    lib().orc_rgb_to_lab(_p(rgb, u16p), C.c_longlong(rgb.size // 3), _p(out, f32p))
+    return out
+
+
+def lab_to_rgb(lab):
+    lab = np.ascontiguousarray(lab, dtype=np.float32)
+    out = np.empty(lab.shape, np.uint16)
+    lib().orc_lab_to_rgb(_p(lab, f32p), C.c_longlong(lab.size // 3), _p(out, u16p))
+    return out
+
+
+def _stats(fn, rgb):
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint16)
+    h, w = rgb.shape[0], rgb.size // (3 * rgb.shape[0])
+    m, s = np.zeros(3), np.zeros(3)
+    fn(_p(rgb, u16p), w, h, _p(m, f64p), _p(s, f64p))
+    return m, s
+
+
+def _transfer(fn, rgb, sm, ss, rm, rs):
+    out = np.ascontiguousarray(rgb, dtype=np.uint16).copy()
+    h, w = out.shape[0], out.size // (3 * out.shape[0])
+    args = [np.ascontiguousarray(v, dtype=np.float64).reshape(3) for v in (sm, ss, rm, rs)]
+    fn(_p(out, u16p), w, h, *[_p(v, f64p) for v in args])
+    return out
+
+
+def lab_stats(rgb):
+    """(mean[3], std[3]) of an (h, w, 3) RGB48 image (GainColorOps.cpp:103-130)."""
+    return _stats(lib().orc_lab_stats, rgb)
+
+
+def reinhard(rgb, src_mean, src_std, ref_mean, ref_std):
+    """Reinhard transfer of an (h, w, 3) RGB48 image (GainColorOps.cpp:132-157)."""
+    return _transfer(lib().orc_reinhard, rgb, src_mean, src_std, ref_mean, ref_std)
+
+
 # -------------------------------------------------------------- reference --
 def ref_tier():
     r = ref()
@@ -287,6 +328,14 @@ def ref_apply_gain(data, gain):
     d = np.ascontiguousarray(data, dtype=np.uint16).copy()
     ref().ref_apply_gain(_p(d, u16p), d.size, C.c_float(gain))
     return d
+
+
+def ref_lab_stats(rgb):
+    return _stats(ref().ref_lab_stats, rgb)
+
+
+def ref_reinhard(rgb, src_mean, src_std, ref_mean, ref_std):
+    return _transfer(ref().ref_reinhard, rgb, src_mean, src_std, ref_mean, ref_std)
 
 
 def ref_invert_homography(hm):

@@ -51,6 +51,14 @@ void ref_demosaic_bayer(int tier, const uint16_t* cfa, int w, int h, uint16_t bl
 
 void ref_apply_gain(uint16_t* data, int n, float gain) { WindowsApp::Core::ApplyGainCpu(data, n, gain); }
 
+void ref_lab_stats(const uint16_t* rgb, int w, int h, double* mean, double* sd) {
+    WindowsApp::Core::ComputeLabStatsCpu(rgb, w, h, mean, sd);
+}
+
+void ref_reinhard(uint16_t* rgb, int w, int h, const double* sm, const double* ss, const double* rm, const double* rs) {
+    WindowsApp::Core::ApplyReinhardColorTransferCpu(rgb, w, h, sm, ss, rm, rs);
+}
+
 int ref_invert_homography(const float* h, float* inv) { return WindowsApp::Core::InvertHomography3x3(h, inv) ? 1 : 0; }
 
 void ref_blend_chunk(const uint16_t* const* warped, int c, int w, int h, uint16_t* out) {

@@ -61,7 +61,8 @@ namespace
     };
 
     // A host backend standing in for Core's CpuComputeBackend: it answers
-    // the align/optimize operations with fixed values and counts the calls.
+    // the align operations (and, were they delegated, the optimize ones)
+    // with fixed values and counts the calls.
     class FakeHost : public IComputeBackend
     {
     public:
@@ -79,7 +80,7 @@ namespace
         ComputeResult DetectAndDescribeFeatures(const unsigned char*, int, int, FeaturePoint*, BriefDescriptor*, int* outCount, int) override
         { ++calls; *outCount = 7; return ComputeResult::SUCCESS; }
         ComputeResult MatchFeatures(const BriefDescriptor*, int, const BriefDescriptor*, int, MatchResult*, int* n, int, float) override
-        { ++calls; *n = 3; return ComputeResult::SUCCESS; }
+        { ++calls; *n = 3; return ComputeResult::INVALID_PARAM; }
         ComputeResult ComputeLabStats(const unsigned short*, int, int, double m[3], double s[3]) override
         { ++calls; for (int i = 0; i < 3; ++i) { m[i] = i + 0.5; s[i] = 2.0; } return ComputeResult::SUCCESS; }
         ComputeResult ApplyReinhardColorTransfer(unsigned short*, int, int, const double[3], const double[3], const double[3], const double[3]) override
@@ -191,17 +192,31 @@ namespace
                       std::strcmp(b->GetLastError(), "Invalid pixel count.") == 0, "gain zero count");
             BurstOpsUnsupported(*b);
         }
-        // align/optimize operations go to the host delegate
+        // align operations go to the host delegate
         {
-            double m[3], s[3];
-            unsigned short px[12] = {};
-            Check(b->ComputeLabStats(px, 2, 2, m, s) == ComputeResult::SUCCESS && m[2] == 2.5 && host->calls == 1,
-                  "ComputeLabStats delegated");
-            Check(b->ApplyReinhardColorTransfer(px, 2, 2, m, s, m, s) == ComputeResult::INVALID_PARAM &&
+            unsigned char img[12] = {};
+            FeaturePoint pts[4];
+            BriefDescriptor desc[4];
+            MatchResult mr[4];
+            int cnt = 0, nm = 0;
+            Check(b->DetectAndDescribeFeatures(img, 2, 2, pts, desc, &cnt, 4) == ComputeResult::SUCCESS && cnt == 7 &&
+                      host->calls == 1, "DetectAndDescribeFeatures delegated");
+            Check(b->MatchFeatures(desc, 1, desc, 1, mr, &nm, 4) == ComputeResult::INVALID_PARAM &&
                       std::strcmp(b->GetLastError(), "fake host error") == 0, "delegate errors propagate");
             HipComputeBackend lone(0);
             Check(lone.Initialize() == ComputeResult::SUCCESS, "second instance initialises");
-            Check(lone.ComputeLabStats(px, 2, 2, m, s) == ComputeResult::NOT_SUPPORTED, "no delegate: NOT_SUPPORTED");
+            Check(lone.DetectAndDescribeFeatures(img, 2, 2, pts, desc, &cnt, 4) == ComputeResult::NOT_SUPPORTED,
+                  "no delegate: NOT_SUPPORTED");
+            // the optimize colour operations run on the GPU, with the CPU backend's validation
+            double m[3] = {50, 0, 0}, sd[3] = {10, 10, 10};
+            unsigned short px[12] = {};
+            Check(lone.ComputeLabStats(nullptr, 2, 2, m, sd) == ComputeResult::INVALID_PARAM &&
+                      std::strcmp(lone.GetLastError(), "Null pointer argument.") == 0, "lab stats null");
+            Check(b->ApplyReinhardColorTransfer(px, 2, 0, m, sd, m, sd) == ComputeResult::INVALID_PARAM &&
+                      std::strcmp(b->GetLastError(), "Invalid dimensions.") == 0, "reinhard bad dimensions");
+            Check(lone.ComputeLabStats(px, 2, 2, m, sd) == ComputeResult::SUCCESS && m[0] == 0.0 && sd[0] == 0.0,
+                  "black image: L mean 0, std 0 without a delegate");
+            Check(host->calls == 2, "optimize operations are not delegated");
         }
         // WarpPerspective
         const int sw = 301, sh = 203, dw = 257, dh = 190;
@@ -249,6 +264,27 @@ namespace
               "DemosaicBayer runs");
         out.Put("demosaic_in", "u16", cfa, {bh, bw});
         out.Put("demosaic_out", "u16", rgb, {bh, bw, 3});
+        // ComputeLabStats + ApplyReinhardColorTransfer (the optimize stage's
+        // colour transfer, OptimizeExecutor.cpp:308-327)
+        {
+            const int lw = 211, lh = 97;
+            const std::vector<unsigned short> tgt = Noise(rng, size_t(lw) * lh * 3, 65535);
+            std::vector<unsigned short> ref = Noise(rng, size_t(lw) * lh * 3, 40000);
+            double rm[3], rs[3], tm[3], ts[3];
+            Check(b->ComputeLabStats(ref.data(), lw, lh, rm, rs) == ComputeResult::SUCCESS, "ComputeLabStats ref");
+            Check(b->ComputeLabStats(tgt.data(), lw, lh, tm, ts) == ComputeResult::SUCCESS, "ComputeLabStats target");
+            std::vector<unsigned short> moved = tgt;
+            Check(b->ApplyReinhardColorTransfer(moved.data(), lw, lh, tm, ts, rm, rs) == ComputeResult::SUCCESS,
+                  "ApplyReinhardColorTransfer runs");
+            out.Put("lab_ref", "u16", ref, {lh, lw, 3});
+            out.Put("lab_tgt", "u16", tgt, {lh, lw, 3});
+            std::vector<double> stats(rm, rm + 3);
+            stats.insert(stats.end(), rs, rs + 3);
+            stats.insert(stats.end(), tm, tm + 3);
+            stats.insert(stats.end(), ts, ts + 3);
+            out.Put("lab_stats", "f64", stats, {4, 3});
+            out.Put("reinhard_out", "u16", moved, {lh, lw, 3});
+        }
         // BlendChunkContributors via the extension interface, as a render
         // driver finds it
         {

@@ -36,7 +36,7 @@ def _load(d):
     with open(os.path.join(d, "manifest.txt")) as f:
         for line in f:
             name, dtype, *shape = line.split()
-            dt = {"u16": np.uint16, "f32": np.float32}[dtype]
+            dt = {"u16": np.uint16, "f32": np.float32, "f64": np.float64}[dtype]
             arrs[name] = np.fromfile(os.path.join(d, name + ".bin"), dtype=dt).reshape([int(s) for s in shape])
     return arrs
 
@@ -55,3 +55,9 @@ def test_operations_through_cpp_interface(tmp_path):
     rgb_cam = np.array([[1.6, -0.4, -0.2, 0.0], [-0.2, 1.5, -0.3, 0.0], [0.05, -0.5, 1.45, 0.0]], np.float32)
     assert np.array_equal(a["demosaic_out"], po.demosaic(a["demosaic_in"], 512, cam_mul, rgb_cam, 0x94949494))
     assert np.array_equal(a["blend_out"], po.blend_chunk(list(a["blend_in"]), 5))
+    # colour transfer: within the tolerance the C library's powf / cbrtf leave open
+    rm, rs = po.lab_stats(a["lab_ref"])
+    tm, ts = po.lab_stats(a["lab_tgt"])
+    assert np.abs(a["lab_stats"] - np.array([rm, rs, tm, ts])).max() <= 1e-4
+    d = np.abs(a["reinhard_out"].astype(np.int32) - po.reinhard(a["lab_tgt"], tm, ts, rm, rs).astype(np.int32))
+    assert d.max() <= 2 and (d == 0).mean() >= 0.99
