@@ -311,6 +311,70 @@ def bench_export(hip, torch, geom, chunks, out, offs, log, max_dims=(8192, 4096)
     return res, rgb48
 
 
+def bench_color(hip, torch, frame_f16, log):
+    """The optimize stage's colour transfer (OptimizeExecutor.cpp:308-327) on
+    one 6000x4000 frame in HBM: Lab statistics of a reference and a target
+    frame, then the Reinhard transfer of the target.  Returns the results and
+    the RGB48 frame (kept for the CPU leg)."""
+    h, w = frame_f16.shape[0], frame_f16.shape[1]
+    rgb = torch.round(frame_f16.float() * 65535.0).clamp_(0, 65535).to(torch.int32).to(torch.int16)
+    ref = torch.flip(rgb, dims=[1]).contiguous()
+    work = rgb.clone()
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    reps = 5
+    with torch.cuda.stream(stream):
+        rc, rm, rs = hip.lab_stats_dev(ref.data_ptr(), w, h, sh)
+        hip.check(rc, "lab stats")
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        for _ in range(reps):
+            rc, tm, ts = hip.lab_stats_dev(rgb.data_ptr(), w, h, sh)
+        ev[1].record(stream)
+        for _ in range(reps):
+            work.copy_(rgb)
+            hip.check(hip.reinhard_dev(work.data_ptr(), w, h, tm, ts, rm, rs, sh), "reinhard")
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+    copy = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        copy[0].record(stream)
+        for _ in range(reps):
+            work.copy_(rgb)
+        copy[1].record(stream)
+        torch.cuda.synchronize()
+    t_stats = ev[0].elapsed_time(ev[1]) / reps  # includes the device-to-host read of the six results
+    t_apply = (ev[1].elapsed_time(ev[2]) - copy[0].elapsed_time(copy[1])) / reps
+    px = h * w
+    r = {"frame": [w, h], "lab_stats_ms": round(t_stats, 3), "lab_stats_mp_s": round(px / (t_stats / 1e3) / 1e6, 1),
+         "reinhard_ms": round(t_apply, 3), "reinhard_mp_s": round(px / (t_apply / 1e3) / 1e6, 1)}
+    log(f"colour transfer: {r}")
+    return r, rgb
+
+
+def cpu_color(rgb, log, rows=400):
+    """Lab statistics + Reinhard transfer of the compiled reference sources
+    (oracle/_ref, one thread; the oracle port where they were not built) on
+    a band of rows of the same frame."""
+    from oracle import pyoracle as po
+
+    img = rgb[:rows].cpu().numpy().view(np.uint16)
+    kind = "reference" if po.ref() is not None else "port"
+    stats = po.ref_lab_stats if kind == "reference" else po.lab_stats
+    transfer = po.ref_reinhard if kind == "reference" else po.reinhard
+    t0 = time.perf_counter()
+    m, s = stats(img)
+    t1 = time.perf_counter()
+    transfer(img, m, s, m + 1.0, s * 1.1)
+    t2 = time.perf_counter()
+    px = img.shape[0] * img.shape[1]
+    r = {"sample": f"{img.shape[1]}x{img.shape[0]} band of the frame", "lab_stats_mp_s": round(px / (t1 - t0) / 1e6, 2),
+         "reinhard_mp_s": round(px / (t2 - t1) / 1e6, 2), "cores": 1, "kind": kind}
+    log(f"CPU colour transfer: {r}")
+    return r
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -480,12 +544,13 @@ def main():
     psnr, maxerr = head.get("psnr_db_vs_fp64"), head.get("max_abs_err_vs_fp64")
 
     extra = None
-    rgb48 = None
+    rgb48 = frame48 = None
     if not args.no_extra:
         exp, rgb48 = bench_export(hip, torch, geom, chunks, out, offs, log)
+        col, frame48 = bench_color(hip, torch, frames[0], log)
         extra = {"burst_median_demosaic": bench_burst(hip, torch, log),
                  "uneven_chunks": bench_uneven(hip, torch, geom, fdesc, args, log),
-                 "preview_export": exp}
+                 "preview_export": exp, "colour_transfer": col}
 
     cpu = None
     if not args.no_cpu:
@@ -503,6 +568,8 @@ def main():
         log(f"CPU baseline: {cpu['value']:.2f} MP/s")
         if rgb48 is not None:
             cpu["preview_export"] = cpu_export(geom, chunks, rgb48, offs, log)
+        if frame48 is not None:
+            cpu["colour_transfer"] = cpu_color(frame48, log)
 
     line = {
         "metric": METRIC, "value": head["mp_s"], "unit": "MP/s", "n_gpus": 1, "steps": args.steps,
