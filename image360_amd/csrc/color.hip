@@ -140,23 +140,35 @@ __global__ void __launch_bounds__(kStatsBlock) lab_stats_kernel(const uint16_t* 
     if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Sums the block partials in block order; mean and deviation as
-// GainColorOps.cpp:123-129.  out[0..2] = mean, out[3..5] = std.
-__global__ void lab_stats_final_kernel(const double* __restrict__ partial, int blocks, size_t n,
-                                       double* __restrict__ out) {
+// Sums the block partials in a fixed order (thread t takes blocks t,
+// t + 256, ... in turn, then a fixed tree over the threads); mean and
+// deviation as GainColorOps.cpp:123-129.  out[0..2] = mean, out[3..5] = std.
+__global__ void __launch_bounds__(kStatsBlock) lab_stats_final_kernel(const double* __restrict__ partial,
+                                                                     int blocks, size_t n,
+                                                                     double* __restrict__ out) {
 #pragma clang fp contract(off)
-    const int c = threadIdx.x;
-    if (c >= 3) return;
-    double s = 0.0, q = 0.0;
-    for (int b = 0; b < blocks; ++b) {
-        s += partial[6 * b + c];
-        q += partial[6 * b + 3 + c];
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < blocks; b += kStatsBlock)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[k] += partial[6 * b + k];
+    __shared__ double red[6][kStatsBlock];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int w = kStatsBlock / 2; w > 0; w /= 2) {
+        if (static_cast<int>(threadIdx.x) < w)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
     }
-    const double np = static_cast<double>(n);
-    const double mean = s / np;
-    const double var = q / np - mean * mean;
-    out[c] = mean;
-    out[3 + c] = sqrt(var > 0.0 ? var : 0.0);
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        const double np = static_cast<double>(n);
+        const double mean = red[c][0] / np;
+        const double var = red[3 + c][0] / np - mean * mean;
+        out[c] = mean;
+        out[3 + c] = sqrt(var > 0.0 ? var : 0.0);
+    }
 }
 
 struct Transfer {
@@ -223,7 +235,7 @@ int run_stats(i360hip_ctx* ctx, const uint16_t* rgb, size_t n, double* mean, dou
     ProfScope ps(ctx, s, "lab_stats", 6.0 * n);
     srgb_lut_kernel<<<kLut / 256, 256, 0, s>>>(w.lut);
     lab_stats_kernel<<<blocks, kStatsBlock, 0, s>>>(rgb, n, w.lut, w.partial);
-    lab_stats_final_kernel<<<1, 64, 0, s>>>(w.partial, blocks, n, w.result);
+    lab_stats_final_kernel<<<1, kStatsBlock, 0, s>>>(w.partial, blocks, n, w.result);
     double host[6];
     I360_TRY(st, hipGetLastError());
     I360_TRY(st, hipMemcpyAsync(host, w.result, sizeof(host), hipMemcpyDeviceToHost, s));

@@ -94,6 +94,12 @@ struct i360hip_ctx {
     // launch configuration of the batched render (i360hip_render_autotune;
     // 0 = built-in default)
     int tune_groups = 0, tune_strip = 0, tune_share = 0;
+    // export composite tables last uploaded (export.hip): argument hash,
+    // the device block they live in, and the offsets of the square offsets
+    // and chunk lists in it
+    uint64_t export_key = 0;
+    void* export_dev = nullptr;
+    size_t export_off[2] = {0, 0};
     std::vector<i360::DevBuf> pinned;
 };
 

@@ -14,7 +14,8 @@
 // scans its square's list from the last entry down and takes the first
 // chunk that writes it, which is the value the in-order loop leaves.  A
 // thread does four consecutive pixels (12 output bytes, three aligned
-// dword stores).  The division is IEEE binary32 (correctly rounded), as
+// dword stores); when they share a row and a square, one scan of the list
+// decides all four and their texel loads go out together.  The division is IEEE binary32 (correctly rounded), as
 // the reference's.
 
 #include <algorithm>
@@ -46,6 +47,62 @@ __device__ __forceinline__ uint32_t u16_of(float v) {
     return static_cast<uint32_t>(b + 0.5f);
 }
 
+// RGB8 of source pixel (sx, sy) of a buffer of width bw.
+template <bool F32>
+__device__ __forceinline__ uint32_t texel_rgb8(const void* p, int bw, int sx, int sy) {
+    const size_t i = (static_cast<size_t>(sy) * bw + sx) * 3;
+    uint32_t r, g, b;
+    if (F32) {
+        const float* s = static_cast<const float*>(p) + i;
+        r = u16_of(s[0] * 65535.0f);
+        g = u16_of(s[1] * 65535.0f);
+        b = u16_of(s[2] * 65535.0f);
+    } else {
+        const uint16_t* s = static_cast<const uint16_t*>(p) + i;
+        r = s[0];
+        g = s[1];
+        b = s[2];
+    }
+    return (r >> 8) | ((g >> 8) << 8) | ((b >> 8) << 16);
+}
+
+// Four pixels (x .. x+3, y) of one row and one square: one scan of the
+// square's list serves all four.  Each pixel keeps the first chunk (from
+// the end of the list) that writes it, as export_pixel does for one pixel;
+// the scan stops once all four are decided.  The texel loads are then
+// independent of each other.
+template <bool F32>
+__device__ __forceinline__ void export_quad(int x, int y, const XChunk* __restrict__ ch,
+                                            const int* __restrict__ off, const int* __restrict__ list,
+                                            int bins_x, float scale, uint32_t v[4]) {
+    const int bin = (y / kBin) * bins_x + x / kBin;
+    const int lo = off[bin];
+    const void* src[4] = {nullptr, nullptr, nullptr, nullptr};
+    int bw[4] = {0, 0, 0, 0}, sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0};
+    unsigned todo = 0xFu;
+    for (int k = off[bin + 1] - 1; k >= lo && todo; --k) {
+        const XChunk c = ch[list[k]];
+        const int dy = y - c.y0;
+        if (dy < 0 || dy >= c.h) continue;
+        const int ty = static_cast<int>(static_cast<float>(dy) / scale);
+        if (ty < 0 || ty >= c.bh) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int dx = x + j - c.x0;
+            if (!((todo >> j) & 1u) || dx < 0 || dx >= c.w) continue;
+            const int tx = static_cast<int>(static_cast<float>(dx) / scale);
+            if (tx < 0 || tx >= c.bw) continue;
+            src[j] = c.p;
+            bw[j] = c.bw;
+            sx[j] = tx;
+            sy[j] = ty;
+            todo &= ~(1u << j);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = src[j] ? texel_rgb8<F32>(src[j], bw[j], sx[j], sy[j]) : 0u;
+}
+
 template <bool F32>
 __device__ __forceinline__ uint32_t export_pixel(int x, int y, const XChunk* __restrict__ ch,
                                                  const int* __restrict__ off, const int* __restrict__ list,
@@ -59,20 +116,7 @@ __device__ __forceinline__ uint32_t export_pixel(int x, int y, const XChunk* __r
         const int sx = static_cast<int>(static_cast<float>(dx) / scale);
         const int sy = static_cast<int>(static_cast<float>(dy) / scale);
         if (sx < 0 || sx >= c.bw || sy < 0 || sy >= c.bh) continue;
-        const size_t i = (static_cast<size_t>(sy) * c.bw + sx) * 3;
-        uint32_t r, g, b;
-        if (F32) {
-            const float* s = static_cast<const float*>(c.p) + i;
-            r = u16_of(s[0] * 65535.0f);
-            g = u16_of(s[1] * 65535.0f);
-            b = u16_of(s[2] * 65535.0f);
-        } else {
-            const uint16_t* s = static_cast<const uint16_t*>(c.p) + i;
-            r = s[0];
-            g = s[1];
-            b = s[2];
-        }
-        return (r >> 8) | ((g >> 8) << 8) | ((b >> 8) << 16);
+        return texel_rgb8<F32>(c.p, c.bw, sx, sy);
     }
     return 0u;
 }
@@ -90,12 +134,17 @@ __global__ void __launch_bounds__(256) export_kernel(const XChunk* __restrict__ 
         int y = static_cast<int>(p0 / out_w);
         int x = static_cast<int>(p0 - static_cast<size_t>(y) * out_w);
         uint32_t v[4];
+        if (p0 + 4 <= n && x + 3 < out_w && x / kBin == (x + 3) / kBin) {
+            export_quad<F32>(x, y, ch, off, list, bins_x, scale, v);
+        } else {
+            // the quad crosses a row end or a square edge: one pixel at a time
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            v[j] = (p0 + j < n) ? export_pixel<F32>(x, y, ch, off, list, bins_x, scale) : 0u;
-            if (++x == out_w) {
-                x = 0;
-                ++y;
+            for (int j = 0; j < 4; ++j) {
+                v[j] = (p0 + j < n) ? export_pixel<F32>(x, y, ch, off, list, bins_x, scale) : 0u;
+                if (++x == out_w) {
+                    x = 0;
+                    ++y;
+                }
             }
         }
         uint8_t* d = out + 3 * p0;
@@ -194,6 +243,48 @@ int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* c
     (void)i360hip_export_size(total_width, total_height, max_dimension, &out_w, &out_h);
     const int bins_x = (out_w + kBin - 1) / kBin, bins_y = (out_h + kBin - 1) / kBin;
     const size_t nbins = static_cast<size_t>(bins_x) * bins_y;
+    const size_t n = static_cast<size_t>(out_w) * out_h;
+    const bool aligned = (reinterpret_cast<uintptr_t>(rgb8_out) & 3) == 0;
+
+    // The tables depend only on the arguments: a repeat of the last call's
+    // arguments launches on the tables already on the device.
+    uint64_t key = 0x9E3779B97F4A7C15ull;
+    auto mix = [&key](uint64_t v) { key = (key ^ v) * 0x100000001B3ull + (key >> 29); };
+    mix(static_cast<uint64_t>(num_chunks));
+    mix(static_cast<uint64_t>(total_width) << 32 | static_cast<uint32_t>(total_height));
+    mix(static_cast<uint64_t>(max_dimension));
+    for (int c = 0; c < num_chunks; ++c) {
+        const i360hip_export_chunk& k = chunks[c];
+        mix(static_cast<uint64_t>(static_cast<uint32_t>(k.x_offset)) << 32 | static_cast<uint32_t>(k.y_offset));
+        mix(static_cast<uint64_t>(static_cast<uint32_t>(k.width)) << 32 | static_cast<uint32_t>(k.height));
+        mix(reinterpret_cast<uintptr_t>(k.pixels));
+        mix(static_cast<uint64_t>(static_cast<uint32_t>(k.buf_width)) << 32 | static_cast<uint32_t>(k.buf_height));
+    }
+    auto launch = [&](const uint8_t* dev, size_t oO, size_t oL) -> int {
+        const auto* dch = reinterpret_cast<const XChunk*>(dev);
+        const auto* doff = reinterpret_cast<const int*>(dev + oO);
+        const auto* dlist = reinterpret_cast<const int*>(dev + oL);
+        const int grid = grid_for(ctx, (n + 3) / 4);
+        {
+            // algorithmic bytes: one source texel and one RGB8 pixel per output pixel
+            ProfScope ps(ctx, s, "export_composite",
+                         static_cast<double>(n) * ((format == I360HIP_PIX_F32 ? 12 : 6) + 3));
+            if (format == I360HIP_PIX_F32) {
+                if (aligned) export_kernel<true, true><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
+                else export_kernel<true, false><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
+            } else {
+                if (aligned) export_kernel<false, true><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
+                else export_kernel<false, false><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
+            }
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return Fail(ctx, I360HIP_KERNEL_LAUNCH_FAILED, hipGetErrorString(e));
+        return I360HIP_SUCCESS;
+    };
+    if (ctx->export_dev && key == ctx->export_key && static_cast<int>(ctx->bufs.size()) > kSlotExport &&
+        ctx->bufs[kSlotExport].ptr == ctx->export_dev)
+        return launch(static_cast<const uint8_t*>(ctx->export_dev), ctx->export_off[0], ctx->export_off[1]);
+    ctx->export_dev = nullptr;
 
     // chunks that can write: a buffer and a placement meeting the output
     std::vector<XChunk> xc;
@@ -236,34 +327,19 @@ int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* c
     std::memcpy(host.data() + oL, list.data(), sizeof(int) * list.size());
 
     Status st;
-    // the previous call's kernel may still read the tables on this stream
-    if (hipStreamSynchronize(s) != hipSuccess) return Fail(ctx, I360HIP_DEVICE_ERROR, "stream synchronisation failed");
+    // an earlier call's kernel, on any stream, may still read the tables
+    if (hipDeviceSynchronize() != hipSuccess) return Fail(ctx, I360HIP_DEVICE_ERROR, "device synchronisation failed");
     auto* dev = static_cast<uint8_t*>(ctx_scratch(ctx, kSlotExport, bytes, st));
     if (!st.ok()) return Fail(ctx, st.code, st.msg);
     if (hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return Fail(ctx, I360HIP_DEVICE_ERROR, "export table upload failed");
 
-    const size_t n = static_cast<size_t>(out_w) * out_h;
-    const bool aligned = (reinterpret_cast<uintptr_t>(rgb8_out) & 3) == 0;
-    const auto* dch = reinterpret_cast<const XChunk*>(dev + oC);
-    const auto* doff = reinterpret_cast<const int*>(dev + oO);
-    const auto* dlist = reinterpret_cast<const int*>(dev + oL);
-    const int grid = grid_for(ctx, (n + 3) / 4);
-    {
-        // algorithmic bytes: one source texel and one RGB8 pixel per output pixel
-        ProfScope ps(ctx, s, "export_composite", static_cast<double>(n) * ((format == I360HIP_PIX_F32 ? 12 : 6) + 3));
-        if (format == I360HIP_PIX_F32) {
-            if (aligned) export_kernel<true, true><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
-            else export_kernel<true, false><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
-        } else {
-            if (aligned) export_kernel<false, true><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
-            else export_kernel<false, false><<<grid, 256, 0, s>>>(dch, doff, dlist, bins_x, out_w, scale, n, rgb8_out);
-        }
-    }
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return Fail(ctx, I360HIP_KERNEL_LAUNCH_FAILED, hipGetErrorString(e));
-    return I360HIP_SUCCESS;
+    ctx->export_key = key;
+    ctx->export_dev = dev;
+    ctx->export_off[0] = oO;
+    ctx->export_off[1] = oL;
+    return launch(dev, oO, oL);
 }
 
 int i360hip_rgb48_from_render_dev(i360hip_ctx* ctx, const float* rgb, size_t num_values, uint16_t* rgb48,

@@ -355,7 +355,9 @@ typedef struct i360hip_export_chunk {
  * I360HIP_PIX_F32 for this library's FP32 render output, which is first
  * rounded to RGB48 as i360hip_rgb48_from_render_dev does.  rgb8_out:
  * device, out_width * out_height * 3 bytes (i360hip_export_size), every
- * byte written.  Uploads its tables synchronously, then enqueues. */
+ * byte written.  A call whose arguments differ from the previous call's
+ * waits for the device to go idle, uploads its tables and enqueues; a repeat
+ * of the previous arguments only enqueues. */
 int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks,
                                  int total_width, int total_height, int max_dimension, int format,
                                  uint8_t* rgb8_out, void* stream);

@@ -155,3 +155,27 @@ def test_gigapixel_canvas_export(hip):
         hip.check(hip.export_composite_dev(args, tw, th, max_dim, _abi.PIX_U16, out.data_ptr()), "export")
         want = po.export_composite([r + (b,) for r, b in zip(rects, host)], tw, th, max_dim)
         assert np.array_equal(out.cpu().numpy().reshape(oh, ow, 3), want), max_dim
+
+
+def test_repeat_call_reads_current_pixels(hip):
+    """A repeat of the previous call's arguments launches on the tables
+    already on the device; it must still read the buffers' current pixels
+    and write the output it is given."""
+    import torch
+
+    rng = np.random.default_rng(13)
+    chunks = [c for c in _random_layout(rng, 480, 320, 96, False) if c[4] is not None]
+    dev = [_dev(c[4]) for c in chunks]
+    args = [(x, y, w, h, d.data_ptr(), b.shape[1], b.shape[0]) for (x, y, w, h, b), d in zip(chunks, dev)]
+    ow, oh = hip.export_size(480, 320, 200)
+    outs = [torch.empty(ow * oh * 3, dtype=torch.uint8, device="cuda") for _ in range(2)]
+    for k in range(3):
+        if k == 2:
+            fresh = rng.integers(0, 65536, chunks[0][4].shape, dtype=np.uint16)
+            dev[0].copy_(_dev(fresh))
+            chunks[0] = chunks[0][:4] + (fresh,)
+        note(f"export repeat with the same arguments, call {k}")
+        out = outs[k % 2]
+        hip.check(hip.export_composite_dev(args, 480, 320, 200, _abi.PIX_U16, out.data_ptr()), "export")
+        want = po.export_composite(chunks, 480, 320, 200)
+        assert np.array_equal(out.cpu().numpy().reshape(oh, ow, 3), want), k
