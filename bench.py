@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--psnr-chunks", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
-    ap.add_argument("--forms", default="plain,precision,batched,surrogate")
+    ap.add_argument("--forms", default="plain,precision,batched,batched_rgb48,surrogate")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch configuration")
     ap.add_argument("--profile-forms", default="batched",
                     help="forms whose per-kernel event timing is printed (the roofline uses the batched form's)")
@@ -117,7 +117,7 @@ def cpu_baseline(geom, frames_host_fn, chunks, threads, bands):
                       f"BlendChunkContributors pipeline with {bands} bands; {dt:.1f} s wall"}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_gigapixel_batched_hbm_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_gigapixel_batched_rgb48_1group_hbm_traffic.json")
 
 
 def cpu_export(geom, chunks, rgb48, offs, log, max_dim=4096):
@@ -141,16 +141,15 @@ def cpu_export(geom, chunks, rgb48, offs, log, max_dim=4096):
 def measured_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated against a
-    1 GiB fill and copy), when they were taken with the same number of
-    chunk groups (a launch covers one group); else None."""
+    1 GiB fill and copy), taken like the roofline's profiled step with one
+    chunk group (a launch covers every chunk); else None."""
     try:
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
     except (OSError, ValueError):
         return None
     k = prof.get("kernels", {}).get(kernel)
-    groups = (tuning or {}).get("chunk_groups", 4)
-    if not k or groups != 4:
+    if not k or prof.get("chunk_groups") != 1:
         return None
     return k["traffic_bytes_per_launch"]
 
@@ -422,7 +421,13 @@ def main():
                       "out": out.data_ptr() + 4 * int(offs[k]), "owner": None})
         contrib += c.contributors
 
-    def render(mode, descs=cdesc, cont=contrib):
+    # RGB48 outputs (the reference's rendered_chunk_rgb48 payload) for the RGB48-output form
+    out16 = torch.empty(int(offs[-1]), dtype=torch.int16, device=dev)
+    cdesc16 = [dict(d, out=out16.data_ptr() + 2 * int(offs[k])) for k, d in enumerate(cdesc)]
+
+    def render(mode, descs=None, cont=contrib):
+        if descs is None:
+            descs = cdesc16 if mode & _abi.RENDER_OUT_RGB48 else cdesc
         hip.check(hip.render_chunks(fdesc, descs, cont, args.bands, mode, sh), f"render mode {mode}")
 
     host_ms = {}
@@ -457,6 +462,7 @@ def main():
     outputs = {}
     for name, mode, steps in (("plain", _abi.RENDER_PLAIN, 1), ("precision", _abi.RENDER_PRECISION, 1),
                               ("batched", _abi.RENDER_BATCHED, args.steps),
+                              ("batched_rgb48", _abi.RENDER_BATCHED | _abi.RENDER_OUT_RGB48, args.steps),
                               ("surrogate", _abi.RENDER_SURROGATE, args.steps)):
         if name not in wanted and name != "batched":
             continue
@@ -473,6 +479,8 @@ def main():
             f"  (host {host_ms.get(mode, 0.0):.2f} ms per call)")
         if name in ("batched", "surrogate"):
             outputs[name] = out.clone()
+        if name == "batched_rgb48":
+            outputs[name] = out16
 
     # per-kernel event timing of one eager step of a form's launch sequence
     def stage_profile(mode):
@@ -493,7 +501,18 @@ def main():
         log(f"stage profile of the {pname} form:")
         for name, v in sorted(pst.items(), key=lambda kv: -kv[1]["total_ms"]):
             log(f"  {name:18s} {v['launches']:5d} launches {v['total_ms']:9.2f} ms ({100 * v['total_ms'] / ptot:5.1f} %)")
-    stats = stage_profile(_abi.RENDER_BATCHED)
+    # The roofline is taken on the headline form's launch sequence, eager and
+    # with one chunk group: each stage is then one launch over every chunk
+    # with nothing beside it, so its event time is the kernel's own duration.
+    # (In the timed four-group replay the groups' launches overlap, and an
+    # event pair would also count the wait behind the other groups.)
+    ok0 = [n for n in ("batched", "batched_rgb48") if n in forms]
+    prof_mode = (_abi.RENDER_BATCHED | _abi.RENDER_OUT_RGB48
+                 if ok0 and max(ok0, key=lambda n: forms[n]["mp_s"]) == "batched_rgb48" else _abi.RENDER_BATCHED)
+    t0_ = tuning or {}
+    hip.render_set_tuning(1, t0_.get("chamfer_strip", 0), t0_.get("grid_share", 0))
+    stats = stage_profile(prof_mode)
+    hip.render_set_tuning(t0_.get("chunk_groups", 0), t0_.get("chamfer_strip", 0), t0_.get("grid_share", 0))
     tot = sum(v["total_ms"] for v in stats.values())
     for name, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"]):
         gbs = v["alg_bytes"] / (v["total_ms"] / 1e3) / 1e9 if v["total_ms"] > 0 else 0
@@ -505,7 +524,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dom_name, tuning),
-                "avg_launch_us": round(dom["total_ms"] / dom["launches"] * 1e3, 2),
+                "avg_launch_us": round(dom["total_ms"] / dom["launches"] * 1e3, 2), "profiled_chunk_groups": 1,
                 "time_share": round(dom["total_ms"] / tot, 3),
                 "latency_bound_share": round(sum(v["total_ms"] for k, v in stats.items()
                                                  if v["alg_bytes"] == 0) / tot, 3)}
@@ -525,7 +544,9 @@ def main():
         torch.cuda.synchronize()
         b = ref_out[: c.area * 3].double()
         for name, o in outputs.items():
-            e = o[int(offs[k]): int(offs[k + 1])].double() - b
+            seg = o[int(offs[k]): int(offs[k + 1])]
+            seg = (seg.to(torch.int32) & 0xFFFF).double() / 65535.0 if seg.dtype == torch.int16 else seg.double()
+            e = seg - b
             acc[name][0] += float((e * e).sum())
             acc[name][1] += e.numel()
             acc[name][2] = max(acc[name][2], float(e.abs().max()))
@@ -538,7 +559,8 @@ def main():
     map_err = hip.render_map_error(fdesc, cdesc, contrib)
     log(f"FP32 tile map vs FP64 map: {map_err:.2e} px")
     # headline: the fastest of the batched / surrogate forms that holds 50 dB
-    ok = [n for n in ("batched", "surrogate") if n in forms and forms[n].get("psnr_db_vs_fp64", 0) >= 50.0]
+    ok = [n for n in ("batched", "batched_rgb48", "surrogate")
+          if n in forms and forms[n].get("psnr_db_vs_fp64", 0) >= 50.0]
     head_name = max(ok, key=lambda n: forms[n]["mp_s"]) if ok else "batched"
     head = forms[head_name]
     psnr, maxerr = head.get("psnr_db_vs_fp64"), head.get("max_abs_err_vs_fp64")
@@ -579,9 +601,11 @@ def main():
         "config": {"workload": "gigapixel panorama render: 48 FP16 RGB frames of 6000x4000 (6x8 grid, 25% overlap, "
                                "focal 5000 px, +-2 deg yaw/pitch jitter, FP64 homographies), every 4096^2 chunk of "
                                "the canvas, warp + chamfer ownership + DP seams + 7-level Laplacian blend",
-                   "form": head_name + (" (persistent per-stage launches over all chunks, HIP graph)"
-                                        if head_name == "batched" else
-                                        " (batched + learned low band at level bands-3, per-tile fallback)"),
+                   "form": head_name + {
+                       "batched": " (persistent per-stage launches over all chunks, HIP graph; FP32 RGB out)",
+                       "batched_rgb48": " (batched, writing the RGB48 rendered_chunk_rgb48 payload the reference "
+                                        "stores, u16(clamp(v*65535)+0.5))",
+                       "surrogate": " (batched + learned low band at level bands-3, per-tile fallback)"}[head_name],
                    "bands": args.bands, "chunks": len(chunks), "output_mp": round(out_px / 1e6, 1),
                    "canvas": [geom.canvas_w, geom.canvas_h],
                    "forms": forms,

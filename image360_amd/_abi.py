@@ -59,6 +59,7 @@ class ExportChunk(C.Structure):
 PIX_F16, PIX_F32, PIX_U16 = 0, 1, 2
 RENDER_FP64, RENDER_PLAIN, RENDER_PRECISION, RENDER_BATCHED, RENDER_SURROGATE = 0, 1, 2, 3, 4
 RENDER_NAMES = {0: "fp64", 1: "plain", 2: "precision", 3: "batched", 4: "surrogate"}
+RENDER_OUT_RGB48 = 0x100  # OR-ed into a render mode: outputs are RGB48 (uint16) blob payloads
 
 
 # name -> (restype, argtypes)

@@ -46,7 +46,8 @@ struct RChunk {
     int lw[kMaxLevels], lh[kMaxLevels];
     int* boxes;          // count * 4 (the items' box pointers point here)
     int32_t* owner;      // w*h
-    float* out;          // FP32 RGB, w*h*3 (caller's)
+    float* out;          // FP32 RGB, w*h*3 (caller's); RGB48 when out16
+    int out16;           // out holds uint16 RGB48 (the reference's output rounding of v * 65535)
     int32_t* owner_out;  // optional (caller's)
     void* recon[kMaxLevels];  // level l >= 1 reconstruction, RGBA in the math type
     int* lohi;

@@ -1429,12 +1429,26 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                 }
             }
             if (ok[k]) {
-                auto* dst = gptr(ch.out + p[k] * 3);
+                float f[3];
 #pragma unroll
                 for (int c3 = 0; c3 < 3; ++c3) {
                     M t = v[c3] < M(1) ? v[c3] : M(1);
                     t = M(0) < t ? t : M(0);
-                    dst[c3] = o[k] < 0 ? 0.0f : static_cast<float>(t);
+                    f[c3] = o[k] < 0 ? 0.0f : static_cast<float>(t);
+                }
+                if (ch.out16) {
+                    // RGB48 payload: u16(clamp(v * 65535) + 0.5) of the FP32 value (SeamBlend.cpp:529-534)
+                    auto* d16 = gptr(reinterpret_cast<uint16_t*>(ch.out) + p[k] * 3);
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        const float w = f[c3] * 65535.0f;
+                        const float lo = (w < 65535.0f) ? w : 65535.0f;
+                        d16[c3] = static_cast<uint16_t>(static_cast<unsigned int>(((0.0f < lo) ? lo : 0.0f) + 0.5f));
+                    }
+                } else {
+                    auto* dst = gptr(ch.out + p[k] * 3);
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; ++c3) dst[c3] = f[c3];
                 }
             }
         }
@@ -1543,7 +1557,7 @@ uint64_t mix(uint64_t h, uint64_t v) {
 
 // Builds the plan (host tables + workspace layout); returns false on error.
 static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc, const int* contrib,
-                       int bands, int mode, Plan& P, Status& st, int surr = kSurrNone) {
+                       int bands, int mode, Plan& P, Status& st, int surr = kSurrNone, bool out16 = false) {
     P = Plan{};
     P.mode = mode;
     P.surr = (mode == kModeBatched) ? surr : kSurrNone;
@@ -1553,6 +1567,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     uint64_t key = mix(0, static_cast<uint64_t>(mode));
     key = mix(key, static_cast<uint64_t>(bands));
     key = mix(key, static_cast<uint64_t>(P.surr));
+    key = mix(key, out16 ? 1u : 0u);
     // layout offsets are relative; absolute pointers are patched after allocation
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
@@ -1580,6 +1595,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         RChunk ch{};
         ch.x = cc.x; ch.y = cc.y; ch.w = cc.width; ch.h = cc.height;
         ch.out = cc.out;
+        ch.out16 = out16 ? 1 : 0;
         ch.owner_out = cc.owner;
         ch.levels = ClampBands(cc.width, cc.height, bands);
         ch.lw[0] = cc.width; ch.lh[0] = cc.height;
@@ -2091,7 +2107,8 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
            rsexport_kernel<MODE><<<persistent_grid(ctx, P.sfeat.n), 256, 0, s>>>(T.chunks, T.tiles + P.sfeat.off, P.sfeat.n, sr->bufs));
     if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
-    RL(st, "final_level0", area0 * (4.0 + 4 * sS + 12.0),
+    const double out_bytes = (c1 > c0 && P.chunks[c0].out16) ? 6.0 : 12.0;  // RGB48 or FP32 RGB per pixel
+    RL(st, "final_level0", area0 * (4.0 + 4 * sS + out_bytes),
        rfinal_kernel<MODE><<<persistent_grid(ctx, n, 8), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
 }
 
@@ -2300,8 +2317,10 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
                        const int* contrib, int bands, int api_mode, hipStream_t s, Status& st,
                        float* export_feat = nullptr, float* export_target = nullptr) {
     RenderCache& rc = cache_of(ctx);
-    const int surr = export_feat ? kSurrExport : (api_mode == I360HIP_RENDER_SURROGATE ? kSurrRender : kSurrNone);
-    const int mode = surr != kSurrNone ? kModeBatched : api_mode;
+    const bool out16 = (api_mode & I360HIP_RENDER_OUT_RGB48) != 0;
+    const int form = api_mode & ~I360HIP_RENDER_OUT_RGB48;
+    const int surr = export_feat ? kSurrExport : (form == I360HIP_RENDER_SURROGATE ? kSurrRender : kSurrNone);
+    const int mode = surr != kSurrNone ? kModeBatched : form;
     if (surr == kSurrRender && !ctx->surr_weights) {
         st.set(I360HIP_INVALID_PARAM, "the SURROGATE form needs parameters (i360hip_surrogate_set)");
         return;
@@ -2313,7 +2332,7 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
         return;
     }
     Plan P;
-    if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st, surr)) return;
+    if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st, surr, out16)) return;
     SurrRun sr;
     sr.mode = surr;
     sr.weights = ctx->surr_weights;
@@ -2415,7 +2434,8 @@ int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num
         std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument.");
         return I360HIP_INVALID_PARAM;
     }
-    if (num_bands < 1 || num_bands > I360HIP_MAX_BANDS || mode < I360HIP_RENDER_FP64 || mode > I360HIP_RENDER_SURROGATE) {
+    const int form = mode & ~I360HIP_RENDER_OUT_RGB48;
+    if (num_bands < 1 || num_bands > I360HIP_MAX_BANDS || form < I360HIP_RENDER_FP64 || form > I360HIP_RENDER_SURROGATE) {
         std::snprintf(ctx->last_error, sizeof(ctx->last_error), "numBands must be 1-12 and mode a known form.");
         return I360HIP_INVALID_PARAM;
     }

@@ -184,6 +184,13 @@ enum {
     I360HIP_RENDER_BATCHED = 3,
     I360HIP_RENDER_SURROGATE = 4
 };
+/* OR-ed into `mode`: the chunks' `out` receive RGB48 (uint16_t, width*height*3)
+ * instead of FP32 RGB, each value u16(clamp(v * 65535) + 0.5) of the FP32
+ * value, the rounding of the reference's blend output (SeamBlend.cpp:529-534)
+ * and the pixel payload of a rendered_chunk_rgb48 blob
+ * (RenderExecutor.cpp:138-142).  Equal to i360hip_rgb48_from_render_dev
+ * applied to the FP32 output. */
+#define I360HIP_RENDER_OUT_RGB48 0x100
 
 typedef struct i360hip_frame {
     const void* pixels; /* device, width*height*3 values */
@@ -197,7 +204,7 @@ typedef struct i360hip_frame {
 typedef struct i360hip_chunk {
     int x, y, width, height; /* canvas rectangle */
     int first, count;        /* contributor indices: contributors[first .. first+count-1] */
-    float* out;              /* device FP32 RGB, width*height*3 */
+    float* out;              /* device FP32 RGB, width*height*3 (RGB48 with I360HIP_RENDER_OUT_RGB48) */
     int32_t* owner;          /* optional device owner map (index into the chunk's list), or NULL */
 } i360hip_chunk;
 

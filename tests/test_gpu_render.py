@@ -220,3 +220,20 @@ def test_batched_replay_follows_changed_arguments(hip, case):
         assert any(not np.array_equal(want[k], ref[k]) for k in range(len(g.chunks)))
     finally:
         g.gains[:] = keep
+
+
+@pytest.mark.parametrize("mode", [_abi.RENDER_PLAIN, _abi.RENDER_PRECISION, _abi.RENDER_BATCHED])
+def test_rgb48_output_is_the_rounded_float_output(hip, case, mode):
+    # with RENDER_OUT_RGB48 the chunk outputs hold the RGB48 blob payload:
+    # the FP32 output rounded as the reference rounds its blend output
+    g, frames, dev = case
+    note(f"render RGB48 output, mode {mode}")
+    f32, own = _gpu_render(hip, g, dev, mode)
+    raw, own16 = _gpu_render(hip, g, dev, mode | _abi.RENDER_OUT_RGB48)
+    for k, c in enumerate(g.chunks):
+        got = raw[k].view(np.uint16).reshape(-1)[: c.h * c.w * 3].reshape(c.h, c.w, 3)
+        assert np.array_equal(got, po.rgb48_from_render(f32[k])), k
+        assert np.array_equal(own16[k], own[k]), k
+    again, _ = _gpu_render(hip, g, dev, mode)  # back to FP32 outputs: a fresh plan, not the RGB48 graph
+    for k in range(len(g.chunks)):
+        assert np.array_equal(again[k], f32[k]), k
