@@ -165,17 +165,29 @@ __global__ void __launch_bounds__(256) export_kernel(const XChunk* __restrict__ 
 
 // FP32 RGB in [0, 1] -> RGB48 by the reference's output rounding; four
 // values per thread where the arrays allow it.
+__device__ __forceinline__ uint2 rgb48_of4(const float4 v) {
+    uint2 o;
+    o.x = u16_of(v.x * 65535.0f) | (u16_of(v.y * 65535.0f) << 16);
+    o.y = u16_of(v.z * 65535.0f) | (u16_of(v.w * 65535.0f) << 16);
+    return o;
+}
+
 __global__ void __launch_bounds__(256) rgb48_kernel(const float* __restrict__ in, uint16_t* __restrict__ out,
                                                     size_t n) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     const size_t n4 = n / 4;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        const float4 v = reinterpret_cast<const float4*>(in)[i];
-        uint2 o;
-        o.x = u16_of(v.x * 65535.0f) | (u16_of(v.y * 65535.0f) << 16);
-        o.y = u16_of(v.z * 65535.0f) | (u16_of(v.w * 65535.0f) << 16);
-        reinterpret_cast<uint2*>(out)[i] = o;
+    const auto* in4 = reinterpret_cast<const float4*>(in);
+    auto* out2 = reinterpret_cast<uint2*>(out);
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // four independent 16-byte loads in flight per thread
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 v0 = in4[i], v1 = in4[i + stride], v2 = in4[i + 2 * stride], v3 = in4[i + 3 * stride];
+        out2[i] = rgb48_of4(v0);
+        out2[i + stride] = rgb48_of4(v1);
+        out2[i + 2 * stride] = rgb48_of4(v2);
+        out2[i + 3 * stride] = rgb48_of4(v3);
     }
+    for (; i < n4; i += stride) out2[i] = rgb48_of4(in4[i]);
     for (size_t i = 4 * n4 + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
         out[i] = static_cast<uint16_t>(u16_of(in[i] * 65535.0f));
 }
