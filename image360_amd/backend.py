@@ -346,6 +346,23 @@ class HipComputeBackend:
         return lib().i360hip_export_composite_dev(self._ctx, arr, len(chunks), total_w, total_h, max_dim, fmt,
                                                   out_ptr, stream)
 
+    def ExportComposite(self, chunks, total_w, total_h, max_dim, out):
+        """Host-buffer composite: chunks = [(x_offset, y_offset, width, height, buf or None)] with buf an
+        (h, w, 3) uint16 (RGB48) or float32 array, all of one dtype; out: (out_h, out_w, 3) uint8."""
+        if (rc := self._ready()) != SUCCESS:
+            return rc
+        bufs = [None if c[4] is None else np.ascontiguousarray(c[4]) for c in chunks]
+        kinds = {b.dtype for b in bufs if b is not None}
+        if len(kinds) > 1:
+            raise TypeError("all chunk buffers of one call must share a dtype")
+        fmt = _abi.PIX_F32 if kinds == {np.dtype(np.float32)} else _abi.PIX_U16
+        arr = (_abi.ExportChunk * max(1, len(chunks)))()
+        for i, (c, b) in enumerate(zip(chunks, bufs)):
+            arr[i] = _abi.ExportChunk(c[0], c[1], c[2], c[3], None if b is None else b.ctypes.data,
+                                      0 if b is None else b.shape[1], 0 if b is None else b.shape[0])
+        out = _c_contig(out, np.uint8)
+        return lib().i360hip_export_composite(self._ctx, arr, len(chunks), total_w, total_h, max_dim, fmt, _ptr(out))
+
     @_ordered
     def rgb48_from_render_dev(self, rgb_ptr, num_values, out_ptr, stream=None) -> int:
         return lib().i360hip_rgb48_from_render_dev(self._ctx, rgb_ptr, num_values, out_ptr, stream)

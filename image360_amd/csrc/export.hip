@@ -31,6 +31,7 @@ void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st);
 namespace {
 
 constexpr int kSlotExport = 20;
+constexpr int kSlotExportStage = 23;  // host-buffer calls: staged chunks and output
 constexpr int kBin = 32;  // output square edge of the chunk lists
 
 struct XChunk {
@@ -233,10 +234,11 @@ int i360hip_export_size(int total_width, int total_height, int max_dimension, in
     return I360HIP_SUCCESS;
 }
 
-int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks,
-                                 int total_width, int total_height, int max_dimension, int format,
-                                 uint8_t* rgb8_out, void* stream) {
-    if (!ctx) return I360HIP_DEVICE_ERROR;
+}  // extern "C"
+
+namespace {
+int export_validate(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks, int total_width,
+                    int total_height, int max_dimension, int format, const uint8_t* rgb8_out) {
     if (max_dimension <= 0) return Fail(ctx, I360HIP_INVALID_PARAM, "max_dimension must be positive");
     if (total_width <= 0 || total_height <= 0) return Fail(ctx, I360HIP_INVALID_PARAM, "empty panorama");
     if (num_chunks < 0 || (num_chunks > 0 && !chunks)) return Fail(ctx, I360HIP_INVALID_PARAM, "bad chunk list");
@@ -246,10 +248,12 @@ int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* c
     for (int c = 0; c < num_chunks; ++c)
         if (chunks[c].pixels && (chunks[c].buf_width < 0 || chunks[c].buf_height < 0))
             return Fail(ctx, I360HIP_INVALID_PARAM, "negative buffer size");
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    (void)hipSetDevice(ctx->device);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    return I360HIP_SUCCESS;
+}
 
+// The composite on device buffers; the caller holds ctx->mu.
+int export_dev_locked(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks, int total_width,
+                      int total_height, int max_dimension, int format, uint8_t* rgb8_out, hipStream_t s) {
     const float scale = i360hip_export_scale(total_width, total_height, max_dimension);
     int out_w = 0, out_h = 0;
     (void)i360hip_export_size(total_width, total_height, max_dimension, &out_w, &out_h);
@@ -352,6 +356,65 @@ int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* c
     ctx->export_off[0] = oO;
     ctx->export_off[1] = oL;
     return launch(dev, oO, oL);
+}
+}  // namespace
+
+extern "C" {
+
+int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks,
+                                 int total_width, int total_height, int max_dimension, int format,
+                                 uint8_t* rgb8_out, void* stream) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    const int v = export_validate(ctx, chunks, num_chunks, total_width, total_height, max_dimension, format, rgb8_out);
+    if (v != I360HIP_SUCCESS) return v;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    return export_dev_locked(ctx, chunks, num_chunks, total_width, total_height, max_dimension, format, rgb8_out, s);
+}
+
+int i360hip_export_composite(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks, int total_width,
+                             int total_height, int max_dimension, int format, uint8_t* rgb8_out) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    const int v = export_validate(ctx, chunks, num_chunks, total_width, total_height, max_dimension, format, rgb8_out);
+    if (v != I360HIP_SUCCESS) return v;
+    int out_w = 0, out_h = 0;
+    (void)i360hip_export_size(total_width, total_height, max_dimension, &out_w, &out_h);
+    const size_t esz = format == I360HIP_PIX_F32 ? 12 : 6;
+    const size_t out_bytes = static_cast<size_t>(out_w) * out_h * 3;
+    auto a256 = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    // stage the chunk buffers and the output in one block of the staging slot
+    std::vector<size_t> at(num_chunks, 0);
+    size_t bytes = a256(out_bytes);
+    for (int c = 0; c < num_chunks; ++c) {
+        at[c] = bytes;
+        if (chunks[c].pixels)
+            bytes += a256(static_cast<size_t>(chunks[c].buf_width) * chunks[c].buf_height * esz);
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    Status st;
+    // the device copies take other addresses than any earlier call's, so
+    // the tables are rebuilt; wait for earlier work before reusing the slot
+    if (hipDeviceSynchronize() != hipSuccess) return Fail(ctx, I360HIP_DEVICE_ERROR, "device synchronisation failed");
+    auto* base = static_cast<uint8_t*>(ctx_scratch(ctx, kSlotExportStage, bytes, st));
+    if (!st.ok()) return Fail(ctx, st.code, st.msg);
+    std::vector<i360hip_export_chunk> dc(chunks, chunks + num_chunks);
+    for (int c = 0; c < num_chunks; ++c) {
+        if (!chunks[c].pixels) continue;
+        const size_t b = static_cast<size_t>(chunks[c].buf_width) * chunks[c].buf_height * esz;
+        if (b && hipMemcpyAsync(base + at[c], chunks[c].pixels, b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            return Fail(ctx, I360HIP_DEVICE_ERROR, "chunk upload failed");
+        dc[c].pixels = base + at[c];
+    }
+    ctx->export_dev = nullptr;  // staged addresses may repeat with other contents
+    const int r = export_dev_locked(ctx, dc.data(), num_chunks, total_width, total_height, max_dimension, format, base,
+                                    ctx->stream);
+    if (r != I360HIP_SUCCESS) return r;
+    if (hipMemcpyAsync(rgb8_out, base, out_bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return Fail(ctx, I360HIP_DEVICE_ERROR, "composite download failed");
+    return I360HIP_SUCCESS;
 }
 
 int i360hip_rgb48_from_render_dev(i360hip_ctx* ctx, const float* rgb, size_t num_values, uint16_t* rgb48,

@@ -368,6 +368,12 @@ typedef struct i360hip_export_chunk {
 int i360hip_export_composite_dev(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks,
                                  int total_width, int total_height, int max_dimension, int format,
                                  uint8_t* rgb8_out, void* stream);
+/* The same composite on host buffers, as ExportPreviewJpeg holds them (the
+ * PixelBuffers read back from storage, PanoramaExporter.cpp:68): chunks'
+ * `pixels` and rgb8_out are host memory.  Stages the buffers in device
+ * memory, composites and copies the result back; synchronous. */
+int i360hip_export_composite(i360hip_ctx* ctx, const i360hip_export_chunk* chunks, int num_chunks, int total_width,
+                             int total_height, int max_dimension, int format, uint8_t* rgb8_out);
 
 /* FP32 RGB render output in [0,1] -> the RGB48 values the reference's blend
  * stores (SeamBlend.cpp:529-534: clamp to [0,65535], then u16(v + 0.5)),

@@ -179,3 +179,17 @@ def test_repeat_call_reads_current_pixels(hip):
         hip.check(hip.export_composite_dev(args, 480, 320, 200, _abi.PIX_U16, out.data_ptr()), "export")
         want = po.export_composite(chunks, 480, 320, 200)
         assert np.array_equal(out.cpu().numpy().reshape(oh, ow, 3), want), k
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_host_buffer_composite(hip, f32):
+    """The host-buffer entry point (PixelBuffers as ExportPreviewJpeg reads
+    them) gives the oracle's composite too."""
+    rng = np.random.default_rng(21 + f32)
+    chunks = _random_layout(rng, 640, 410, 128, f32)
+    for max_dim in (640, 211):
+        note(f"export host buffers f32={f32} max_dim={max_dim}")
+        ow, oh = hip.export_size(640, 410, max_dim)
+        out = np.full((oh, ow, 3), 0xCD, np.uint8)
+        hip.check(hip.ExportComposite(chunks, 640, 410, max_dim, out), "ExportComposite")
+        assert np.array_equal(out, po.export_composite(chunks, 640, 410, max_dim)), max_dim
