@@ -8,7 +8,9 @@
 // products stay FP32.  The input tile (18x18 with halo, zero outside the
 // image) and the BF16 weights (widened to FP32 by a 16-bit shift) are
 // staged in LDS; each wavefront accumulates four output rows (four 16-pixel
-// M blocks) over all output channels.  The backward data pass is the same
+// M blocks) over all output channels.  The grid is persistent: a block
+// stages the weights once and loads the next tile's inputs into registers
+// while the current tile's MFMAs run.  The backward data pass is the same
 // kernel on the ReLU-masked output gradient with the weights flipped and
 // transposed; the weight gradient is a per-tile partial sum (LDS-staged
 // input and output-gradient tiles) reduced across tiles in a fixed order,
@@ -80,90 +82,122 @@ __device__ __forceinline__ void stage_tile(float* lds, const float* __restrict__
 typedef float surr_f4 __attribute__((ext_vector_type(4)));
 template <int CIN, int COUT, bool BIAS, bool RELU>
 __global__ void __launch_bounds__(256) surr_conv_kernel(const SImg* __restrict__ imgs, const STile* __restrict__ tiles,
-                                                        const float* __restrict__ in, const float* __restrict__ mask,
-                                                        float* __restrict__ out, const uint32_t* __restrict__ wpk,
+                                                        int ntiles, const float* __restrict__ in,
+                                                        const float* __restrict__ mask, float* __restrict__ out,
+                                                        const uint32_t* __restrict__ wpk,
                                                         const float* __restrict__ bias) {
     static_assert(CIN % 4 == 0, "four input channels per MFMA step");
     constexpr int CP = CIN + 2;             // LDS channel pitch: the 16 pixels of a read spread over the banks
     constexpr int NB = (COUT + 15) / 16;    // 16-channel output blocks
     constexpr int NP = NB * 16;             // output channels, padded
+    constexpr int NV = kTH * kTH * (CIN / 4);  // float4 groups of one input tile
+    constexpr int PER = (NV + 255) / 256;      // of them per thread
     __shared__ float xs[kTH * kTH * CP];
     __shared__ uint16_t ws[9 * CIN * NP];   // BF16 weights [tap * CIN + ci][co], zero for co >= COUT
-    const STile tl = tiles[blockIdx.x];
-    const SImg im = imgs[tl.img];
-    const int x0 = tl.tx * kT, y0 = tl.ty * kT;
-    // input rows [y0-1, y0+17) x cols [x0-1, x0+17); 0 outside the image and the mask
-    for (int i = threadIdx.x; i < kTH * kTH * (CIN / 4); i += blockDim.x) {
-        const int c4 = i % (CIN / 4), pc = i / (CIN / 4);
-        const int col = pc % kTH, row = pc / kTH;
-        const int x = x0 - 1 + col, y = y0 - 1 + row;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x >= 0 && y >= 0 && x < im.w && y < im.h) {
-            const size_t p = static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x;
-            v = *reinterpret_cast<const float4*>(in + p * CIN + 4 * c4);
-            if (mask) {
-                const float4 m = *reinterpret_cast<const float4*>(mask + p * CIN + 4 * c4);
-                v.x = m.x > 0.f ? v.x : 0.f;
-                v.y = m.y > 0.f ? v.y : 0.f;
-                v.z = m.z > 0.f ? v.z : 0.f;
-                v.w = m.w > 0.f ? v.w : 0.f;
-            }
-        }
-        float* d = xs + pc * CP + 4 * c4;  // 8-byte aligned (CP even)
-        *reinterpret_cast<float2*>(d) = make_float2(v.x, v.y);
-        *reinterpret_cast<float2*>(d + 2) = make_float2(v.z, v.w);
-    }
-    // weights: BF16 pairs wpk[(tap * CIN + ci) * COUT / 2 + co / 2], low half = even co
+    // weights, once per block: BF16 pairs wpk[(tap * CIN + ci) * COUT / 2 + co / 2], low half = even co
     const auto* w16 = reinterpret_cast<const uint16_t*>(wpk);
     for (int i = threadIdx.x; i < 9 * CIN * NP; i += blockDim.x) {
         const int kk = i / NP, n = i % NP;
         ws[i] = n < COUT ? w16[kk * COUT + n] : uint16_t(0);
     }
-    __syncthreads();
+    // input rows [y0-1, y0+17) x cols [x0-1, x0+17) of tile t into registers;
+    // 0 outside the image and the mask
+    float4 pre[PER];
+    auto fetch = [&](int t) {
+        const STile tl = tiles[t];
+        const SImg im = imgs[tl.img];
+        const int x0 = tl.tx * kT, y0 = tl.ty * kT;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            const int c4 = i % (CIN / 4), pc = i / (CIN / 4);
+            const int x = x0 - 1 + pc % kTH, y = y0 - 1 + pc / kTH;
+            if (i < NV && x >= 0 && y >= 0 && x < im.w && y < im.h) {
+                const size_t p = static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x;
+                v = *reinterpret_cast<const float4*>(in + p * CIN + 4 * c4);
+                if (mask) {
+                    const float4 m = *reinterpret_cast<const float4*>(mask + p * CIN + 4 * c4);
+                    v.x = m.x > 0.f ? v.x : 0.f;
+                    v.y = m.y > 0.f ? v.y : 0.f;
+                    v.z = m.z > 0.f ? v.z : 0.f;
+                    v.w = m.w > 0.f ? v.w : 0.f;
+                }
+            }
+            pre[j] = v;
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (i >= NV) continue;
+            float* d = xs + (i / (CIN / 4)) * CP + 4 * (i % (CIN / 4));  // 8-byte aligned (CP even)
+            *reinterpret_cast<float2*>(d) = make_float2(pre[j].x, pre[j].y);
+            *reinterpret_cast<float2*>(d + 2) = make_float2(pre[j].z, pre[j].w);
+        }
+    };
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    surr_f4 acc[4][NB];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[m][nb] = surr_f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-    for (int tap = 0; tap < 9; ++tap) {
-        const int ky = tap / 3, kx = tap % 3;
-        // this lane's A column: pixel li of row (4 wave + m), tap (ky, kx)
-        const float* xa = xs + ((4 * wave + ky) * kTH + li + kx) * CP + lk;
-        const uint16_t* wb = ws + (tap * CIN + lk) * NP + li;
-#pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 4) {
-            float bv[NB];
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) bv[nb] = __uint_as_float(static_cast<uint32_t>(wb[c0 * NP + nb * 16]) << 16);
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const float av = xa[m * kTH * CP + c0];
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-                    acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nb], acc[m][nb], 0, 0, 0);
-            }
-        }
+    int t = blockIdx.x;
+    if (t < ntiles) {
+        fetch(t);
+        commit();
     }
+    __syncthreads();
+    // persistent: the next tile's inputs are in flight while this tile's MFMAs run
+    for (; t < ntiles; t += gridDim.x) {
+        const int tn = t + static_cast<int>(gridDim.x);
+        if (tn < ntiles) fetch(tn);
+        surr_f4 acc[4][NB];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const int y = y0 + 4 * wave + m;
+        for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int co = nb * 16 + li;
-            if (co >= COUT || y >= im.h) continue;
-            const float bc = BIAS ? bias[co] : 0.0f;
+            for (int nb = 0; nb < NB; ++nb) acc[m][nb] = surr_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+            // this lane's A column: pixel li of row (4 wave + m), tap (ky, kx)
+            const float* xa = xs + ((4 * wave + ky) * kTH + li + kx) * CP + lk;
+            const uint16_t* wb = ws + (tap * CIN + lk) * NP + li;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = x0 + 4 * lk + r;
-                if (x >= im.w) continue;
-                float v = acc[m][nb][r] + bc;
-                if (RELU) v = fmaxf(v, 0.f);
-                out[(static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x) * COUT + co] = v;
+            for (int c0 = 0; c0 < CIN; c0 += 4) {
+                float bv[NB];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) bv[nb] = __uint_as_float(static_cast<uint32_t>(wb[c0 * NP + nb * 16]) << 16);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float av = xa[m * kTH * CP + c0];
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nb], acc[m][nb], 0, 0, 0);
+                }
             }
         }
+        const STile tl = tiles[t];
+        const SImg im = imgs[tl.img];
+        const int x0 = tl.tx * kT, y0 = tl.ty * kT;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int y = y0 + 4 * wave + m;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + li;
+                if (co >= COUT || y >= im.h) continue;
+                const float bc = BIAS ? bias[co] : 0.0f;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int x = x0 + 4 * lk + r;
+                    if (x >= im.w) continue;
+                    float v = acc[m][nb][r] + bc;
+                    if (RELU) v = fmaxf(v, 0.f);
+                    out[(static_cast<size_t>(im.pix0) + static_cast<size_t>(y) * im.w + x) * COUT + co] = v;
+                }
+            }
+        }
+        __syncthreads();  // every wave is done reading this tile from LDS
+        if (tn < ntiles) commit();
+        __syncthreads();
     }
 }
 
@@ -332,8 +366,23 @@ Batch make_batch(int n, int h, int w) {
 template <int CIN, int COUT, bool BIAS, bool RELU>
 void conv(const SImg* imgs, const STile* tiles, int ntiles, const float* in, const float* mask, float* out,
           const uint16_t* w, const float* bias, hipStream_t s) {
-    surr_conv_kernel<CIN, COUT, BIAS, RELU><<<ntiles, 256, 0, s>>>(imgs, tiles, in, mask, out,
-                                                                   reinterpret_cast<const uint32_t*>(w), bias);
+    if (ntiles <= 0) return;
+    // persistent grid: as many blocks as can be resident at once (LDS and
+    // registers), per CU; a block that had to wait would run as a tail
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, surr_conv_kernel<CIN, COUT, BIAS, RELU>, 256, 0) !=
+                hipSuccess || per_cu <= 0)
+            per_cu = 1;
+        resident = cus * per_cu;
+    }
+    const int grid = std::min(ntiles, resident);
+    surr_conv_kernel<CIN, COUT, BIAS, RELU><<<grid, 256, 0, s>>>(imgs, tiles, ntiles, in, mask, out,
+                                                                 reinterpret_cast<const uint32_t*>(w), bias);
 }
 
 // Forward through all layers; act[j] receives layer j's output.
