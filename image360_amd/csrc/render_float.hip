@@ -1178,10 +1178,11 @@ __global__ void rscflag_init_kernel(const RChunk* __restrict__ chunks, int nch, 
 // One block per 16x16 surrogate tile: fall back when the mean predicted
 // squared error of the low band (10^out[3]) exceeds `thr`.
 __global__ void __launch_bounds__(256) rsflag_kernel(const SurrImg* __restrict__ imgs,
-                                                     const SurrTile* __restrict__ tiles, SurrBufs sb, float thr) {
+                                                     const SurrTile* __restrict__ tiles, int t0, SurrBufs sb,
+                                                     float thr) {
     __shared__ float ssum[256];
     __shared__ int scnt[256];
-    const SurrTile tl = tiles[blockIdx.x];
+    const SurrTile tl = tiles[t0 + blockIdx.x];
     const SurrImg im = imgs[tl.img];
     const int x = tl.tx * 16 + (threadIdx.x & 15), y = tl.ty * 16 + (threadIdx.x >> 4);
     float e = 0.f;
@@ -1203,7 +1204,7 @@ __global__ void __launch_bounds__(256) rsflag_kernel(const SurrImg* __restrict__
     }
     if (threadIdx.x == 0) {
         const int f = ssum[0] > thr * static_cast<float>(max(scnt[0], 1)) ? 1 : 0;
-        sb.tflag[blockIdx.x] = f;
+        sb.tflag[t0 + blockIdx.x] = f;
         if (f) atomicOr(&sb.cflag[im.tag], 1);
     }
 }
@@ -1496,6 +1497,8 @@ struct Plan {
     std::vector<SurrImg> simgs;      // one per chunk with a surrogate level
     std::vector<SurrTile> stiles;    // 16x16 tiles of those images
     Range sfeat;                     // 64x4 level-K tiles (RTile, id = chunk)
+    std::vector<Range> sfeat_c;      // ... of each chunk (in chunk order, within sfeat)
+    std::vector<Range> stile_c;      // each chunk's 16x16 surrogate tiles (in chunk order)
     SurrBufs sbufs{};                // offsets until patched
 };
 
@@ -1700,8 +1703,10 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     }
     if (P.surr != kSurrNone) {
         // level-K images of every chunk with a surrogate level, side by side
+        P.stile_c.assign(P.chunks.size(), Plan::Range{0, 0});
         for (size_t c = 0; c < P.chunks.size(); ++c) {
             RChunk& ch = P.chunks[c];
+            P.stile_c[c].off = static_cast<int>(P.stiles.size());
             if (ch.sk < 1) continue;
             const int w = ch.lw[ch.sk], h = ch.lh[ch.sk];
             ch.spix0 = P.snpix;
@@ -1711,6 +1716,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             P.simgs.push_back(SurrImg{P.snpix, w, h, static_cast<int>(c)});
             for (int ty = 0; ty < (h + 15) / 16; ++ty)
                 for (int tx = 0; tx < ch.stx; ++tx) P.stiles.push_back(SurrTile{img, tx, ty, 0});
+            P.stile_c[c].n = static_cast<int>(P.stiles.size()) - P.stile_c[c].off;
             P.snpix += w * h;
         }
         const size_t px = static_cast<size_t>(std::max(1, P.snpix));
@@ -1803,9 +1809,12 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             P.comp[c][l].n = static_cast<int>(P.tiles.size()) - P.comp[c][l].off;
         }
     P.sfeat.off = static_cast<int>(P.tiles.size());
+    P.sfeat_c.assign(nch, Plan::Range{0, 0});
     for (int c = 0; c < nch; ++c) {
         const RChunk& ch = P.chunks[c];
+        P.sfeat_c[c].off = static_cast<int>(P.tiles.size());
         if (ch.sk >= 1) add(c, 0, (ch.lw[ch.sk] - 1) / kCompTW, 0, (ch.lh[ch.sk] - 1) / kCompTH);
+        P.sfeat_c[c].n = static_cast<int>(P.tiles.size()) - P.sfeat_c[c].off;
     }
     P.sfeat.n = static_cast<int>(P.tiles.size()) - P.sfeat.off;
     return true;
@@ -2065,46 +2074,55 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
                rcompose_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, l, phase, cflag));
         }
     };
+    // the surrogate stages of chunks [c0, c1): their feature tiles and
+    // network tiles are contiguous sub-ranges (chunk order)
+    int fo = 0, fn = 0, so = 0, sn = 0, spx = 0;
+    if (sr && !P.sfeat_c.empty() && !P.stile_c.empty()) {
+        span(P.sfeat_c, fo, fn);
+        span(P.stile_c, so, sn);
+        for (int k = c0; k < c1; ++k)
+            if (P.chunks[k].sk >= 1) spx += P.chunks[k].lw[P.chunks[k].sk] * P.chunks[k].lh[P.chunks[k].sk];
+    }
     if (surr) {
-        const int nch = static_cast<int>(P.chunks.size());
-        RL(st, "surrogate_flags", 0.0, rscflag_init_kernel<<<(nch + 255) / 256, 256, 0, s>>>(T.chunks, nch, sr->bufs.cflag));
+        const int nc = c1 - c0;
+        RL(st, "surrogate_flags", 0.0,
+           rscflag_init_kernel<<<(nc + 255) / 256, 256, 0, s>>>(T.chunks + c0, nc, sr->bufs.cflag + c0));
     }
     reduces(surr ? 1 : 0);
-    if constexpr (MODE == kModeBatched) if (sr && P.sfeat.n > 0) {
-        const int fo = P.sfeat.off, fn = P.sfeat.n;
-        const double px = static_cast<double>(P.snpix);
+    if constexpr (MODE == kModeBatched) if (sr && fn > 0) {
+        const double px = static_cast<double>(spx);
         RL(st, "surrogate_features", px * 64.0,
            rsfeat_kernel<MODE><<<persistent_grid(ctx, fn), 256, 0, s>>>(T.items, T.chunks, T.tiles + fo, fn, sr->bufs));
         RL(st, "surrogate_blur", px * 80.0,
            rsblur_kernel<<<persistent_grid(ctx, fn), 256, 0, s>>>(T.chunks, T.tiles + fo, fn, sr->bufs, 0));
         RL(st, "surrogate_blur", px * 96.0,
            rsblur_kernel<<<persistent_grid(ctx, fn), 256, 0, s>>>(T.chunks, T.tiles + fo, fn, sr->bufs, 1));
-        if (surr) {
+        if (surr && sn > 0) {
             {
                 ProfScope ps_(ctx, s, "surrogate_network", 0.0);
-                if (surr_forward_tables(T.simgs, T.stiles, T.nst, sr->weights, sr->biases, sr->bufs.feat, sr->bufs.out,
+                if (surr_forward_tables(T.simgs, T.stiles + so, sn, sr->weights, sr->biases, sr->bufs.feat, sr->bufs.out,
                                         sr->bufs.tmp0, sr->bufs.tmp1, s) != hipSuccess) {
                     st.set(I360HIP_KERNEL_LAUNCH_FAILED, "surrogate network launch failed");
                     return;
                 }
             }
             RL(st, "surrogate_flags", px * 16.0,
-               rsflag_kernel<<<T.nst, 256, 0, s>>>(T.simgs, T.stiles, sr->bufs, sr->threshold));
+               rsflag_kernel<<<sn, 256, 0, s>>>(T.simgs, T.stiles, so, sr->bufs, sr->threshold));
         }
     }
     if (surr) {
         reduces(2);
         composes(2);
-        if constexpr (MODE == kModeBatched) if (P.sfeat.n > 0)
-            RL(st, "surrogate_apply", static_cast<double>(P.snpix) * 48.0,
-               rsapply_kernel<MODE><<<persistent_grid(ctx, P.sfeat.n), 256, 0, s>>>(T.chunks, T.tiles + P.sfeat.off, P.sfeat.n, sr->bufs));
+        if constexpr (MODE == kModeBatched) if (fn > 0)
+            RL(st, "surrogate_apply", static_cast<double>(spx) * 48.0,
+               rsapply_kernel<MODE><<<persistent_grid(ctx, fn), 256, 0, s>>>(T.chunks, T.tiles + fo, fn, sr->bufs));
         composes(1);
     } else {
         composes(0);
     }
-    if constexpr (MODE == kModeBatched) if (exporting && P.sfeat.n > 0)
-        RL(st, "surrogate_export", static_cast<double>(P.snpix) * 28.0,
-           rsexport_kernel<MODE><<<persistent_grid(ctx, P.sfeat.n), 256, 0, s>>>(T.chunks, T.tiles + P.sfeat.off, P.sfeat.n, sr->bufs));
+    if constexpr (MODE == kModeBatched) if (exporting && fn > 0)
+        RL(st, "surrogate_export", static_cast<double>(spx) * 28.0,
+           rsexport_kernel<MODE><<<persistent_grid(ctx, fn), 256, 0, s>>>(T.chunks, T.tiles + fo, fn, sr->bufs));
     if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
     const double out_bytes = (c1 > c0 && P.chunks[c0].out16) ? 6.0 : 12.0;  // RGB48 or FP32 RGB per pixel
@@ -2234,11 +2252,12 @@ static bool aux_streams(i360hip_ctx* ctx, int n) {
     return true;
 }
 
-static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, hipStream_t s, Status& st) {
+static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, hipStream_t s, Status& st,
+                           const SurrRun* sr = nullptr) {
     const int nch = static_cast<int>(P.chunks.size());
     const int G = render_groups(ctx, nch);
     if (G <= 1 || !aux_streams(ctx, G - 1)) {
-        enqueue<kModeBatched>(ctx, P, T, 0, nch, s, st);
+        enqueue<kModeBatched>(ctx, P, T, 0, nch, s, st, sr);
         return;
     }
     RenderCache& rc = cache_of(ctx);
@@ -2266,7 +2285,7 @@ static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, 
     if (const char* e = std::getenv("I360HIP_GRID_SHARE")) share = std::max(1, std::atoi(e));
     g_grid_share = share;
     for (int g = 0; g < ng && st.ok(); ++g)
-        enqueue<kModeBatched>(ctx, P, T, cut[g], cut[g + 1], g == 0 ? s : rc.aux[g - 1], st);
+        enqueue<kModeBatched>(ctx, P, T, cut[g], cut[g + 1], g == 0 ? s : rc.aux[g - 1], st, sr);
     g_grid_share = 1;
     for (int g = 1; g < ng; ++g)
         if (hipEventRecord(rc.join[g - 1], rc.aux[g - 1]) != hipSuccess ||
@@ -2376,8 +2395,12 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
                 break;
             default:
                 if (Q.chunks.empty()) break;
-                if (surr != kSurrNone) {
+                if (surr == kSurrExport) {
                     enqueue<kModeBatched>(ctx, Q, T, 0, static_cast<int>(Q.chunks.size()), ss, st, &sr);
+                    break;
+                }
+                if (surr == kSurrRender) {  // chunk groups on concurrent streams, as the batched form
+                    enqueue_groups(ctx, Q, T, ss, st, &sr);
                     break;
                 }
                 enqueue_groups(ctx, Q, T, ss, st);
