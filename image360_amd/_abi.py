@@ -108,6 +108,8 @@ SIGNATURES = {
     "i360hip_export_size": (i32, [i32, i32, i32, i32p, i32p]),
     "i360hip_export_composite_dev": (i32, [vp, C.POINTER(ExportChunk), i32, i32, i32, i32, i32, vp, vp]),
     "i360hip_export_composite": (i32, [vp, C.POINTER(ExportChunk), i32, i32, i32, i32, i32, vp]),
+    "i360hip_pixel_buffer_to_blob": (i32, [vp, vp, i32, i32, vp, sz]),
+    "i360hip_pixel_buffer_from_blob": (i32, [vp, vp, sz, vp, sz, C.POINTER(i32), C.POINTER(i32), C.POINTER(sz)]),
     "i360hip_rgb48_from_render_dev": (i32, [vp, vp, sz, vp, vp]),
     "i360hip_compute_lab_stats": (i32, [vp, vp, i32, i32, vp, vp]),
     "i360hip_apply_reinhard_u16": (i32, [vp, vp, i32, i32, vp, vp, vp, vp]),

@@ -363,6 +363,26 @@ class HipComputeBackend:
         out = _c_contig(out, np.uint8)
         return lib().i360hip_export_composite(self._ctx, arr, len(chunks), total_w, total_h, max_dim, fmt, _ptr(out))
 
+    def pixel_buffer_to_blob(self, rgb48_ptr, width, height):
+        """(rc, bytes) of StorageEngine::WritePixelBuffer's framing of a device RGB48 buffer."""
+        import torch
+
+        torch.cuda.synchronize()  # the buffer is usually written on torch's streams
+        blob = np.zeros(8 + width * height * 6, np.uint8)
+        rc = lib().i360hip_pixel_buffer_to_blob(self._ctx, rgb48_ptr, width, height, _ptr(blob), blob.size)
+        return rc, blob.tobytes()
+
+    def pixel_buffer_from_blob(self, blob: bytes, rgb48_ptr, capacity_values):
+        """(rc, width, height, values) after uploading a blob's payload to a device buffer."""
+        import torch
+
+        torch.cuda.synchronize()
+        b = np.frombuffer(blob, np.uint8)
+        w, h, n = C.c_int32(0), C.c_int32(0), C.c_size_t(0)
+        rc = lib().i360hip_pixel_buffer_from_blob(self._ctx, b.ctypes.data if b.size else None, b.size, rgb48_ptr,
+                                                  capacity_values, C.byref(w), C.byref(h), C.byref(n))
+        return rc, w.value, h.value, n.value
+
     @_ordered
     def rgb48_from_render_dev(self, rgb_ptr, num_values, out_ptr, stream=None) -> int:
         return lib().i360hip_rgb48_from_render_dev(self._ctx, rgb_ptr, num_values, out_ptr, stream)

@@ -417,6 +417,49 @@ int i360hip_export_composite(i360hip_ctx* ctx, const i360hip_export_chunk* chunk
     return I360HIP_SUCCESS;
 }
 
+int i360hip_pixel_buffer_to_blob(i360hip_ctx* ctx, const uint16_t* rgb48, int width, int height, uint8_t* blob,
+                                 size_t blob_size) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!blob || (!rgb48 && width > 0 && height > 0)) return Fail(ctx, I360HIP_INVALID_PARAM, "Null pointer argument.");
+    if (width < 0 || height < 0) return Fail(ctx, I360HIP_INVALID_PARAM, "Invalid dimensions.");
+    const size_t payload = static_cast<size_t>(width) * height * 3 * sizeof(uint16_t);
+    if (blob_size != 8 + payload) return Fail(ctx, I360HIP_INVALID_PARAM, "blob_size must be 8 + width*height*6");
+    const int32_t hdr[2] = {width, height};  // StorageEngine.cpp:132-140: int32 width, int32 height, then the values
+    std::memcpy(blob, hdr, sizeof(hdr));
+    if (payload == 0) return I360HIP_SUCCESS;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (hipMemcpyAsync(blob + 8, rgb48, payload, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return Fail(ctx, I360HIP_DEVICE_ERROR, "pixel download failed");
+    return I360HIP_SUCCESS;
+}
+
+int i360hip_pixel_buffer_from_blob(i360hip_ctx* ctx, const uint8_t* blob, size_t blob_size, uint16_t* rgb48,
+                                   size_t capacity_values, int* out_width, int* out_height, size_t* out_values) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!blob || !out_width || !out_height || !out_values) return Fail(ctx, I360HIP_INVALID_PARAM, "Null pointer argument.");
+    // StorageEngine.cpp:146-167: shorter than the header or an odd payload is no pixel buffer
+    if (blob_size < 8) return Fail(ctx, I360HIP_INVALID_PARAM, "blob shorter than its 8-byte header");
+    const size_t payload = blob_size - 8;
+    if (payload % sizeof(uint16_t) != 0) return Fail(ctx, I360HIP_INVALID_PARAM, "odd payload size");
+    int32_t hdr[2];
+    std::memcpy(hdr, blob, sizeof(hdr));
+    const size_t values = payload / sizeof(uint16_t);
+    if (values > capacity_values || (values > 0 && !rgb48))
+        return Fail(ctx, I360HIP_INVALID_PARAM, "payload larger than the device buffer");
+    *out_width = hdr[0];
+    *out_height = hdr[1];
+    *out_values = values;
+    if (values == 0) return I360HIP_SUCCESS;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (hipMemcpyAsync(rgb48, blob + 8, payload, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return Fail(ctx, I360HIP_DEVICE_ERROR, "pixel upload failed");
+    return I360HIP_SUCCESS;
+}
+
 int i360hip_rgb48_from_render_dev(i360hip_ctx* ctx, const float* rgb, size_t num_values, uint16_t* rgb48,
                                   void* stream) {
     if (!ctx) return I360HIP_DEVICE_ERROR;

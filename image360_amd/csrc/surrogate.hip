@@ -16,6 +16,7 @@
 // input and output-gradient tiles) reduced across tiles in a fixed order,
 // so training is deterministic.
 
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -369,8 +370,8 @@ void conv(const SImg* imgs, const STile* tiles, int ntiles, const float* in, con
     if (ntiles <= 0) return;
     // persistent grid: as many blocks as can be resident at once (LDS and
     // registers), per CU; a block that had to wait would run as a tail
-    static int resident = 0;
-    if (resident == 0) {
+    static std::atomic<int> resident{0};  // per instantiation; the same on every call of a process
+    if (resident.load(std::memory_order_relaxed) == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
@@ -378,9 +379,9 @@ void conv(const SImg* imgs, const STile* tiles, int ntiles, const float* in, con
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, surr_conv_kernel<CIN, COUT, BIAS, RELU>, 256, 0) !=
                 hipSuccess || per_cu <= 0)
             per_cu = 1;
-        resident = cus * per_cu;
+        resident.store(cus * per_cu, std::memory_order_relaxed);
     }
-    const int grid = std::min(ntiles, resident);
+    const int grid = std::min(ntiles, resident.load(std::memory_order_relaxed));
     surr_conv_kernel<CIN, COUT, BIAS, RELU><<<grid, 256, 0, s>>>(imgs, tiles, ntiles, in, mask, out,
                                                                  reinterpret_cast<const uint32_t*>(w), bias);
 }

@@ -383,6 +383,21 @@ int i360hip_export_composite(i360hip_ctx* ctx, const i360hip_export_chunk* chunk
 int i360hip_rgb48_from_render_dev(i360hip_ctx* ctx, const float* rgb, size_t num_values, uint16_t* rgb48,
                                   void* stream);
 
+/* StorageEngine::WritePixelBuffer's framing (StorageEngine.cpp:130-144) of a
+ * device RGB48 buffer: blob = int32 width, int32 height (host byte order),
+ * then the width*height*3 values; blob_size must be 8 + width*height*6.  The
+ * bytes then go to StorageEngine::WriteBlob under the tag
+ * rendered_chunk_rgb48.  Synchronous. */
+int i360hip_pixel_buffer_to_blob(i360hip_ctx* ctx, const uint16_t* rgb48, int width, int height, uint8_t* blob,
+                                 size_t blob_size);
+/* StorageEngine::ReadPixelBuffer (StorageEngine.cpp:146-167) into device
+ * memory: INVALID_PARAM where the reference returns nullopt (shorter than
+ * the header, odd payload) or when the payload exceeds capacity_values.
+ * Like the reference, the payload is not checked against width*height*3;
+ * out_values receives its length in values.  Synchronous. */
+int i360hip_pixel_buffer_from_blob(i360hip_ctx* ctx, const uint8_t* blob, size_t blob_size, uint16_t* rgb48,
+                                   size_t capacity_values, int* out_width, int* out_height, size_t* out_values);
+
 /* ---- optimize-stage colour operations --------------------------------------
  * IComputeBackend::ComputeLabStats and ::ApplyReinhardColorTransfer
  * (IComputeBackend.h:65-73; CpuComputeBackend.cpp:234-257 for the

@@ -193,3 +193,32 @@ def test_host_buffer_composite(hip, f32):
         out = np.full((oh, ow, 3), 0xCD, np.uint8)
         hip.check(hip.ExportComposite(chunks, 640, 410, max_dim, out), "ExportComposite")
         assert np.array_equal(out, po.export_composite(chunks, 640, 410, max_dim)), max_dim
+
+
+def test_pixel_buffer_blob_round_trip(hip):
+    """StorageEngine's framing (StorageEngine.cpp:130-167): int32 width,
+    int32 height, then the RGB48 values; a blob read back lands on the
+    device unchanged, and the reference's rejections hold."""
+    import struct
+
+    import torch
+
+    rng = np.random.default_rng(31)
+    h, w = 37, 53
+    px = rng.integers(0, 65536, (h, w, 3), dtype=np.uint16)
+    dev = _dev(px)
+    note("pixel buffer blob round trip")
+    rc, blob = hip.pixel_buffer_to_blob(dev.data_ptr(), w, h)
+    assert rc == 0 and len(blob) == 8 + px.nbytes
+    assert struct.unpack("<ii", blob[:8]) == (w, h) and blob[8:] == px.tobytes()
+    back = torch.zeros(h * w * 3, dtype=torch.int16, device="cuda")
+    rc, bw, bh, n = hip.pixel_buffer_from_blob(blob, back.data_ptr(), back.numel())
+    assert (rc, bw, bh, n) == (0, w, h, px.size)
+    assert np.array_equal(back.cpu().numpy().view(np.uint16).reshape(h, w, 3), px)
+    # an empty buffer frames to its header alone
+    rc, empty = hip.pixel_buffer_to_blob(dev.data_ptr(), 0, 0)
+    assert rc == 0 and empty == struct.pack("<ii", 0, 0)
+    # what ReadPixelBuffer refuses, and a payload larger than the device buffer
+    for bad in (blob[:7], blob[:8 + 3]):
+        assert hip.pixel_buffer_from_blob(bad, back.data_ptr(), back.numel())[0] == _abi.INVALID_PARAM
+    assert hip.pixel_buffer_from_blob(blob, back.data_ptr(), back.numel() - 1)[0] == _abi.INVALID_PARAM
