@@ -222,3 +222,27 @@ def test_pixel_buffer_blob_round_trip(hip):
     for bad in (blob[:7], blob[:8 + 3]):
         assert hip.pixel_buffer_from_blob(bad, back.data_ptr(), back.numel())[0] == _abi.INVALID_PARAM
     assert hip.pixel_buffer_from_blob(blob, back.data_ptr(), back.numel() - 1)[0] == _abi.INVALID_PARAM
+
+
+def test_gigapixel_canvas_at_full_scale_is_the_chunks_shifted(hip):
+    """At max_dim = the canvas's longest edge the scale is exactly 1, so the
+    composite of the full gigapixel canvas (37977 x 19405, 2.2 GB of RGB8)
+    must be every chunk's RGB48 >> 8 at its offset: a closed form checked
+    at the largest size, built independently with torch on the GPU."""
+    import torch
+
+    tw, th = 37977, 19405
+    assert hip.export_scale(tw, th, tw) == 1.0
+    rects = chunk_grid(tw, th, 4096)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    dev = [torch.randint(-32768, 32768, (h, w, 3), dtype=torch.int16, device="cuda", generator=gen)
+           for (x, y, w, h) in rects]
+    args = [(x, y, w, h, d.data_ptr(), w, h) for (x, y, w, h), d in zip(rects, dev)]
+    note("export gigapixel canvas at full scale")
+    out = torch.empty((th, tw, 3), dtype=torch.uint8, device="cuda")
+    hip.check(hip.export_composite_dev(args, tw, th, tw, _abi.PIX_U16, out.data_ptr()), "export")
+    want = torch.empty_like(out)
+    for (x, y, w, h), d in zip(rects, dev):
+        want[y:y + h, x:x + w] = ((d.to(torch.int32) & 0xFFFF) >> 8).to(torch.uint8)
+    assert torch.equal(out, want)
