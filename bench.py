@@ -291,7 +291,11 @@ def bench_export(hip, torch, geom, chunks, out, offs, log, max_dims=(8192, 4096)
         return ev[0].elapsed_time(ev[1]) / reps
 
     t48 = timed(lambda: hip.check(hip.rgb48_from_render_dev(out.data_ptr(), n, rgb48.data_ptr(), sh), "rgb48"))
-    res = {"rgb48_from_render": {"values": n, "ms": round(t48, 3), "gb_s": round(n * 6 / (t48 / 1e3) / 1e9, 1)}}
+    g48 = n * 6 / (t48 / 1e3) / 1e9
+    res = {"rgb48_from_render": {"values": n, "ms": round(t48, 3), "gb_s": round(g48, 1),
+                                 "roofline": {"bound": "hbm", "achieved": round(g48, 1), "peak": HBM_PEAK_GBS,
+                                              "unit": "GB/s", "frac": round(g48 / HBM_PEAK_GBS, 4),
+                                              "traffic": n * 6}}}
     for md in max_dims:
         ow, oh = hip.export_size(geom.canvas_w, geom.canvas_h, md)
         comp = torch.empty(ow * oh * 3, dtype=torch.uint8, device="cuda")
@@ -304,7 +308,10 @@ def bench_export(hip, torch, geom, chunks, out, offs, log, max_dims=(8192, 4096)
             alg = ow * oh * (3 * esz + 3)
             r["from_rgb48" if fmt == _abi.PIX_U16 else "from_fp32"] = {
                 "ms": round(t, 3), "mp_s": round(ow * oh / (t / 1e3) / 1e6, 1),
-                "gb_s_alg": round(alg / (t / 1e3) / 1e9, 1)}
+                "gb_s_alg": round(alg / (t / 1e3) / 1e9, 1),
+                "roofline": {"bound": "hbm (gather)", "achieved": round(alg / (t / 1e3) / 1e9, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(alg / (t / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
         res[f"max_dim_{md}"] = r
     log(f"export: {res}")
     return res, rgb48
