@@ -194,3 +194,36 @@ def test_trained_surrogate_gains_over_composite(hip):
     got = sg.low_band_psnr(vf, vt, tr.predict(vf))
     note(f"held-out low band: composite {base:.2f} dB, surrogate {got:.2f} dB")
     assert got > base + 1.0
+
+
+def test_surrogate_form_rgb48_output(hip):
+    # the RGB48 output flag on the surrogate form (chunk groups, per-group
+    # network) writes the FP32 output's RGB48 rounding, with the same
+    # fallback decisions
+    import torch
+
+    from image360_amd import _abi
+    from oracle import pyoracle as po
+
+    g, frames = _small_pano(13)
+    w32, b = sg.init_params(2)
+    _set_params(hip, sg.bf16_bits(w32), b, 1e-4)
+    fd, _, _, _ = sg.describe(g, frames)
+    note("surrogate form, RGB48 output")
+    outs = {}
+    for mode in (_abi.RENDER_SURROGATE, _abi.RENDER_SURROGATE | _abi.RENDER_OUT_RGB48):
+        bufs, descs, contrib = [], [], []
+        for c in g.chunks:
+            o = torch.zeros((c.h, c.w, 3), dtype=torch.float32, device="cuda")
+            descs.append({"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": len(contrib),
+                          "count": len(c.contributors), "out": o.data_ptr(), "owner": None})
+            contrib += c.contributors
+            bufs.append(o)
+        hip.check(hip.render_chunks(fd, descs, contrib, 7, mode), f"render mode {mode}")
+        torch.cuda.synchronize()
+        outs[mode] = ([o.cpu().numpy() for o in bufs], hip.surrogate_stats())
+    (f32, st32), (raw, st16) = outs[_abi.RENDER_SURROGATE], outs[_abi.RENDER_SURROGATE | _abi.RENDER_OUT_RGB48]
+    assert st32 == st16 and st32[0] > 0
+    for k, c in enumerate(g.chunks):
+        got = raw[k].view(np.uint16).reshape(-1)[: c.h * c.w * 3].reshape(c.h, c.w, 3)
+        assert np.array_equal(got, po.rgb48_from_render(f32[k])), k
