@@ -227,3 +227,23 @@ def test_surrogate_form_rgb48_output(hip):
     for k, c in enumerate(g.chunks):
         got = raw[k].view(np.uint16).reshape(-1)[: c.h * c.w * 3].reshape(c.h, c.w, 3)
         assert np.array_equal(got, po.rgb48_from_render(f32[k])), k
+
+
+def test_surrogate_form_same_bytes_for_any_chunk_grouping(hip):
+    # the surrogate stages run per chunk group on concurrent streams; the
+    # output and the fallback decisions must not depend on the grouping
+    from image360_amd import _abi
+
+    g, frames = _small_pano(14)
+    w32, b = sg.init_params(3)
+    _set_params(hip, sg.bf16_bits(w32), b, 1e-4)
+    res = []
+    try:
+        for groups in (1, 2, 4):
+            hip.render_set_tuning(chunk_groups=groups)
+            note(f"surrogate form, {groups} chunk group(s), {len(g.chunks)} chunks")
+            res.append((_render(hip, g, frames, _abi.RENDER_SURROGATE), hip.surrogate_stats()))
+    finally:
+        hip.render_set_tuning()
+    for out, stats in res[1:]:
+        assert bool((out == res[0][0]).all()) and stats == res[0][1]
