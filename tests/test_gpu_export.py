@@ -246,3 +246,19 @@ def test_gigapixel_canvas_at_full_scale_is_the_chunks_shifted(hip):
     for (x, y, w, h), d in zip(rects, dev):
         want[y:y + h, x:x + w] = ((d.to(torch.int32) & 0xFFFF) >> 8).to(torch.uint8)
     assert torch.equal(out, want)
+
+
+def test_float_input_special_values(hip):
+    """FP32 render output with NaN, +-inf and out-of-range values goes
+    through the reference's clamp order (NaN and +inf -> 65535, -inf -> 0)
+    before the 8-bit shift, as the oracle restates it."""
+    rng = np.random.default_rng(41)
+    buf = rng.uniform(-0.5, 1.5, (40, 60, 3)).astype(np.float32)
+    buf.reshape(-1)[rng.integers(0, buf.size, 300)] = np.nan
+    buf.reshape(-1)[rng.integers(0, buf.size, 300)] = np.inf
+    buf.reshape(-1)[rng.integers(0, buf.size, 300)] = -np.inf
+    chunks = [(0, 0, 60, 40, buf), (30, 10, 60, 40, buf[::-1].copy())]
+    for max_dim in (90, 37):
+        note(f"export FP32 special values max_dim={max_dim}")
+        got = _composite(hip, chunks, 90, 50, max_dim, _abi.PIX_F32)
+        assert np.array_equal(got, po.export_composite(chunks, 90, 50, max_dim)), max_dim
