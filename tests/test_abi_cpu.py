@@ -43,3 +43,20 @@ def test_create_without_gpu_reports_no_gpu():
 def test_null_context_is_rejected():
     L = _abi.lib()
     assert L.i360hip_apply_gain_u16(None, None, 4, 2.0) == _abi.DEVICE_ERROR
+
+
+def test_python_constants_match_the_header():
+    import re
+
+    from image360_amd import _abi
+
+    text = open(os.path.join(_abi.HEADER_DIR, "i360hip.h")).read()
+    vals = {m.group(1): int(m.group(2), 0) for m in re.finditer(r"\b(I360HIP_[A-Z0-9_]+)\s*=\s*(0x[0-9a-fA-F]+|\d+)", text)}
+    vals.update({m.group(1): int(m.group(2), 0)
+                 for m in re.finditer(r"#define\s+(I360HIP_[A-Z0-9_]+)\s+(0x[0-9a-fA-F]+|\d+)", text)})
+    want = {"I360HIP_PIX_F16": _abi.PIX_F16, "I360HIP_PIX_F32": _abi.PIX_F32, "I360HIP_PIX_U16": _abi.PIX_U16,
+            "I360HIP_RENDER_FP64": _abi.RENDER_FP64, "I360HIP_RENDER_PLAIN": _abi.RENDER_PLAIN,
+            "I360HIP_RENDER_PRECISION": _abi.RENDER_PRECISION, "I360HIP_RENDER_BATCHED": _abi.RENDER_BATCHED,
+            "I360HIP_RENDER_SURROGATE": _abi.RENDER_SURROGATE, "I360HIP_RENDER_OUT_RGB48": _abi.RENDER_OUT_RGB48}
+    for name, v in want.items():
+        assert vals.get(name) == v, (name, vals.get(name), v)
