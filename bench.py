@@ -138,18 +138,19 @@ def cpu_export(geom, chunks, rgb48, offs, log, max_dim=4096):
     return r
 
 
-def measured_traffic(kernel, tuning):
+def measured_traffic(kernel, groups):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated against a
-    1 GiB fill and copy), taken like the roofline's profiled step with one
-    chunk group (a launch covers every chunk); else None."""
+    1 GiB fill and copy), when they were taken with the same number of
+    chunk groups as the roofline's profiled step (normally one, a launch
+    covering every chunk); else None."""
     try:
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
     except (OSError, ValueError):
         return None
     k = prof.get("kernels", {}).get(kernel)
-    if not k or prof.get("chunk_groups") != 1:
+    if not k or prof.get("chunk_groups") != groups:
         return None
     return k["traffic_bytes_per_launch"]
 
@@ -530,8 +531,11 @@ def main():
     achieved = dom["alg_bytes"] / (dom["total_ms"] / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(dom_name, tuning),
-                "avg_launch_us": round(dom["total_ms"] / dom["launches"] * 1e3, 2), "profiled_chunk_groups": 1,
+                "traffic": measured_traffic(dom_name, dom["launches"]),
+                "avg_launch_us": round(dom["total_ms"] / dom["launches"] * 1e3, 2),
+                # one launch of the dominant stage per chunk group (an
+                # I360HIP_RENDER_GROUPS override beats the one-group setting)
+                "profiled_chunk_groups": dom["launches"],
                 "time_share": round(dom["total_ms"] / tot, 3),
                 "latency_bound_share": round(sum(v["total_ms"] for k, v in stats.items()
                                                  if v["alg_bytes"] == 0) / tot, 3)}
