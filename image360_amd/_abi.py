@@ -73,6 +73,7 @@ SIGNATURES = {
     "i360hip_destroy": (None, [vp]),
     "i360hip_get_device_info": (i32, [vp, C.POINTER(DeviceInfo)]),
     "i360hip_last_error": (C.c_char_p, [vp]),
+    "i360hip_thread_error": (C.c_char_p, []),
     "i360hip_version": (C.c_char_p, []),
     "i360hip_profile_enable": (i32, [vp, i32]),
     "i360hip_profile_read": (i32, [vp, C.POINTER(KernelStat), i32, C.POINTER(i32)]),

@@ -29,6 +29,7 @@ namespace {
 
 char g_create_error[512] = "";
 std::mutex g_create_mu;
+thread_local char t_error[512] = "";
 
 enum Slot : int { kSlotA = 0, kSlotB, kSlotC, kSlotPlanes, kSlotCount };
 
@@ -56,15 +57,20 @@ void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st) {
     return b.ptr;
 }
 
+void set_error(i360hip_ctx* ctx, const char* msg) {
+    std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", msg);
+    std::snprintf(t_error, sizeof(t_error), "%s", msg);
+}
+
 namespace {
 
 int Finish(i360hip_ctx* ctx, const Status& st) {
-    if (!st.ok()) std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+    if (!st.ok()) set_error(ctx, st.msg);
     return st.code;
 }
 
 int Fail(i360hip_ctx* ctx, int code, const char* msg) {
-    std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", msg);
+    set_error(ctx, msg);
     return code;
 }
 
@@ -81,8 +87,7 @@ extern "C" {
 
 const char* i360hip_version(void) { return "i360hip abi 1 gfx950"; }
 
-int i360hip_create(int device, i360hip_ctx** out_ctx) {
-    std::lock_guard<std::mutex> lk(g_create_mu);
+static int create_locked(int device, i360hip_ctx** out_ctx) {
     g_create_error[0] = 0;
     if (!out_ctx) {
         std::snprintf(g_create_error, sizeof(g_create_error), "Null pointer argument.");
@@ -144,7 +149,16 @@ void i360hip_destroy(i360hip_ctx* ctx) {
     delete ctx;
 }
 
+int i360hip_create(int device, i360hip_ctx** out_ctx) {
+    std::lock_guard<std::mutex> lk(g_create_mu);
+    const int rc = create_locked(device, out_ctx);
+    if (rc != I360HIP_SUCCESS) std::snprintf(t_error, sizeof(t_error), "%s", g_create_error);
+    return rc;
+}
+
 const char* i360hip_last_error(const i360hip_ctx* ctx) { return ctx ? ctx->last_error : g_create_error; }
+
+const char* i360hip_thread_error(void) { return t_error; }
 
 int i360hip_get_device_info(i360hip_ctx* ctx, i360hip_device_info* out) {
     if (!ctx) return I360HIP_DEVICE_ERROR;

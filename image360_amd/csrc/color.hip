@@ -194,11 +194,11 @@ __global__ void __launch_bounds__(256) reinhard_kernel(uint16_t* __restrict__ rg
 }
 
 int Fail(i360hip_ctx* ctx, int code, const char* msg) {
-    std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", msg);
+    set_error(ctx, msg);
     return code;
 }
 int Finish(i360hip_ctx* ctx, const Status& st) {
-    if (!st.ok()) std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+    if (!st.ok()) set_error(ctx, st.msg);
     return st.code;
 }
 

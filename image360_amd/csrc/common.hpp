@@ -131,6 +131,10 @@ __device__ __forceinline__ void lds_barrier() {
 // Context-owned device buffer `slot` of at least `bytes` bytes (grown on
 // demand, kept until the context is destroyed).
 void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st);
+// Records the text of a failing call: in the context (i360hip_last_error,
+// shared by every thread that calls it) and for the calling thread
+// (i360hip_thread_error, which no other thread's failure can replace).
+void set_error(i360hip_ctx* ctx, const char* msg);
 // Drops the context's cached render plan and graph.
 void render_release(i360hip_ctx* ctx);
 }  // namespace i360

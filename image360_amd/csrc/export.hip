@@ -200,7 +200,7 @@ int grid_for(const i360hip_ctx* ctx, size_t work) {
 }
 
 int Fail(i360hip_ctx* ctx, int code, const char* msg) {
-    std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", msg);
+    set_error(ctx, msg);
     return code;
 }
 

@@ -2142,6 +2142,9 @@ struct RenderCache {
     hipEvent_t fork = nullptr;
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> join;
+    // recorded after the last enqueued render (any stream): the workspace
+    // and tables are not rewritten before it has completed
+    hipEvent_t done = nullptr;
 };
 
 static RenderCache& cache_of(i360hip_ctx* ctx) {
@@ -2159,9 +2162,27 @@ void render_release(i360hip_ctx* ctx) {
     }
     for (auto e : rc->join) (void)hipEventDestroy(e);
     if (rc->fork) (void)hipEventDestroy(rc->fork);
+    if (rc->done) {
+        (void)hipEventSynchronize(rc->done);
+        (void)hipEventDestroy(rc->done);
+    }
     delete rc;
     ctx->render_cache = nullptr;
 }
+
+// A render enqueued on one stream may still be reading the workspace, the
+// tables or the surrogate parameters when a later call on another stream
+// rewrites them (a new plan, new parameters): every render records `done` on
+// its stream, and a call that rewrites waits for it first.
+static void mark_done(RenderCache& rc, hipStream_t s, Status& st) {
+    if (!rc.done && hipEventCreateWithFlags(&rc.done, hipEventDisableTiming) != hipSuccess) {
+        rc.done = nullptr;
+        st.set(I360HIP_DEVICE_ERROR, "render completion event creation failed");
+        return;
+    }
+    if (hipEventRecord(rc.done, s) != hipSuccess) st.set(I360HIP_DEVICE_ERROR, "render completion record failed");
+}
+static bool wait_renders(const RenderCache& rc) { return !rc.done || hipEventSynchronize(rc.done) == hipSuccess; }
 
 // Uploads the tables of `P` into the table slot; fills T.
 static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s, Status& st, int slot) {
@@ -2348,6 +2369,7 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     const uint64_t ck = graphable ? call_key(ctx, frames, nf, chunks, nc, contrib, bands, api_mode) : 0;
     if (graphable && !ctx->profiling && rc.graph && rc.mode == api_mode && rc.call == ck) {
         if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
+        else mark_done(rc, s, st);
         return;
     }
     Plan P;
@@ -2362,6 +2384,11 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     if (reuse) {  // other arguments, the same plan
         rc.call = ck;
         if (hipGraphLaunch(rc.graph, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
+        else mark_done(rc, s, st);
+        return;
+    }
+    if (!wait_renders(rc)) {
+        st.set(I360HIP_DEVICE_ERROR, "waiting for the previous render failed");
         return;
     }
     if (rc.graph) {
@@ -2413,16 +2440,19 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
             hipMemcpyAsync(export_feat, Q.sbufs.feat, sizeof(float) * kSurrIn * static_cast<size_t>(Q.snpix),
                            hipMemcpyDeviceToDevice, s) != hipSuccess)
             st.set(I360HIP_DEVICE_ERROR, "surrogate feature copy failed");
+        mark_done(rc, s, st);
         return;
     }
     if (!graphable || ctx->profiling) {
         body(s);
+        mark_done(rc, s, st);
         return;
     }
     // batched: capture once, replay on later identical calls
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         body(s);
+        mark_done(rc, s, st);
         return;
     }
     body(s);
@@ -2441,6 +2471,7 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     rc.graph = ge;
     rc.call = ck;
     if (hipGraphLaunch(ge, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
+    else mark_done(rc, s, st);
 }
 
 }  // namespace i360
@@ -2454,12 +2485,12 @@ int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num
                           int mode, void* stream) {
     if (!ctx) return I360HIP_DEVICE_ERROR;
     if ((!frames && num_frames > 0) || (!chunks && num_chunks > 0) || (!contributors && num_chunks > 0)) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument.");
+        set_error(ctx, "Null pointer argument.");
         return I360HIP_INVALID_PARAM;
     }
     const int form = mode & ~I360HIP_RENDER_OUT_RGB48;
     if (num_bands < 1 || num_bands > I360HIP_MAX_BANDS || form < I360HIP_RENDER_FP64 || form > I360HIP_RENDER_SURROGATE) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "numBands must be 1-12 and mode a known form.");
+        set_error(ctx, "numBands must be 1-12 and mode a known form.");
         return I360HIP_INVALID_PARAM;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -2467,7 +2498,7 @@ int i360hip_render_chunks(i360hip_ctx* ctx, const i360hip_frame* frames, int num
     if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     run_render(ctx, frames, num_frames, chunks, num_chunks, contributors, num_bands, mode, s, st);
-    if (!st.ok()) std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+    if (!st.ok()) set_error(ctx, st.msg);
     return st.code;
 }
 
@@ -2481,7 +2512,7 @@ int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int 
     if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
     Plan P;
     if (!build_plan(frames, num_frames, chunks, num_chunks, contributors, 1, kModePrecision, P, st)) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        set_error(ctx, st.msg);
         return st.code;
     }
     // only the item table and warp tiles are used; no workspace needed
@@ -2490,7 +2521,7 @@ int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int 
     upload_tables(ctx, P, T, ctx->stream, st, kSlotRender + 3);
     auto* bits = static_cast<unsigned int*>(ctx_scratch(ctx, kSlotRender + 2, 256, st));
     if (!st.ok()) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        set_error(ctx, st.msg);
         return st.code;
     }
     (void)hipMemsetAsync(bits, 0, sizeof(unsigned int), ctx->stream);
@@ -2522,17 +2553,21 @@ extern "C" {
 int i360hip_surrogate_set(i360hip_ctx* ctx, const uint16_t* weights, const float* biases, float threshold_mse) {
     if (!ctx) return I360HIP_DEVICE_ERROR;
     if (!weights || !biases || !(threshold_mse >= 0.0f)) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument or negative threshold.");
+        set_error(ctx, "Null pointer argument or negative threshold.");
         return I360HIP_INVALID_PARAM;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
     Status st;
     if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
     constexpr int kSlotSurrParams = 18;
+    if (!wait_renders(cache_of(ctx))) {  // an enqueued SURROGATE render may still read the parameters
+        set_error(ctx, "waiting for the previous render failed");
+        return I360HIP_DEVICE_ERROR;
+    }
     const size_t wb = A256(2 * static_cast<size_t>(I360HIP_SURR_WEIGHTS));
     auto* dev = static_cast<uint8_t*>(ctx_scratch(ctx, kSlotSurrParams, wb + 4 * I360HIP_SURR_BIASES, st));
     if (!st.ok()) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        set_error(ctx, st.msg);
         return st.code;
     }
     if (hipMemcpyAsync(dev, weights, 2 * static_cast<size_t>(I360HIP_SURR_WEIGHTS), hipMemcpyDeviceToDevice,
@@ -2540,7 +2575,7 @@ int i360hip_surrogate_set(i360hip_ctx* ctx, const uint16_t* weights, const float
         hipMemcpyAsync(dev + wb, biases, 4 * static_cast<size_t>(I360HIP_SURR_BIASES), hipMemcpyDeviceToDevice,
                        ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "surrogate parameter copy failed");
+        set_error(ctx, "surrogate parameter copy failed");
         return I360HIP_DEVICE_ERROR;
     }
     ctx->surr_weights = reinterpret_cast<const uint16_t*>(dev);
@@ -2576,7 +2611,7 @@ int i360hip_surrogate_features(i360hip_ctx* ctx, const i360hip_frame* frames, in
     if (!ctx) return I360HIP_DEVICE_ERROR;
     if (!feat || !target || (!frames && num_frames > 0) || (!chunks && num_chunks > 0) ||
         (!contributors && num_chunks > 0) || num_bands < 1 || num_bands > I360HIP_MAX_BANDS) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument or numBands outside 1-12.");
+        set_error(ctx, "Null pointer argument or numBands outside 1-12.");
         return I360HIP_INVALID_PARAM;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -2585,7 +2620,7 @@ int i360hip_surrogate_features(i360hip_ctx* ctx, const i360hip_frame* frames, in
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     run_render(ctx, frames, num_frames, chunks, num_chunks, contributors, num_bands, I360HIP_RENDER_BATCHED, s, st,
                feat, target);
-    if (!st.ok()) std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+    if (!st.ok()) set_error(ctx, st.msg);
     return st.code;
 }
 
@@ -2604,7 +2639,7 @@ int i360hip_surrogate_stats(i360hip_ctx* ctx, int* tiles, int* fallback_tiles) {
     if (hipSetDevice(ctx->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(tf.data(), P.sbufs.tflag, 4 * tf.size(), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(cf.data(), P.sbufs.cflag, 4 * cf.size(), hipMemcpyDeviceToHost) != hipSuccess) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "surrogate flag read-back failed");
+        set_error(ctx, "surrogate flag read-back failed");
         return I360HIP_DEVICE_ERROR;
     }
     int fb = 0;
@@ -2624,7 +2659,7 @@ int i360hip_render_autotune(i360hip_ctx* ctx, const i360hip_frame* frames, int n
     if (!ctx) return I360HIP_DEVICE_ERROR;
     if (!out || (!frames && num_frames > 0) || (!chunks && num_chunks > 0) || (!contributors && num_chunks > 0) ||
         num_bands < 1 || num_bands > I360HIP_MAX_BANDS) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "Null pointer argument or numBands outside 1-12.");
+        set_error(ctx, "Null pointer argument or numBands outside 1-12.");
         return I360HIP_INVALID_PARAM;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -2633,7 +2668,7 @@ int i360hip_render_autotune(i360hip_ctx* ctx, const i360hip_frame* frames, int n
     hipStream_t s = ctx->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "autotune event creation failed");
+        set_error(ctx, "autotune event creation failed");
         return I360HIP_DEVICE_ERROR;
     }
     // time one render (graph captured on the first call, then replayed) of
@@ -2676,7 +2711,7 @@ int i360hip_render_autotune(i360hip_ctx* ctx, const i360hip_frame* frames, int n
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!st.ok()) {
-        std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+        set_error(ctx, st.msg);
         return st.code;
     }
     out->chunk_groups = bestG;

@@ -515,7 +515,7 @@ using namespace i360;
 namespace {
 
 int Finish(i360hip_ctx* ctx, const Status& st) {
-    if (!st.ok()) std::snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", st.msg);
+    if (!st.ok()) set_error(ctx, st.msg);
     return st.code;
 }
 

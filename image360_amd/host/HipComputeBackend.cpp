@@ -43,20 +43,31 @@ namespace WindowsApp { namespace Compute
     ComputeResult HipComputeBackend::FromAbi(int rc)
     {
         if (rc == I360HIP_SUCCESS) return ComputeResult::SUCCESS;
-        return Fail(ToResult(rc), i360hip_last_error(m_ctx));
+        // the calling thread's own message: another thread's failure on the
+        // same context cannot have replaced it since the call returned
+        return Fail(ToResult(rc), i360hip_thread_error());
     }
 
     ComputeResult HipComputeBackend::Initialize()
     {
         if (m_ctx) return ComputeResult::SUCCESS;
+        // The align operations go to the host delegate, which answers them
+        // only once initialised (CpuComputeBackend returns CUDA_ERROR "Not
+        // initialized." before); the reference's factory initialises every
+        // backend it hands out (ComputeBackendFactory.cpp:108-136).
+        if (m_host && !m_host->IsInitialized())
+        {
+            const ComputeResult hr = m_host->Initialize();
+            if (hr != ComputeResult::SUCCESS) return Fail(hr, m_host->GetLastError());
+        }
         i360hip_ctx* ctx = nullptr;
         const int rc = i360hip_create(m_device, &ctx);
-        if (rc != I360HIP_SUCCESS) return Fail(ToResult(rc), i360hip_last_error(nullptr));
+        if (rc != I360HIP_SUCCESS) return Fail(ToResult(rc), i360hip_thread_error());
         i360hip_device_info di{};
         const int rc2 = i360hip_get_device_info(ctx, &di);
         if (rc2 != I360HIP_SUCCESS)
         {
-            const ComputeResult r = Fail(ToResult(rc2), i360hip_last_error(ctx));
+            const ComputeResult r = Fail(ToResult(rc2), i360hip_thread_error());
             i360hip_destroy(ctx);
             return r;
         }
@@ -96,7 +107,13 @@ namespace WindowsApp { namespace Compute
 
     const char* HipComputeBackend::GetLastError() const
     {
-        return m_lastError;
+        // A snapshot taken under the lock: another thread's failure may
+        // replace m_lastError at any time, but not the text handed out
+        // here (valid until this thread's next GetLastError call).
+        thread_local char snapshot[sizeof(m_lastError)];
+        std::lock_guard<std::mutex> lk(m_mu);
+        std::memcpy(snapshot, m_lastError, sizeof(snapshot));
+        return snapshot;
     }
 
     // ---- render / ingest operations (validation as CpuComputeBackend.cpp:112-193)

@@ -8,8 +8,10 @@
 //  * every pointer is caller-owned host memory, outputs pre-allocated;
 //  * calls are synchronous (results valid on return);
 //  * results are ComputeResult codes, the text of the last failure is
-//    returned by GetLastError() (one shared 512-byte buffer, as in every
-//    reference backend);
+//    returned by GetLastError() (one 512-byte record per instance, shared by
+//    its callers as in every reference backend; the pointer returned is a
+//    snapshot taken under the instance's lock, valid until the calling
+//    thread's next GetLastError call);
 //  * argument validation runs in CpuComputeBackend's order and with its
 //    messages (CpuComputeBackend.cpp:112-193);
 //  * one instance may be called from several threads at once (TaskScheduler

@@ -76,8 +76,16 @@ typedef struct i360hip_device_info {
 int i360hip_create(int device, i360hip_ctx** out_ctx);
 void i360hip_destroy(i360hip_ctx* ctx);
 int i360hip_get_device_info(i360hip_ctx* ctx, i360hip_device_info* out);
-/* Last error text of `ctx`; with ctx == NULL, the last i360hip_create error. */
+/* Last error text of `ctx`; with ctx == NULL, the last i360hip_create error.
+ * Shared by every thread calling on the context, like the reference backends'
+ * m_lastError buffer behind IComputeBackend::GetLastError. */
 const char* i360hip_last_error(const i360hip_ctx* ctx);
+/* Error text of the calling thread's last failing call (on any context, or
+ * i360hip_create); a failure on another thread cannot replace it.  Valid
+ * until this thread's next failing call.  A wrapper that reads the message
+ * right after a failing call (HipComputeBackend does) uses this one; the
+ * reference has no per-thread form. */
+const char* i360hip_thread_error(void);
 /* Library build identification (gfx target, ABI version). */
 const char* i360hip_version(void);
 

@@ -77,10 +77,12 @@ namespace
         ComputeResult MedianStack(const unsigned short**, int, unsigned short*, int, int, float) override { return ComputeResult::SUCCESS; }
         ComputeResult ApplyGain(unsigned short*, int, float) override { return ComputeResult::SUCCESS; }
         ComputeResult DemosaicBayer(const unsigned short*, int, int, unsigned short, const float[4], const float[3][4], uint32_t, unsigned short*) override { return ComputeResult::SUCCESS; }
+        // like CpuComputeBackend, the align operations answer CUDA_ERROR
+        // until Initialize() has run
         ComputeResult DetectAndDescribeFeatures(const unsigned char*, int, int, FeaturePoint*, BriefDescriptor*, int* outCount, int) override
-        { ++calls; *outCount = 7; return ComputeResult::SUCCESS; }
+        { ++calls; if (!inited) return ComputeResult::CUDA_ERROR; *outCount = 7; return ComputeResult::SUCCESS; }
         ComputeResult MatchFeatures(const BriefDescriptor*, int, const BriefDescriptor*, int, MatchResult*, int* n, int, float) override
-        { ++calls; *n = 3; return ComputeResult::INVALID_PARAM; }
+        { ++calls; if (!inited) return ComputeResult::CUDA_ERROR; *n = 3; return ComputeResult::INVALID_PARAM; }
         ComputeResult ComputeLabStats(const unsigned short*, int, int, double m[3], double s[3]) override
         { ++calls; for (int i = 0; i < 3; ++i) { m[i] = i + 0.5; s[i] = 2.0; } return ComputeResult::SUCCESS; }
         ComputeResult ApplyReinhardColorTransfer(unsigned short*, int, int, const double[3], const double[3], const double[3], const double[3]) override
@@ -126,6 +128,15 @@ namespace
         Check(WindowsApp::Hip::RecommendedChunkSize(288ull << 30) == 4096 &&
                   WindowsApp::Hip::RecommendedChunkSize(8ull << 30) == 2048 &&
                   WindowsApp::Hip::RecommendedChunkSize((8ull << 30) - 1) == 1024, "RecommendedChunkSize thresholds");
+        {
+            // the host delegate is initialised with the HIP backend, so its
+            // align operations answer (the reference's factory initialises
+            // every backend it returns)
+            auto host = std::make_shared<FakeHost>();
+            HipComputeBackend hb(0, host);
+            (void)hb.Initialize();
+            Check(host->inited, "Initialize initialises the host delegate");
+        }
         auto fake = std::make_shared<FakeHost>();
         BackendSelection sel = WindowsApp::Hip::SelectComputeBackend(ComputeBackendKind::Auto, [&] { return fake; });
         Check(sel.kind == ComputeBackendKind::Cpu && !sel.usedGpu && sel.backend == fake && fake->inited &&

@@ -29,6 +29,34 @@ def _run(*args, timeout=120):
 def test_contract_without_gpu():
     out = _run("cpu")
     assert "ok Auto falls back to the supplied CPU backend" in out
+    assert "ok Initialize initialises the host delegate" in out
+
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("sanitizer", ["thread", "address,undefined"])
+def test_host_layer_concurrency_under_sanitizers(tmp_path, sanitizer):
+    """HipComputeBackend and the factory, built with a sanitizer and linked
+    against the CPU stand-in of the C ABI (tests/cpp/abi_stub.cpp): eight
+    threads on one instance mixing succeeding calls, failing C ABI calls with
+    thread-specific messages, validation failures and delegated operations.
+    The sanitizer's report (a data race, a memory error, undefined
+    behaviour) fails the run."""
+    exe = str(tmp_path / "host_concurrency_test")
+    host = os.path.join(ROOT, "image360_amd", "host")
+    cmd = ["g++", "-std=c++17", "-O1", "-g", f"-fsanitize={sanitizer}", "-fno-omit-frame-pointer",
+           "-fno-sanitize-recover=all", "-w", f"-I{host}", f"-I{os.path.join(host, 'contract')}",
+           f"-I{os.path.join(ROOT, 'include')}", os.path.join(host, "HipComputeBackend.cpp"),
+           os.path.join(host, "HipBackendFactory.cpp"), os.path.join(ROOT, "tests", "cpp", "abi_stub.cpp"),
+           os.path.join(ROOT, "tests", "cpp", "host_concurrency_test.cpp"), "-lpthread", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66", ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=env)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "ok concurrent results" in r.stdout and "ok error texts whole" in r.stdout
 
 
 def _load(d):
