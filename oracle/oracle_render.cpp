@@ -39,6 +39,11 @@ namespace {
 
 inline size_t Px(int x, int y, int w) { return static_cast<size_t>(y) * w + x; }
 
+// Planted mistakes, for the tests that show a pin has teeth (0 = none):
+// 1 swaps the bilinear weights w10 and w01; 2 applies the gain after the
+// clamp.  Set only by orc_render_plant, from those tests.
+int g_plant = 0;
+
 template <typename T>
 void Blur5T(const std::vector<T>& src, int w, int h, int ch, std::vector<T>& dst) {
     const T k0 = T(1) / T(16), k1 = T(4) / T(16), k2 = T(6) / T(16);
@@ -135,14 +140,20 @@ void WarpF(const float* src, int sw, int sh, const T* m, float gain, int x0, int
             T fx = u - T(sx0), fy = v - T(sy0);
             T omx = T(1) - fx, omy = T(1) - fy;
             T w00 = omx * omy, w10 = fx * omy, w01 = omx * fy, w11 = fx * fy;
+            if (g_plant == 1) std::swap(w10, w01);
             const float* r0 = src + (static_cast<size_t>(sy0) * sw + sx0) * 3;
             const float* r1 = r0 + static_cast<size_t>(sw) * 3;
             size_t p = Px(x, y, w);
             cov[p] = 1;
             for (int c = 0; c < 3; ++c) {
                 T val = ((w00 * T(r0[c]) + w10 * T(r0[3 + c])) + w01 * T(r1[c])) + w11 * T(r1[3 + c]);
-                if (useGain) val = val * T(gain);
-                val = val < T(0) ? T(0) : (T(1) < val ? T(1) : val);
+                if (g_plant == 2) {
+                    val = val < T(0) ? T(0) : (T(1) < val ? T(1) : val);
+                    if (useGain) val = val * T(gain);
+                } else {
+                    if (useGain) val = val * T(gain);
+                    val = val < T(0) ? T(0) : (T(1) < val ? T(1) : val);
+                }
                 rgb[p * 3 + c] = val;
             }
         }
@@ -368,6 +379,29 @@ void orc_render_chunk_f32(const float* const* srcs, const int* sw, const int* sh
                           int32_t* ownerOut, const int32_t* ownerIn) {
     RenderChunkT<float>(srcs, sw, sh, hinv, gains, c, x0, y0, w, h, numBands, out, ownerOut, ownerIn);
 }
+
+// One contributor's warp into a chunk (the render's first step): float RGB
+// (w*h*3) and the coverage bit (w*h) of every chunk pixel.
+void orc_warp_float_f64(const float* src, int sw, int sh, const double* hinv, float gain, int x0, int y0, int w,
+                        int h, double* rgb, uint8_t* cov) {
+    std::vector<double> r;
+    std::vector<uint8_t> c;
+    WarpF<double>(src, sw, sh, hinv, gain, x0, y0, w, h, r, c);
+    std::memcpy(rgb, r.data(), r.size() * sizeof(double));
+    std::memcpy(cov, c.data(), c.size());
+}
+
+void orc_warp_float_f32(const float* src, int sw, int sh, const float* hinv, float gain, int x0, int y0, int w,
+                        int h, float* rgb, uint8_t* cov) {
+    std::vector<float> r;
+    std::vector<uint8_t> c;
+    WarpF<float>(src, sw, sh, hinv, gain, x0, y0, w, h, r, c);
+    std::memcpy(rgb, r.data(), r.size() * sizeof(float));
+    std::memcpy(cov, c.data(), c.size());
+}
+
+// Test hook: plants mistake `which` (see g_plant) in the float warp; 0 removes it.
+void orc_render_plant(int which) { g_plant = which; }
 
 // FP64 adjugate inverse of a forward homography (local -> canvas), the
 // binary64 counterpart of HomographyMath.cpp:94-116.

@@ -195,6 +195,42 @@ def render_chunk(srcs, hinv, gains, x0, y0, w, h, num_bands, fp64=True, owner_in
     return (out, owner) if want_owner else out
 
 
+def warp_float(src, hinv, gain, x0, y0, w, h, fp64=True):
+    """One contributor's float warp into a w x h chunk at canvas (x0, y0):
+    (rgb (h, w, 3), coverage (h, w) uint8).  src: float32 RGB frame."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    sh, sw = src.shape[:2]
+    cov = np.zeros((h, w), np.uint8)
+    if fp64:
+        m = np.ascontiguousarray(hinv, dtype=np.float64).reshape(9)
+        rgb = np.zeros((h, w, 3), np.float64)
+        lib().orc_warp_float_f64(_p(src, f32p), sw, sh, _p(m, f64p), C.c_float(gain), x0, y0, w, h, _p(rgb, f64p),
+                                 _p(cov, u8p))
+    else:
+        m = np.ascontiguousarray(hinv, dtype=np.float32).reshape(9)
+        rgb = np.zeros((h, w, 3), np.float32)
+        lib().orc_warp_float_f32(_p(src, f32p), sw, sh, _p(m, f32p), C.c_float(gain), x0, y0, w, h, _p(rgb, f32p),
+                                 _p(cov, u8p))
+    return rgb, cov
+
+
+class planted:
+    """Context manager planting a mistake in the float warp (1: w10 and w01
+    swapped; 2: gain applied after the clamp), for tests that show a pin
+    fails on it."""
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        lib().orc_render_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_render_plant(0)
+        return False
+
+
 # ---------------------------------------------------------------- export --
 def export_scale(total_w, total_h, max_dim):
     f = lib().orc_export_scale

@@ -79,3 +79,127 @@ def test_binary32_oracle_close_to_binary64():
                                want_owner=True)
     assert psnr(o32, o64) > 80
     assert np.max(np.abs(o32 - o64)) < 1e-3
+
+
+# ---- the float warp's fractional path (oracle_render.cpp WarpF) ---------
+# The render tests compare the GPU with this oracle, so its bilinear
+# weights, perspective divide at non-integer positions, validity rule and
+# gain order are pinned here: against the reference's own RGB48 warp
+# (golden vectors written by the compiled reference), a closed form and the
+# RGB48 oracle's gain, each shown to fail on a planted mistake.
+
+def _warp_pin_residual(golden, fp64):
+    """Largest |trunc(float warp * 65535) - reference RGB48 warp| over every
+    golden warp case, with the float warp run on the sources scaled to
+    [0,1]; also the number of pixels the two disagree on covering."""
+    g = golden("warp")
+    cases = [(g["smoke_src"], g["smoke_h"], (0, 0), g["smoke_out"])]
+    cases += [(g["rand_src"], g["rand_h"][k], tuple(g["rand_off"][k]), g["rand_out"][k]) for k in range(6)]
+    worst, cover_diff, covered = 0, 0, 0
+    for src16, hm, (ox, oy), ref in cases:
+        src = src16.astype(np.float32) / np.float32(65535)
+        hinv = hm.astype(np.float64) if fp64 else hm
+        h, w = ref.shape[:2]
+        rgb, cov = po.warp_float(src, hinv, 1.0, int(ox), int(oy), w, h, fp64=fp64)
+        got = np.floor(rgb.astype(np.float64) * 65535.0)
+        on = cov.astype(bool)
+        covered += int(on.sum())
+        # the reference writes (0,0,0) where it has no sample
+        cover_diff += int((~on & ref.any(axis=2)).sum())
+        if on.any():
+            worst = max(worst, int(np.abs(got[on] - ref[on].astype(np.float64)).max()))
+    return worst, cover_diff, covered
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_float_warp_matches_reference_rgb48_warp(golden, fp64):
+    # rotated, scaled and perspective homographies (one with its singular
+    # line across the output), offsets, fractional samples everywhere
+    worst, cover_diff, covered = _warp_pin_residual(golden, fp64)
+    assert covered > 20000
+    assert cover_diff == 0
+    # binary32: the reference's own coordinate arithmetic, so only the value
+    # scaling and its truncation differ; binary64 coordinates move a sample
+    # by ~1e-5 px, worth < 2 LSB at full-range texel differences
+    assert worst <= (2 if fp64 else 1), worst
+
+
+def test_float_warp_reference_pin_fails_on_swapped_weights(golden):
+    with po.planted(1):
+        worst, _, _ = _warp_pin_residual(golden, fp64=False)
+    assert worst > 1000
+
+
+def _ramp_case(gain=1.0):
+    sw, sh = 97, 61
+    y, x = np.mgrid[0:sh, 0:sw].astype(np.float64)
+    a = np.array([0.05, 0.2, 0.1])
+    b = np.array([0.8, 0.1, -0.3]) / sw
+    d = np.array([0.07, 0.6, 0.5]) / sh
+    src = (a + x[..., None] * b + y[..., None] * d).astype(np.float32)
+    fwd = np.array([[0.93, -0.21, 31.4], [0.17, 1.08, 12.7], [4.1e-4, -2.7e-4, 1.0]])
+    hinv = np.linalg.inv(fwd)
+    x0, y0, w, h = 7, -3, 150, 110
+    X, Y = np.meshgrid(np.arange(x0, x0 + w, dtype=np.float64), np.arange(y0, y0 + h, dtype=np.float64))
+    m = hinv.reshape(9)
+    q = (m[6] * X + m[7] * Y) + m[8]
+    u = ((m[0] * X + m[1] * Y) + m[2]) / q
+    v = ((m[3] * X + m[4] * Y) + m[5]) / q
+    fu, fv = np.floor(u), np.floor(v)
+    valid = (np.abs(q) >= 1e-10) & (fu >= 0) & (fv >= 0) & (fu <= sw - 2) & (fv <= sh - 2)
+    want = np.clip((a + u[..., None] * b + v[..., None] * d) * gain, 0.0, 1.0)
+    return src, hinv, (x0, y0, w, h), valid, want
+
+
+def _ramp_residual(gain=1.0):
+    src, hinv, (x0, y0, w, h), valid, want = _ramp_case(gain)
+    rgb, cov = po.warp_float(src, hinv, gain, x0, y0, w, h, fp64=True)
+    assert np.array_equal(cov.astype(bool), valid)
+    assert valid.mean() > 0.3
+    return float(np.abs(rgb - want)[valid].max())
+
+
+def test_float_warp_samples_an_affine_ramp_exactly():
+    # bilinear interpolation reproduces an affine function exactly, so the
+    # warp of a ramp image is the ramp at the projective source position:
+    # this pins the weights, the perspective divide at non-integer
+    # positions and the validity rule (up to the float32 storage of the
+    # ramp, ~3e-8); with a gain that saturates, also that the gain comes
+    # before the clamp
+    assert _ramp_residual() < 1e-7
+    assert _ramp_residual(gain=1.7) < 2e-7
+
+
+def test_float_warp_ramp_pin_fails_on_planted_mistakes():
+    with po.planted(1):
+        assert _ramp_residual() > 1e-3
+    with po.planted(2):
+        assert _ramp_residual(gain=1.7) > 0.1
+
+
+def _gain_residual(gain):
+    rng = np.random.default_rng(17)
+    src16 = rng.integers(0, 65536, size=(53, 71, 3)).astype(np.uint16)
+    hm = np.array([0.97, -0.12, 9.25, 0.11, 1.02, -4.5, 3e-4, -2e-4, 1.0], np.float32)
+    w, h = 80, 60
+    ref = po.apply_gain(po.warp_perspective(src16, w, h, hm, 0, 0), gain)
+    rgb, cov = po.warp_float(src16.astype(np.float32) / np.float32(65535), hm, gain, 0, 0, w, h, fp64=False)
+    on = cov.astype(bool)
+    got = np.floor(rgb.astype(np.float64) * 65535.0)
+    return int(np.abs(got[on] - ref[on].astype(np.float64)).max()), float(rgb.max())
+
+
+@pytest.mark.parametrize("gain", [0.61, 1.3, 1.9])
+def test_float_warp_gain_matches_rgb48_apply_gain(gain):
+    # the RGB48 chain truncates twice (warp, then gain): at most
+    # ceil(gain) + 1 LSB apart; gain 1.9 saturates most pixels, which both
+    # chains clamp to full scale
+    worst, top = _gain_residual(gain)
+    assert worst <= int(np.ceil(gain)) + 1, worst
+    assert top <= 1.0
+
+
+def test_float_warp_gain_pin_fails_on_gain_after_clamp():
+    with po.planted(2):
+        worst, top = _gain_residual(1.9)
+    assert top > 1.0 and worst > 1000
