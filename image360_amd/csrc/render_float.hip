@@ -127,6 +127,10 @@ __global__ void rbox_init_kernel(RItem* items, const RChunk* chunks, int n) {
 // --------------------------------------------------------------- warp ---
 constexpr int kWarpTW = 64, kWarpTH = 16;
 
+// RGB48 frames are read as v * (1/65535) in binary32, whatever the form's
+// maths type (the value a binary32 [0,1] frame of the same image holds).
+constexpr float kInv65535 = 1.0f / 65535.0f;
+
 template <typename M>
 __device__ __forceinline__ void fetch_rgb(const void* src, int fmt, size_t idx, M (&o)[3]) {
     if (fmt == I360HIP_PIX_F16) {
@@ -134,6 +138,11 @@ __device__ __forceinline__ void fetch_rgb(const void* src, int fmt, size_t idx, 
         o[0] = static_cast<M>(static_cast<float>(p[0]));
         o[1] = static_cast<M>(static_cast<float>(p[1]));
         o[2] = static_cast<M>(static_cast<float>(p[2]));
+    } else if (fmt == I360HIP_PIX_U16) {
+        const auto* p = gptr(static_cast<const uint16_t*>(src) + idx * 3);
+        o[0] = static_cast<M>(static_cast<float>(p[0]) * kInv65535);
+        o[1] = static_cast<M>(static_cast<float>(p[1]) * kInv65535);
+        o[2] = static_cast<M>(static_cast<float>(p[2]) * kInv65535);
     } else {
         const auto* p = gptr(static_cast<const float*>(src) + idx * 3);
         o[0] = static_cast<M>(p[0]); o[1] = static_cast<M>(p[1]); o[2] = static_cast<M>(p[2]);
@@ -1617,7 +1626,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             }
             const i360hip_frame& fr = frames[f];
             if (!fr.pixels || fr.width < 2 || fr.height < 2 ||
-                (fr.format != I360HIP_PIX_F16 && fr.format != I360HIP_PIX_F32)) {
+                (fr.format != I360HIP_PIX_F16 && fr.format != I360HIP_PIX_F32 && fr.format != I360HIP_PIX_U16)) {
                 st.set(I360HIP_INVALID_PARAM, "frame %d: invalid pixels/size/format", f);
                 return false;
             }

@@ -153,6 +153,48 @@ def small(seed=20261203):
     return Geometry(fwv, fhv, Hs, 2400, 1100, chunks, np.ones(4, np.float32))
 
 
+# Demosaic parameters of the small configuration's raw frames: 14-bit RGGB
+# (LibRaw filters 0x94949494), black level 600 DN, daylight-like white
+# balance and a mild camera-to-sRGB matrix (rows sum to 1).
+BAYER_RGGB = 0x94949494
+BAYER_BLACK = 600
+BAYER_CAM_MUL = np.array([1.9, 1.0, 1.55, 1.0], np.float32)
+BAYER_RGB_CAM = np.array([[1.45, -0.3, -0.15, 0.0], [-0.12, 1.3, -0.18, 0.0], [0.04, -0.34, 1.3, 0.0]], np.float32)
+
+
+def bayer_mosaic(rgb, rng, gain=0.5, read_sigma=3.0, scale=12000.0):
+    """14-bit RGGB raw of a float RGB image in [0,1] (h, w, 3): signal
+    = black + scale * v, Poisson shot noise at `gain` DN per electron plus
+    Gaussian read noise of `read_sigma` DN, rounded and clipped to
+    0..16383.  numpy PCG64 draws, so the same seed gives the same frame on
+    every machine."""
+    h, w = rgb.shape[:2]
+    cfa = np.empty((h, w), np.float64)
+    cfa[0::2, 0::2] = rgb[0::2, 0::2, 0]
+    cfa[0::2, 1::2] = rgb[0::2, 1::2, 1]
+    cfa[1::2, 0::2] = rgb[1::2, 0::2, 1]
+    cfa[1::2, 1::2] = rgb[1::2, 1::2, 2]
+    signal = cfa * scale
+    dn = rng.poisson(signal / gain) * gain + BAYER_BLACK + rng.normal(0.0, read_sigma, size=signal.shape)
+    return np.clip(np.rint(dn), 0, 16383).astype(np.uint16)
+
+
+def small_bayer(seed=20261203):
+    """The small configuration's inputs: the small() geometry and its 4
+    14-bit RGGB raw frames of 1024x768, cut from one seeded 2400x1100 scene
+    (sum-of-sinusoids texture, periods 8-128 px, plus rectangles) through
+    each frame's homography, with Poisson-Gaussian noise (gain 0.5, read
+    sigma 3 DN).  Returns (geometry, [cfa u16 (768, 1024)])."""
+    g = small(seed)
+    scene = make_scene(seed, g.canvas_w, g.canvas_h, cell=256)
+    rng = np.random.default_rng(seed + 11)
+    cfas = []
+    for i in range(g.num_frames):
+        rgb = render_frame_torch(scene, g.H[i], int(g.frame_w[i]), int(g.frame_h[i]), device="cpu").numpy()
+        cfas.append(bayer_mosaic(rgb.astype(np.float64), rng))
+    return g, cfas
+
+
 def uneven_chunks(seed=20261203, count=512, lo=256, hi=4096, base=None):
     """`count` render chunks with log-uniform edge lengths in [lo, hi], each
     placed where 1..6 frames of the gigapixel geometry overlap it."""

@@ -176,14 +176,15 @@ int i360hip_blend_chunk_u16(i360hip_ctx* ctx, const uint16_t* const* warped, int
  *             (chunks without such a tile skip the reduce and compose of
  *             the levels above K).  Chunks with fewer than 4 levels are
  *             blended plainly.
- * Frames: interleaved RGB (3 values per pixel), FP16 or FP32, device
- * memory.  Chunks name their contributors by index into `frames` through
+ * Frames: interleaved RGB (3 values per pixel), FP16, FP32 or RGB48
+ * (uint16, each value v read as v * (1/65535) in binary32: the demosaic's
+ * output and the reference's stored colour blobs), device memory.  Chunks name their contributors by index into `frames` through
  * contributors[first .. first+count-1] (at most 32 are used, in order).
  * No counterpart in IComputeBackend (the reference blends on the host). */
 enum {
     I360HIP_PIX_F16 = 0,
     I360HIP_PIX_F32 = 1,
-    I360HIP_PIX_U16 = 2 /* RGB48 (export composite input only) */
+    I360HIP_PIX_U16 = 2 /* RGB48 */
 };
 enum {
     I360HIP_RENDER_FP64 = 0,
@@ -204,7 +205,7 @@ typedef struct i360hip_frame {
     const void* pixels; /* device, width*height*3 values */
     int width;
     int height;
-    int format;         /* I360HIP_PIX_F16 or I360HIP_PIX_F32 */
+    int format;         /* I360HIP_PIX_F16, I360HIP_PIX_F32 or I360HIP_PIX_U16 */
     float gain;         /* applied when |gain-1| > 1e-6 (RenderExecutor.cpp:111) */
     double h[9];        /* forward homography frame -> canvas, row-major */
 } i360hip_frame;

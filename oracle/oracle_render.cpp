@@ -18,6 +18,10 @@
 //     smoothness weight rescaled to 2/65535 per pixel;
 //   * compose: SeamBlend.cpp:447-546 in the scalar type T, output clamped to
 //     [0,1] (no u16 rounding), zero where no contributor covers.
+// Loops whose iterations write disjoint outputs with the same arithmetic
+// run on OpenMP threads (rows of a plane, pixels, contributors' chamfer
+// planes); every value is computed exactly as in a serial run, and the
+// accumulation over contributors keeps its order.
 // T = double is the FP64 ground truth (inverse homography and coordinate
 // map in binary64 too); T = float with a binary32 inverse homography is the
 // plain FP32 form.
@@ -51,6 +55,7 @@ void Blur5T(const std::vector<T>& src, int w, int h, int ch, std::vector<T>& dst
     dst.assign(src.size(), T(0));
     auto cx = [w](int x) { return x < 0 ? 0 : (x > w - 1 ? w - 1 : x); };
     auto cy = [h](int y) { return y < 0 ? 0 : (y > h - 1 ? h - 1 : y); };
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x)
             for (int c = 0; c < ch; ++c) {
@@ -59,6 +64,7 @@ void Blur5T(const std::vector<T>& src, int w, int h, int ch, std::vector<T>& dst
                 T d = k1 * row[cx(x + 1) * ch + c], e = k0 * row[cx(x + 2) * ch + c];
                 tmp[Px(x, y, w) * ch + c] = (((a + b) + m) + d) + e;
             }
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x)
             for (int c = 0; c < ch; ++c) {
@@ -76,6 +82,7 @@ void ReduceT(const std::vector<T>& src, int w, int h, int ch, std::vector<T>& ds
     ow = (w + 1) / 2;
     oh = (h + 1) / 2;
     dst.assign(static_cast<size_t>(ow) * oh * ch, T(0));
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < oh; ++y)
         for (int x = 0; x < ow; ++x)
             for (int c = 0; c < ch; ++c)
@@ -87,6 +94,7 @@ void ExpandT(const std::vector<T>& src, int w, int h, int ch, int ow, int oh, st
     dst.assign(static_cast<size_t>(ow) * oh * ch, T(0));
     T sx = (w > 1) ? T(w - 1) / T(std::max(1, ow - 1)) : T(0);
     T sy = (h > 1) ? T(h - 1) / T(std::max(1, oh - 1)) : T(0);
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < oh; ++y) {
         T fy = sy * T(y);
         int y0 = std::min(static_cast<int>(fy), h - 1), y1 = std::min(y0 + 1, h - 1);
@@ -124,6 +132,7 @@ void WarpF(const float* src, int sw, int sh, const T* m, float gain, int x0, int
     rgb.assign(n * 3, T(0));
     cov.assign(n, 0);
     const bool useGain = std::fabs(gain - 1.0f) >= 1e-6f;
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < h; ++y) {
         T Y = T(y0 + y);
         for (int x = 0; x < w; ++x) {
@@ -256,6 +265,7 @@ void OwnerMapT(const std::vector<std::vector<T>>& rgb, const std::vector<std::ve
     std::vector<const float*> dp(c);
     std::vector<std::array<int, 4>> box(c);
     std::vector<int> valid(c);
+#pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < c; ++i) {
         orc_coverage_distance(cov[i].data(), w, h, dist[i].data());
         dp[i] = dist[i].data();
@@ -290,6 +300,7 @@ void ComposeT(const std::vector<std::vector<T>>& rgb, const std::vector<std::vec
     for (int i = 0; i < c; ++i) {
         if (!owns[i]) continue;
         std::vector<T> wm(n), num(n * 3), den(n);
+        #pragma omp parallel for schedule(static)
         for (size_t p = 0; p < n; ++p) {
             wm[p] = owner[p] == i ? T(1) : T(0);
             T m = cov[i][p] ? T(1) : T(0);
@@ -305,6 +316,7 @@ void ComposeT(const std::vector<std::vector<T>>& rgb, const std::vector<std::vec
             g[l].w = np[l].w; g[l].h = np[l].h;
             size_t m = static_cast<size_t>(g[l].w) * g[l].h;
             g[l].d.resize(m * 3);
+            #pragma omp parallel for schedule(static)
             for (size_t p = 0; p < m; ++p) {
                 T dv = dpy[l].d[p];
                 T inv = dv > T(1e-6) ? T(1) / dv : T(0);
@@ -320,6 +332,7 @@ void ComposeT(const std::vector<std::vector<T>>& rgb, const std::vector<std::vec
             std::vector<T> up;
             if (l + 1 < L) ExpandT(g[l + 1].d, g[l + 1].w, g[l + 1].h, 3, g[l].w, g[l].h, up);
             size_t m = static_cast<size_t>(band[l].w) * band[l].h;
+            #pragma omp parallel for schedule(static)
             for (size_t p = 0; p < m; ++p) {
                 T wt = wp[l].d[p];
                 for (int k = 0; k < 3; ++k) {
@@ -336,9 +349,11 @@ void ComposeT(const std::vector<std::vector<T>>& rgb, const std::vector<std::vec
         std::vector<T> up;
         ExpandT(rec, rw, rh, 3, band[l].w, band[l].h, up);
         rec.resize(up.size());
+        #pragma omp parallel for schedule(static)
         for (size_t p = 0; p < up.size(); ++p) rec[p] = band[l].d[p] + up[p];
         rw = band[l].w; rh = band[l].h;
     }
+    #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n; ++p)
         for (int k = 0; k < 3; ++k) {
             T v = rec[p * 3 + k];

@@ -237,3 +237,43 @@ def test_rgb48_output_is_the_rounded_float_output(hip, case, mode):
     again, _ = _gpu_render(hip, g, dev, mode)  # back to FP32 outputs: a fresh plan, not the RGB48 graph
     for k in range(len(g.chunks)):
         assert np.array_equal(again[k], f32[k]), k
+
+
+@pytest.mark.parametrize("mode", [_abi.RENDER_PRECISION, _abi.RENDER_BATCHED])
+def test_replans_on_two_streams_without_host_sync(hip, mode):
+    # different chunk sets rendered back to back on two streams with no host
+    # synchronisation: each call re-plans (new tables and workspace) while
+    # the previous one may still run on the other stream; every output must
+    # equal its single-stream render
+    import torch
+
+    ga, fa = small_case(seed=3, chunk=256)
+    gb, fb = grid_case(seed=9, chunk=160)
+    da, db = _upload(fa), _upload(fb)
+    note(f"two-stream re-plans, mode {mode}")
+    ref_a, _ = _gpu_render(hip, ga, da, mode)
+    ref_b, _ = _gpu_render(hip, gb, db, mode)
+
+    def args(g, dev):
+        frames = [{"ptr": t.data_ptr(), "w": int(g.frame_w[i]), "h": int(g.frame_h[i]), "fmt": _abi.PIX_F16,
+                   "gain": float(g.gains[i]), "H": g.H[i]} for i, t in enumerate(dev)]
+        outs, descs, contrib = [], [], []
+        for c in g.chunks:
+            o = torch.zeros((c.h, c.w, 3), dtype=torch.float32, device="cuda")
+            descs.append({"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": len(contrib),
+                          "count": len(c.contributors), "out": o.data_ptr(), "owner": None})
+            contrib += c.contributors
+            outs.append(o)
+        return frames, descs, contrib, outs
+
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    calls = [(ga, da, s1), (gb, db, s2), (ga, da, s1), (gb, db, s2)]
+    prepared = [args(g, d) for g, d, _ in calls]
+    torch.cuda.synchronize()
+    for (g, d, s), (frames, descs, contrib, _) in zip(calls, prepared):
+        hip.check(hip.render_chunks(frames, descs, contrib, BANDS, mode, s.cuda_stream), "render_chunks")
+    torch.cuda.synchronize()
+    for n, ((g, _, _), (_, _, _, outs)) in enumerate(zip(calls, prepared)):
+        ref = ref_a if g is ga else ref_b
+        for k in range(len(g.chunks)):
+            assert np.array_equal(outs[k].cpu().numpy(), ref[k]), (n, k)
