@@ -115,6 +115,7 @@ SIGNATURES = {
     "i360hip_compute_lab_stats": (i32, [vp, vp, i32, i32, vp, vp]),
     "i360hip_apply_reinhard_u16": (i32, [vp, vp, i32, i32, vp, vp, vp, vp]),
     "i360hip_compute_lab_stats_dev": (i32, [vp, vp, i32, i32, vp, vp, vp]),
+    "i360hip_compute_lab_stats_dev_noalloc": (i32, [vp, vp, i32, i32, vp, vp, vp]),
     "i360hip_apply_reinhard_u16_dev": (i32, [vp, vp, i32, i32, vp, vp, vp, vp, vp]),
 }
 

@@ -121,6 +121,11 @@ static int create_locked(int device, i360hip_ctx** out_ctx) {
         std::snprintf(g_create_error, sizeof(g_create_error), "hipSetDevice: %s", HipName(e));
         return I360HIP_NO_GPU;
     }
+    if (!color_scratch_setup(device)) {
+        std::snprintf(g_create_error, sizeof(g_create_error),
+                      "hipGetSymbolAddress of the colour scratch array failed on device %d", device);
+        return I360HIP_DEVICE_ERROR;
+    }
     auto* ctx = new i360hip_ctx();
     ctx->device = device;
     ctx->props = props;

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -211,6 +212,25 @@ struct ColorWork {
     double* result;
 };
 
+// ---- scratch of the allocation-free statistics entry point --------------
+// The sRGB table, the block partials (for at most kNoAllocMaxBlocks blocks)
+// and the result, in one file-scope device array.  Each GPU holds its own
+// copy from the library's load until the process ends; its device address
+// on each GPU is looked up once (color_scratch_setup, from i360hip_create
+// with that device current).  Every call rewrites all of it before reading
+// it, so nothing needs clearing; calls that overlap on one GPU (two
+// contexts, or streams, at once) share it and race.
+constexpr int kNoAllocMaxBlocks = 2048;
+constexpr size_t kNoAllocPartialOff = sizeof(float) * kLut;
+constexpr size_t kNoAllocResultOff = kNoAllocPartialOff + sizeof(double) * 6 * kNoAllocMaxBlocks;
+constexpr size_t kNoAllocBytes = (kNoAllocResultOff + sizeof(double) * 6 + 255) & ~static_cast<size_t>(255);
+
+__device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
 ColorWork color_work(i360hip_ctx* ctx, Status& st) {
     const size_t oP = sizeof(float) * kLut;
     const size_t oR = oP + sizeof(double) * 6 * stats_blocks(ctx);
@@ -225,14 +245,14 @@ int grid_for(const i360hip_ctx* ctx, size_t n) {
     return static_cast<int>(std::max<size_t>(1, std::min(cap, (n + 255) / 256)));
 }
 
-// Enqueues the statistics of a device image and copies the six results to
-// the host (synchronous on s).
+// Enqueues the statistics of a device image on the table, partials and
+// result of `w` and copies the six results to the host (synchronous on s).
+// `timed`: bracket the launches with the context's profiling events when
+// profiling is on (the allocation-free entry point takes no events).
 int run_stats(i360hip_ctx* ctx, const uint16_t* rgb, size_t n, double* mean, double* sd, hipStream_t s,
-              Status& st) {
-    ColorWork w = color_work(ctx, st);
-    if (!st.ok()) return st.code;
+              Status& st, const ColorWork& w, bool timed = true) {
     const int blocks = stats_blocks(ctx);
-    ProfScope ps(ctx, s, "lab_stats", 6.0 * n);
+    ProfScope ps(timed ? ctx : nullptr, s, "lab_stats", 6.0 * n);
     srgb_lut_kernel<<<kLut / 256, 256, 0, s>>>(w.lut);
     lab_stats_kernel<<<blocks, kStatsBlock, 0, s>>>(rgb, n, w.lut, w.partial);
     lab_stats_final_kernel<<<1, kStatsBlock, 0, s>>>(w.partial, blocks, n, w.result);
@@ -266,6 +286,19 @@ int run_reinhard(i360hip_ctx* ctx, uint16_t* rgb, size_t n, const double* sm, co
 }
 
 }  // namespace
+
+// Looks up the scratch array's address on `device` (the current device)
+// once; returns false when the runtime cannot give it.
+bool color_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return false;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return true;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return false;
+    g_scratch_ptr[device] = static_cast<uint8_t*>(p);
+    return true;
+}
+
 }  // namespace i360
 
 using namespace i360;
@@ -281,7 +314,36 @@ int i360hip_compute_lab_stats_dev(i360hip_ctx* ctx, const uint16_t* rgb, int wid
     Status st;
     if (hipSetDevice(ctx->device) != hipSuccess) return Fail(ctx, I360HIP_DEVICE_ERROR, "hipSetDevice failed");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-    run_stats(ctx, rgb, static_cast<size_t>(width) * height, out_mean, out_std, s, st);
+    const ColorWork w = color_work(ctx, st);
+    if (st.ok()) run_stats(ctx, rgb, static_cast<size_t>(width) * height, out_mean, out_std, s, st, w);
+    return Finish(ctx, st);
+}
+
+int i360hip_compute_lab_stats_dev_noalloc(i360hip_ctx* ctx, const uint16_t* rgb, int width, int height,
+                                          double* out_mean, double* out_std, void* stream) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!rgb || !out_mean || !out_std) return Fail(ctx, I360HIP_INVALID_PARAM, "Null pointer argument.");
+    if (width <= 0 || height <= 0) return Fail(ctx, I360HIP_INVALID_PARAM, "Invalid dimensions.");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Status st;
+    if (hipSetDevice(ctx->device) != hipSuccess) return Fail(ctx, I360HIP_DEVICE_ERROR, "hipSetDevice failed");
+    const int blocks = stats_blocks(ctx);
+    if (blocks > kNoAllocMaxBlocks) {
+        st.set(I360HIP_INVALID_PARAM,
+               "the statistics of this device need %d block partials; the allocation-free scratch holds %d",
+               blocks, kNoAllocMaxBlocks);
+        return Finish(ctx, st);
+    }
+    uint8_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratch_ptr[ctx->device];
+    }
+    if (!base) return Fail(ctx, I360HIP_DEVICE_ERROR, "the scratch array's address on this device is unknown");
+    const ColorWork w{reinterpret_cast<float*>(base), reinterpret_cast<double*>(base + kNoAllocPartialOff),
+                      reinterpret_cast<double*>(base + kNoAllocResultOff)};
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    run_stats(ctx, rgb, static_cast<size_t>(width) * height, out_mean, out_std, s, st, w, false);
     return Finish(ctx, st);
 }
 
@@ -298,7 +360,8 @@ int i360hip_compute_lab_stats(i360hip_ctx* ctx, const uint16_t* rgb, int width, 
     if (!st.ok()) return Finish(ctx, st);
     if (hipMemcpyAsync(d, rgb, n * 6, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         return Fail(ctx, I360HIP_DEVICE_ERROR, "image upload failed");
-    run_stats(ctx, d, n, out_mean, out_std, ctx->stream, st);
+    const ColorWork w = color_work(ctx, st);
+    if (st.ok()) run_stats(ctx, d, n, out_mean, out_std, ctx->stream, st, w);
     return Finish(ctx, st);
 }
 

@@ -135,6 +135,9 @@ void* ctx_scratch(i360hip_ctx* ctx, int slot, size_t bytes, Status& st);
 // shared by every thread that calls it) and for the calling thread
 // (i360hip_thread_error, which no other thread's failure can replace).
 void set_error(i360hip_ctx* ctx, const char* msg);
+// Looks up, once per device, the device address of the allocation-free
+// statistics entry point's scratch array (color.hip); `device` is current.
+bool color_scratch_setup(int device);
 // Drops the context's cached render plan and graph.
 void render_release(i360hip_ctx* ctx);
 }  // namespace i360

@@ -428,6 +428,19 @@ int i360hip_compute_lab_stats_dev(i360hip_ctx* ctx, const uint16_t* rgb, int wid
 int i360hip_apply_reinhard_u16_dev(i360hip_ctx* ctx, uint16_t* rgb_in_out, int width, int height,
                                    const double* src_mean, const double* src_std, const double* ref_mean,
                                    const double* ref_std, void* stream);
+/* i360hip_compute_lab_stats_dev that allocates nothing: no counterpart in
+ * the reference; it is for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results, bit for bit (same kernels and
+ * grid).  Its scratch (the sRGB table, the block partials, the result;
+ * 352 KB) is a static device array that each GPU holds from the library's
+ * load until the process ends, shared by every call on that GPU: calls
+ * that overlap on one GPU race (calls on one context are serialised).
+ * INVALID_PARAM when the device needs more block partials than the array
+ * holds (2048; MI355X needs 1024).  Returns after the result has reached
+ * the host. */
+int i360hip_compute_lab_stats_dev_noalloc(i360hip_ctx* ctx, const uint16_t* rgb, int width, int height,
+                                          double* out_mean, double* out_std, void* stream);
 
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop
