@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the burst median+demosaic and uneven-chunk measurements")
     ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "image360_amd", "surrogate_weights.npz"))
+    ap.add_argument("--only", choices=["burst", "uneven", "small"], default=None,
+                    help="run only this workload's GPU measurement and print its figures (for profiling runs)")
     ap.add_argument("--surrogate-threshold", type=float, default=1e-5,
                     help="mean predicted squared error per tile above which the surrogate falls back (1e-5 = 50 dB)")
     return ap.parse_args()
@@ -117,7 +119,202 @@ def cpu_baseline(geom, frames_host_fn, chunks, threads, bands):
                       f"BlendChunkContributors pipeline with {bands} bands; {dt:.1f} s wall"}
 
 
+REF_FLAGS = ("g++ -std=c++20 -O3 -DNDEBUG (CMake's Release flags); the AVX-512 tier's units add -mavx512f "
+             "-mavx512bw -mavx512dq -mavx512vl -mfma (WindowsApp.Core/CMakeLists.txt:67-76)")
+TIERS = {2: "avx512", 1: "avx2", 0: "scalar", -1: "none (oracle port)"}
+
+
+def cpu_info(threads):
+    """The host the CPU baseline ran on: model, logical CPUs, physical cores,
+    SMT, the CPUs this process may use, the threads used, the reference's
+    SIMD tier on this host and the compiler flags of the reference build."""
+    from oracle import pyoracle as po
+
+    model, cores, logical = "", set(), 0
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and not model:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count()
+    return {"model": model, "logical_cpus": logical or os.cpu_count(), "physical_cores": len(cores) or None,
+            "smt": smt, "cpus_allowed": allowed, "threads_used": threads,
+            "simd_tier": TIERS.get(po.ref_tier(), "?"), "reference_build": REF_FLAGS}
+
+
+def stratified(items, key, count):
+    """`count` items at evenly spaced quantiles of `key` (a sample whose
+    distribution follows the whole set's)."""
+    order = sorted(items, key=key)
+    if count >= len(order):
+        return order
+    return [order[int((i + 0.5) * len(order) / count)] for i in range(count)]
+
+
+def cpu_burst(frames_host, shifts, gpu_median, threads):
+    """The reference MedianStack (median of 16, sigma 2) over the aligned
+    planes and DemosaicBayer of the result, at the host's SIMD tier, each
+    split into one row band per thread (a 2-row halo keeps the demosaic's
+    neighbourhoods and Bayer phase); GB/s of input read, as the GPU figure."""
+    from oracle import pyoracle as po
+
+    tier = po.ref_tier()
+    h, w = frames_host[0].shape
+    hc = h - 9
+    n = hc * w
+    ptrs = [f.ctypes.data + 2 * ((8 - dy) * w + (8 - dx)) for f, (dx, dy) in zip(frames_host, shifts)]
+    med = np.empty(n, np.uint16)
+    per = max(3, (n // threads) // 3 * 3)
+    bands = [(s0, min(per, n - s0)) for s0 in range(0, n, per)]
+
+    def mband(b):
+        s0, cnt = b
+        po.ref_median_stack_ptrs([p + 2 * s0 for p in ptrs], med.ctypes.data + 2 * s0, cnt, 2.0, tier=tier)
+
+    rgb = np.empty((hc, w, 3), np.uint16)
+    per_r = max(2, ((hc + threads - 1) // threads + 1) // 2 * 2)
+    cam = [2.0, 1.0, 1.6, 1.0]
+    rc = [[1.7, -0.5, -0.2, 0.0], [-0.2, 1.5, -0.3, 0.0], [0.05, -0.55, 1.5, 0.0]]
+
+    def dband(y0):
+        y1 = min(hc, y0 + per_r)
+        a, z = max(0, y0 - 2), min(hc, y1 + 2)
+        tmp = np.empty((z - a, w, 3), np.uint16)
+        po.ref_demosaic_ptrs(med.ctypes.data + 2 * a * w, w, z - a, 512, cam, rc, 0x94949494, tmp.ctypes.data,
+                             tier=tier)
+        rgb[y0:y1] = tmp[y0 - a: y1 - a]
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(mband, bands))
+        t1 = time.perf_counter()
+        list(ex.map(dband, range(0, hc, per_r)))
+        t2 = time.perf_counter()
+    read = 16 * n * 2
+    # outside the timing: GCC fuses multiply-adds in the reference's AVX
+    # units (its default floating-point contraction with -mfma), which moves
+    # a few clip decisions; the same sources built without contraction give
+    # the Scalar tier's values, which the GPU reproduces bit for bit
+    nc_equal = None
+    if po.ref_nc() is not None:
+        med_nc = np.empty(n, np.uint16)
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(lambda b: po.ref_median_stack_ptrs([p + 2 * b[0] for p in ptrs], med_nc.ctypes.data + 2 * b[0],
+                                                           b[1], 2.0, tier=tier, lib=po.ref_nc()), bands))
+        nc_equal = bool(np.array_equal(med_nc, gpu_median))
+    return {"median_ms": round((t1 - t0) * 1e3, 1), "median_input_gb_s": round(read / (t1 - t0) / 1e9, 2),
+            "median_plus_demosaic_ms": round((t2 - t0) * 1e3, 1), "input_gb_s": round(read / (t2 - t0) / 1e9, 2),
+            "median_values_differing_from_gpu": int(np.count_nonzero(med != gpu_median)),
+            "median_equals_gpu_without_fp_contraction": nc_equal, "cores": threads, "kind": "reference",
+            "sample": f"the whole workload: median of 16 planes of {w}x{hc}, then demosaic (u16 out), "
+                      f"{len(bands)} median bands and {len(range(0, hc, per_r))} demosaic row bands on {threads} "
+                      f"threads, {TIERS.get(tier)} tier"}
+
+
+def cpu_small(geom, cfas, threads):
+    """The small raw-frame chain on the compiled reference: DemosaicBayer of
+    the 4 raws (one ingest task per image, RawIngestExecutor.cpp:90), then
+    the chunk's render task (RenderExecutor.cpp:70-140: binary32 inverse,
+    WarpPerspective, BlendChunkContributors at 5 bands) on one thread."""
+    from image360_amd import synth
+    from image360_amd.render import invert_homography_f32
+    from oracle import pyoracle as po
+
+    tier = po.ref_tier()
+    c = geom.chunks[0]
+    with cf.ThreadPoolExecutor(max_workers=min(threads, len(cfas))) as ex:
+        t0 = time.perf_counter()
+        rgb = list(ex.map(lambda x: po.ref_demosaic(x, synth.BAYER_BLACK, synth.BAYER_CAM_MUL, synth.BAYER_RGB_CAM,
+                                                      synth.BAYER_RGGB, tier=tier), cfas))
+        t1 = time.perf_counter()
+    warped = [po.ref_warp_perspective(rgb[i], c.w, c.h, invert_homography_f32(geom.H[i]), c.x, c.y, tier=tier)
+              for i in c.contributors]
+    po.ref_blend_chunk_bands(warped, 5)
+    t2 = time.perf_counter()
+    return {"ingest_ms": round((t1 - t0) * 1e3, 1), "render_ms": round((t2 - t1) * 1e3, 1),
+            "total_ms": round((t2 - t0) * 1e3, 1), "render_mp_s": round(c.area / (t2 - t1) / 1e6, 2),
+            "cores": min(threads, len(cfas)), "kind": "reference",
+            "sample": f"the whole workload: 4 demosaics in parallel, then one {c.w}x{c.h} chunk render task, "
+                      f"{TIERS.get(tier)} tier"}
+
+
+def bench_small(hip, torch, log):
+    """The small raw-frame workload on the GPU: demosaic of the 4 raws into
+    resident RGB48 frames, then the batched render of the 2400x1100 chunk at
+    5 bands (FP32 out), checked against the FP64 form."""
+    from image360_amd import _abi, synth
+
+    geom, cfas = synth.small_bayer()
+    c = geom.chunks[0]
+    d_cfa = [torch.from_numpy(x.view(np.int16)).cuda() for x in cfas]
+    rgb = [torch.empty((x.shape[0], x.shape[1], 3), dtype=torch.int16, device="cuda") for x in cfas]
+    fdesc = [{"ptr": f.data_ptr(), "w": int(geom.frame_w[i]), "h": int(geom.frame_h[i]), "fmt": _abi.PIX_U16,
+              "gain": 1.0, "H": geom.H[i]} for i, f in enumerate(rgb)]
+    out = torch.empty((c.h, c.w, 3), dtype=torch.float32, device="cuda")
+    ref = torch.empty_like(out)
+    desc = [{"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": 0, "count": len(c.contributors), "out": out.data_ptr(),
+             "owner": None}]
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def demosaic_all():
+        for x, d, o in zip(cfas, d_cfa, rgb):
+            hip.check(hip.demosaic_dev(d.data_ptr(), x.shape[1], x.shape[0], synth.BAYER_BLACK, synth.BAYER_CAM_MUL,
+                                       synth.BAYER_RGB_CAM, synth.BAYER_RGGB, o.data_ptr(), stream=sh), "demosaic")
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    with torch.cuda.stream(stream):
+        demosaic_all()
+        hip.check(hip.render_chunks(fdesc, desc, c.contributors, 5, _abi.RENDER_BATCHED, sh), "small render")
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(5):
+            ev[0].record(stream)
+            demosaic_all()
+            ev[1].record(stream)
+            hip.check(hip.render_chunks(fdesc, desc, c.contributors, 5, _abi.RENDER_BATCHED, sh), "small render")
+            ev[2].record(stream)
+            torch.cuda.synchronize()
+            t = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]))
+            best = t if best is None or sum(t) < sum(best) else best
+        hip.check(hip.render_chunks(fdesc, [dict(desc[0], out=ref.data_ptr())], c.contributors, 5, _abi.RENDER_FP64,
+                                    sh), "small fp64")
+        torch.cuda.synchronize()
+    e = (out.double() - ref.double())
+    mse = float((e * e).mean())
+    r = {"frames": "4 x 1024x768 14-bit RGGB raws -> 2400x1100 at 5 bands", "demosaic_ms": round(best[0], 3),
+         "render_ms": round(best[1], 3), "render_mp_s": round(c.area / (best[1] / 1e3) / 1e6, 1),
+         "psnr_db_vs_fp64": round(10 * np.log10(1.0 / mse), 2) if mse > 0 else float("inf"),
+         "max_abs_err_vs_fp64": float(e.abs().max())}
+    log(f"small raw-frame chain: {r}")
+    return r, geom, cfas
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_gigapixel_batched_rgb48_1group_hbm_traffic.json")
+# serial dependency chains (rows of the chamfer, lines of the seam DP): their
+# time share is reported as latency-bound beside the roofline
+LATENCY_KERNELS = ("chamfer", "seam_scan", "seam_dp", "seam_reassign")
 
 
 def cpu_export(geom, chunks, rgb48, offs, log, max_dim=4096):
@@ -195,7 +392,7 @@ def bench_burst(hip, torch, log):
          "median_input_gb_s": round(read / (t_med / 1e3) / 1e9, 1), "median_plus_demosaic_ms": round(t_all, 3),
          "input_gb_s": round(read / (t_all / 1e3) / 1e9, 1)}
     log(f"burst median-of-16 + demosaic: {r}")
-    return r
+    return r, frames, shifts, med
 
 
 def bench_uneven(hip, torch, geom, fdesc, args, log, count=512):
@@ -231,7 +428,7 @@ def bench_uneven(hip, torch, geom, fdesc, args, log, count=512):
         res[name] = {"ms": round(ms, 2), "mp_s": round(area / (ms / 1e3) / 1e6, 1)}
     res["batched_vs_loop"] = round(res["batched"]["mp_s"] / res["per_chunk_loop"]["mp_s"], 2)
     log(f"uneven chunks: {res}")
-    return res
+    return res, g
 
 
 def load_surrogate(hip, args, log):
@@ -399,6 +596,11 @@ def main():
         f"contributors/chunk {np.mean([len(c.contributors) for c in chunks]):.2f}")
     hip = HipComputeBackend(0)
     hip.check(hip.Initialize(), "Initialize")
+    if args.only in ("burst", "small"):
+        r = bench_burst(hip, torch, log)[0] if args.only == "burst" else bench_small(hip, torch, log)[0]
+        print(json.dumps({"workload": args.only, **r}), flush=True)
+        hip.Shutdown()
+        return
     dev = torch.device("cuda")
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
@@ -421,6 +623,11 @@ def main():
     log(f"generated {geom.num_frames} FP16 frames in {time.time() - t0:.1f} s")
     fdesc = [{"ptr": f.data_ptr(), "w": int(geom.frame_w[i]), "h": int(geom.frame_h[i]), "fmt": _abi.PIX_F16,
               "gain": float(geom.gains[i]), "H": geom.H[i]} for i, f in enumerate(frames)]
+    if args.only == "uneven":
+        r = bench_uneven(hip, torch, geom, fdesc, args, log)[0]
+        print(json.dumps({"workload": "uneven", **r}), flush=True)
+        hip.Shutdown()
+        return
     offs = np.cumsum([0] + [c.area * 3 for c in chunks])
     out = torch.empty(int(offs[-1]), dtype=torch.float32, device=dev)
     contrib, cdesc = [], []
@@ -490,6 +697,37 @@ def main():
         if name == "batched_rgb48":
             outputs[name] = out16
 
+    # A call that cannot replay a captured graph: new output buffers change
+    # the plan (plan + table upload + capture + instantiate + launch), on the
+    # warm context and on a fresh one (which also allocates its workspace).
+    # Host wall time from the call to the finished render.
+    cold = {}
+    if "batched_rgb48" in forms:
+        head_mode = _abi.RENDER_BATCHED | _abi.RENDER_OUT_RGB48
+        out2 = torch.empty_like(out16)
+        cdesc2 = [dict(d, out=out2.data_ptr() + 2 * int(offs[k])) for k, d in enumerate(cdesc)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            render(head_mode, cdesc2)
+        torch.cuda.synchronize()
+        cold["replan_warm_context_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        hip2 = HipComputeBackend(0)
+        hip2.check(hip2.Initialize(), "Initialize")
+        if tuning:
+            hip2.render_set_tuning(tuning["chunk_groups"], tuning["chamfer_strip"], tuning["grid_share"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            hip2.check(hip2.render_chunks(fdesc, cdesc2, contrib, args.bands, head_mode, sh), "cold render")
+        torch.cuda.synchronize()
+        cold["first_call_fresh_context_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        hip2.Shutdown()
+        del out2
+        torch.cuda.empty_cache()
+        cold["warm_replay_ms"] = forms["batched_rgb48"]["ms_per_step"]
+        log(f"cold calls: {cold}")
+
     # per-kernel event timing of one eager step of a form's launch sequence
     def stage_profile(mode):
         hip.profile_enable(True)
@@ -538,7 +776,7 @@ def main():
                 "profiled_chunk_groups": dom["launches"],
                 "time_share": round(dom["total_ms"] / tot, 3),
                 "latency_bound_share": round(sum(v["total_ms"] for k, v in stats.items()
-                                                 if v["alg_bytes"] == 0) / tot, 3)}
+                                                 if k in LATENCY_KERNELS) / tot, 3)}
 
     # accuracy: FP64 form on a sample of chunks vs the batched / surrogate outputs
     sample = sorted(range(len(chunks)), key=lambda k: (-len(chunks[k].contributors), -chunks[k].area))[:args.psnr_chunks]
@@ -581,24 +819,44 @@ def main():
     if not args.no_extra:
         exp, rgb48 = bench_export(hip, torch, geom, chunks, out, offs, log)
         col, frame48 = bench_color(hip, torch, frames[0], log)
-        extra = {"burst_median_demosaic": bench_burst(hip, torch, log),
-                 "uneven_chunks": bench_uneven(hip, torch, geom, fdesc, args, log),
+        burst, bframes, bshifts, bmed = bench_burst(hip, torch, log)
+        uneven, ugeom = bench_uneven(hip, torch, geom, fdesc, args, log)
+        small, sgeom, scfas = bench_small(hip, torch, log)
+        extra = {"burst_median_demosaic": burst, "uneven_chunks": uneven, "small_raw_chain": small,
                  "preview_export": exp, "colour_transfer": col}
 
     cpu = None
     if not args.no_cpu:
         threads = cpu_threads(args.cpu_threads)
         ncpu = args.cpu_chunks or threads
-        pool = sorted(chunks, key=lambda c: (-c.area, -len(c.contributors)))
-        csample = [c for c in pool if c.area == synth.CHUNK * synth.CHUNK][:ncpu] or pool[:ncpu]
+        # a sample whose contributor counts and areas follow the canvas's
+        csample = stratified(chunks, lambda c: (len(c.contributors), c.area), ncpu)
+        frame_cache = {}
 
         def host_frame(fi):
-            u16 = torch.round(frames[fi].float() * 65535.0).clamp_(0, 65535).to(torch.int32).cpu().numpy()
-            return u16.astype(np.uint16).reshape(int(geom.frame_h[fi]), int(geom.frame_w[fi]), 3)
+            if fi not in frame_cache:
+                u16 = torch.round(frames[fi].float() * 65535.0).clamp_(0, 65535).to(torch.int32).cpu().numpy()
+                frame_cache[fi] = u16.astype(np.uint16).reshape(int(geom.frame_h[fi]), int(geom.frame_w[fi]), 3)
+            return frame_cache[fi]
 
         log(f"CPU baseline: {len(csample)} chunks on {threads} threads")
         cpu = cpu_baseline(geom, host_frame, csample, threads, args.bands)
-        log(f"CPU baseline: {cpu['value']:.2f} MP/s")
+        cpu["sample"] += (f"; mean contributors {np.mean([len(c.contributors) for c in csample]):.2f} "
+                          f"(canvas {np.mean([len(c.contributors) for c in chunks]):.2f}), stratified by "
+                          f"contributor count and area")
+        cpu["host"] = cpu_info(threads)
+        log(f"CPU baseline: {cpu['value']:.2f} MP/s on {cpu['host']}")
+        if extra is not None:
+            hb = [f.cpu().numpy().view(np.uint16) for f in bframes]
+            cpu["burst_median_demosaic"] = cpu_burst(hb, bshifts, bmed.cpu().numpy().view(np.uint16), threads)
+            log(f"CPU burst: {cpu['burst_median_demosaic']}")
+            del hb
+            cpu["small_raw_chain"] = cpu_small(sgeom, scfas, threads)
+            log(f"CPU small raw chain: {cpu['small_raw_chain']}")
+            usample = stratified(ugeom.chunks, lambda c: (c.area, len(c.contributors)), 2 * threads)
+            cu = cpu_baseline(ugeom, host_frame, usample, threads, 6)
+            cpu["uneven_chunks"] = cu
+            log(f"CPU uneven chunks: {cu['value']:.2f} MP/s ({cu['sample']})")
         if rgb48 is not None:
             cpu["preview_export"] = cpu_export(geom, chunks, rgb48, offs, log)
         if frame48 is not None:
@@ -622,7 +880,8 @@ def main():
                    "forms": forms,
                    "psnr_db_vs_fp64": psnr, "max_abs_err_vs_fp64": maxerr,
                    "psnr_sample_chunks": len(sample), "map_max_err_px": map_err, "launch_tuning": tuning,
-                   "speedup_vs_plain": round(head["mp_s"] / forms["plain"]["mp_s"], 2) if "plain" in forms else None},
+                   "speedup_vs_plain": round(head["mp_s"] / forms["plain"]["mp_s"], 2) if "plain" in forms else None,
+                   "cold_calls": cold or None},
         "roofline": roofline, "cpu_baseline": cpu, "other_configs": extra,
     }
     print(json.dumps(line), flush=True)

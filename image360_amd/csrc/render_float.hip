@@ -1991,7 +1991,15 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
     {
         const int i0 = P.chunks[c0].first;
         const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
-        ProfScope ps_(ctx, s, "chamfer", 0.0);
+        // algorithmic bytes: coverage byte in, distance out (forward), then
+        // distance in and out (backward), 13 B per pixel of each item's
+        // level-0 support box (the coverage box lies inside it)
+        double box_px = 0;
+        for (int i = i0; i < i1; ++i) {
+            const int* b = P.items[i].lbox[0];
+            if (b[2] >= b[0] && b[3] >= b[1]) box_px += static_cast<double>(b[2] - b[0] + 1) * (b[3] - b[1] + 1);
+        }
+        ProfScope ps_(ctx, s, "chamfer", 13.0 * box_px);
         // the warp wrote the sheared coverage: no shear pre-pass
         if (i1 > i0 && launch_chamfer(T.chamfer + i0, i1 - i0, P.max_w, s, ctx->tune_strip, true) != hipSuccess) {
             st.set(I360HIP_KERNEL_LAUNCH_FAILED, "chamfer launch failed");

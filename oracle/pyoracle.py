@@ -49,6 +49,21 @@ def ref():
     return _REF
 
 
+_REF_NC = None
+
+
+def ref_nc():
+    """The compiled reference sources built with floating-point contraction
+    off (no fused multiply-adds in the AVX units), or None."""
+    global _REF_NC
+    if _REF_NC is None:
+        path = os.path.join(_HERE, "_ref", "libi360ref_nc.so")
+        if not os.path.exists(path):
+            return None
+        _REF_NC = C.CDLL(path)
+    return _REF_NC
+
+
 def _ptr_array(planes, t=u16p):
     arr = (t * len(planes))()
     for i, p in enumerate(planes):
@@ -332,12 +347,30 @@ def ref_tier():
     return -1 if r is None else r.ref_simd_tier()
 
 
-def ref_median_stack(planes, sigma=2.0, tier=0):
+def ref_median_stack(planes, sigma=2.0, tier=0, lib=None):
     planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
     out = np.zeros_like(planes[0])
     n = out.size // 3
-    ref().ref_median_stack(tier, _ptr_array(planes), len(planes), _p(out, u16p), n, 1, C.c_float(sigma))
+    (lib or ref()).ref_median_stack(tier, _ptr_array(planes), len(planes), _p(out, u16p), n, 1, C.c_float(sigma))
     return out
+
+
+def ref_median_stack_ptrs(plane_ptrs, out_ptr, num_scalars, sigma=2.0, tier=0, lib=None):
+    """The reference MedianStack over `num_scalars` (a multiple of 3) values
+    at raw host addresses (one band of the planes)."""
+    arr = (u16p * len(plane_ptrs))()
+    for i, p in enumerate(plane_ptrs):
+        arr[i] = C.cast(C.c_void_p(p), u16p)
+    (lib or ref()).ref_median_stack(tier, arr, len(plane_ptrs), C.cast(C.c_void_p(out_ptr), u16p), num_scalars // 3, 1,
+                           C.c_float(sigma))
+
+
+def ref_demosaic_ptrs(cfa_ptr, w, h, black, cam_mul, rgb_cam, filters, out_ptr, tier=0):
+    """The reference DemosaicBayer on a w x h CFA block at a raw host address."""
+    cm = np.ascontiguousarray(cam_mul, dtype=np.float32)
+    rc = np.ascontiguousarray(rgb_cam, dtype=np.float32).reshape(12)
+    ref().ref_demosaic_bayer(tier, C.cast(C.c_void_p(cfa_ptr), u16p), w, h, C.c_uint16(black), _p(cm, f32p),
+                             _p(rc, f32p), C.c_uint32(filters), C.cast(C.c_void_p(out_ptr), u16p))
 
 
 def ref_warp_perspective(src, dst_w, dst_h, invH, off_x=0, off_y=0, tier=0):
@@ -349,13 +382,13 @@ def ref_warp_perspective(src, dst_w, dst_h, invH, off_x=0, off_y=0, tier=0):
     return out
 
 
-def ref_demosaic(cfa, black, cam_mul, rgb_cam, filters, tier=0):
+def ref_demosaic(cfa, black, cam_mul, rgb_cam, filters, tier=0, lib=None):
     cfa = np.ascontiguousarray(cfa, dtype=np.uint16)
     h, w = cfa.shape
     cm = np.ascontiguousarray(cam_mul, dtype=np.float32)
     rc = np.ascontiguousarray(rgb_cam, dtype=np.float32).reshape(12)
     out = np.zeros((h, w, 3), np.uint16)
-    ref().ref_demosaic_bayer(tier, _p(cfa, u16p), w, h, C.c_uint16(black), _p(cm, f32p), _p(rc, f32p),
+    (lib or ref()).ref_demosaic_bayer(tier, _p(cfa, u16p), w, h, C.c_uint16(black), _p(cm, f32p), _p(rc, f32p),
                              C.c_uint32(filters), _p(out, u16p))
     return out
 
