@@ -161,6 +161,11 @@ struct TileMap {
     float fx, fy, a00, a01, a10, a11, g0, g1, q0;
     float margin;  // bound on |FP32 position - FP64 position| (source px) inside the tile
     int ok;
+    // every pixel of the tile lies in the chunk and the item's support box,
+    // and its sample is valid with the FP32 position deciding it (exact
+    // position at least twice the margin inside the frame, denominator
+    // above 1/2): the warp may skip every per-pixel test
+    int interior;
 };
 
 __device__ __forceinline__ TileMap tile_map(const double* h, double X0, double Y0) {
@@ -193,6 +198,17 @@ __device__ __forceinline__ TileMap tile_map(const double* h, double X0, double Y
     return t;
 }
 
+// Tile-local FP32 source position of offset (dx, dy) (the precision
+// forms' map where den > 1/2): both warp paths evaluate exactly this.
+__device__ __forceinline__ void tile_pos(const TileMap& tm, int dx, int dy, float den, float& lx, float& ly) {
+    const float r = __builtin_amdgcn_rcpf(den);  // 1 ulp; inside the tile's error margin
+    lx = tm.fx + (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) * r;
+    ly = tm.fy + (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) * r;
+}
+__device__ __forceinline__ float tile_den(const TileMap& tm, int dx, int dy) {
+    return (1.0f + tm.g0 * static_cast<float>(dx)) + tm.g1 * static_cast<float>(dy);
+}
+
 // Validity of canvas pixel (x, y) of the chunk under the FP64 map
 // (WarpPerspective_Scalar.cpp:108-122 in binary64).
 __device__ __forceinline__ bool fp64_valid(const RItem& it, const RChunk& ch, int x, int y) {
@@ -223,12 +239,10 @@ __device__ __forceinline__ bool map_pixel(const RItem& it, const RChunk& ch, con
         // the tile's error bound from the validity edges (u = 0, u = fw-1,
         // v = 0, v = fh-1); only pixels within that band, a thin ring at the
         // footprint edge, evaluate the FP64 map.
-        const float den = (1.0f + tm.g0 * static_cast<float>(dx)) + tm.g1 * static_cast<float>(dy);
+        const float den = tile_den(tm, dx, dy);
         float lx, ly;
         if (den > 0.5f) {
-            const float r = __builtin_amdgcn_rcpf(den);  // 1 ulp; inside the tile's error margin
-            lx = tm.fx + (tm.a00 * static_cast<float>(dx) + tm.a01 * static_cast<float>(dy)) * r;
-            ly = tm.fy + (tm.a10 * static_cast<float>(dx) + tm.a11 * static_cast<float>(dy)) * r;
+            tile_pos(tm, dx, dy, den, lx, ly);
             const float ux = static_cast<float>(tm.ix) + lx, uy = static_cast<float>(tm.iy) + ly;
             const float ex = static_cast<float>(it.fw - 1), ey = static_cast<float>(it.fh - 1);
             const float m = tm.margin;
@@ -346,6 +360,39 @@ __device__ __forceinline__ void decode_pair_h(const u4a& q, uint32_t sh, const v
     b[0] = d1.y; b[1] = d2.x; b[2] = d2.y;
 }
 
+// Whether every pixel of an FP16 item's warp tile lies in the chunk and the
+// item's support box, has an FP32 denominator above 1/2, and has its exact
+// source position at least twice the tile's error margin inside
+// [0, fw-1) x [0, fh-1): then the FP32 position decides every pixel valid
+// and no clamp changes it.  The FP32 denominator is monotone in dx and dy
+// (so its minimum is at a corner), and with a positive denominator the
+// projective image of the tile is the convex hull of its corners' images,
+// so the four corners decide.
+__device__ __forceinline__ bool tile_interior(const RItem& it, const RChunk& ch, const RTile& tl,
+                                              const TileMap& tm) {
+    if (!tm.ok || it.fmt != I360HIP_PIX_F16) return false;
+    const int x0 = tl.tx * kWarpTW, y0 = tl.ty * kWarpTH;
+    const int x1 = x0 + kWarpTW - 1, y1 = y0 + kWarpTH - 1;
+    if (x1 >= ch.w || y1 >= ch.h) return false;
+    const int* b = it.lbox[0];
+    if (x0 < b[0] || y0 < b[1] || x1 > b[2] || y1 > b[3]) return false;
+    const double m2 = 2.0 * static_cast<double>(tm.margin);
+    const double ex = static_cast<double>(it.fw - 1) - m2, ey = static_cast<double>(it.fh - 1) - m2;
+    const double* m = it.hinv;
+    const double q0 = static_cast<double>(tm.q0);
+    for (int k = 0; k < 4; ++k) {
+        const int dx = (k & 1) ? kWarpTW - 1 : 0, dy = (k & 2) ? kWarpTH - 1 : 0;
+        if (!(tile_den(tm, dx, dy) > 0.6f)) return false;
+        const double X = static_cast<double>(ch.x + x0 + dx), Y = static_cast<double>(ch.y + y0 + dy);
+        const double q = (m[6] * X + m[7] * Y) + m[8];
+        if (!(q / q0 > 0.5)) return false;
+        const double u = ((m[0] * X + m[1] * Y) + m[2]) / q;
+        const double v = ((m[3] * X + m[4] * Y) + m[5]) / q;
+        if (!(u >= m2 && u <= ex && v >= m2 && v <= ey)) return false;
+    }
+    return true;
+}
+
 // Per-tile FP32 maps of the precision forms, one thread per warp tile
 // (the FP64 reduction runs fully parallel here instead of serially inside
 // the warp's tile loop).
@@ -358,7 +405,9 @@ __global__ void __launch_bounds__(256) rtilemap_kernel(const RItem* __restrict__
     const RTile tl = tiles[e];
     const RItem& it = items[tl.id];
     const RChunk& ch = chunks[it.chunk];
-    maps[e] = tile_map(it.hinv, static_cast<double>(ch.x + tl.tx * kWarpTW), static_cast<double>(ch.y + tl.ty * kWarpTH));
+    TileMap t = tile_map(it.hinv, static_cast<double>(ch.x + tl.tx * kWarpTW), static_cast<double>(ch.y + tl.ty * kWarpTH));
+    t.interior = tile_interior(it, ch, tl, t) ? 1 : 0;
+    maps[e] = t;
 }
 
 // Warp of one item's tile.  Each wavefront owns 64x4 pixels and works
@@ -385,6 +434,71 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
         const int tx0 = tl.tx * kWarpTW, ty0 = tl.ty * kWarpTH;
         TileMap tm{};
         if (MM == kModePrecision) tm = maps[e];
+        if constexpr (MM == kModePrecision) {
+            if (tm.interior) {
+                // every pixel valid, in the chunk and in the support box:
+                // the same FP32 positions, texel loads and blend as below,
+                // without the per-pixel tests, clamps and box reduction
+                const void* src = it.src;
+                const int fw = it.fw;
+                const size_t nbytes = static_cast<size_t>(fw) * it.fh * 6;
+                const float gain = it.gain;
+                const bool useGain = fabsf(gain - 1.0f) >= 1e-6f;
+                const int lx = threadIdx.x & 63;
+                const int x = tx0 + lx;
+                size_t i0[R];
+                float fx[R], fy[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int ly = wave * R + r;
+                    float px, py;
+                    tile_pos(tm, lx, ly, tile_den(tm, lx, ly), px, py);
+                    const float flx = floorf(px), fly = floorf(py);
+                    const int sx0 = tm.ix + static_cast<int>(flx), sy0 = tm.iy + static_cast<int>(fly);
+                    fx[r] = px - flx;
+                    fy[r] = py - fly;
+                    i0[r] = static_cast<size_t>(sy0) * fw + sx0;
+                }
+                u4a q0[R], q1[R];
+                uint32_t s0[R], s1[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    q0[r] = load_pair_f16(src, i0[r], nbytes, s0[r]);
+                    q1[r] = load_pair_f16(src, i0[r] + fw, nbytes, s1[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float omx = 1.0f - fx[r], omy = 1.0f - fy[r];
+                    const float wq[4] = {omx * omy, fx[r] * omy, omx * fy[r], fx[r] * fy[r]};
+                    _Float16 a[3], b[3], c[3], d[3];
+                    decode_pair_h(q0[r], s0[r], src, i0[r], a, b);
+                    decode_pair_h(q1[r], s1[r], src, i0[r] + fw, c, d);
+                    float v[4];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        float acc = __builtin_fmaf(wq[0], static_cast<float>(a[k]), 0.0f);
+                        acc = __builtin_fmaf(wq[1], static_cast<float>(b[k]), acc);
+                        acc = __builtin_fmaf(wq[2], static_cast<float>(c[k]), acc);
+                        float vv = __builtin_fmaf(wq[3], static_cast<float>(d[k]), acc);
+                        if (useGain) vv = vv * gain;
+                        v[k] = vv < 0.0f ? 0.0f : (1.0f < vv ? 1.0f : vv);
+                    }
+                    v[3] = 1.0f;
+                    const int y = ty0 + wave * R + r;
+                    const size_t p = static_cast<size_t>(y) * ch.w + x;
+                    st4(it.warped, p, v, static_cast<S*>(nullptr));
+                    gptr(it.cov)[p] = 1;
+                    gptr(it.scov)[ShearAt(x, y, shift, DistPitch(ch.w))] = 1;
+                }
+                if (lx == 0) {
+                    typedef int i4v __attribute__((ext_vector_type(4)));
+                    i4v bv;
+                    bv.x = tx0; bv.y = ty0 + wave * R; bv.z = tx0 + kWarpTW - 1; bv.w = ty0 + wave * R + R - 1;
+                    *gptr(reinterpret_cast<i4v*>(wbox) + static_cast<size_t>(e) * 4 + wave) = bv;
+                }
+                continue;
+            }
+        }
         const IBox fb = box_of(it.lbox[0]);
         const void* src = it.src;
         const int fw = it.fw, fmt = it.fmt;
