@@ -782,7 +782,11 @@ struct MapImg {
     double h[9];
     int cx, cy, w;
     __device__ __forceinline__ void rgb(size_t i, double (&o)[3]) const {
-        const int x = static_cast<int>(i % static_cast<size_t>(w)), y = static_cast<int>(i / static_cast<size_t>(w));
+        // chunk pixel indices are below 2^31 (the planner rejects larger
+        // chunks): a 32-bit division
+        const uint32_t ii = static_cast<uint32_t>(i), uw = static_cast<uint32_t>(w);
+        const uint32_t yq = ii / uw;
+        const int x = static_cast<int>(ii - yq * uw), y = static_cast<int>(yq);
         const double X = static_cast<double>(cx + x), Y = static_cast<double>(cy + y);
         double u = (h[0] * X + h[1] * Y) + h[2];
         double v = (h[3] * X + h[4] * Y) + h[5];
@@ -1455,29 +1459,54 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         const int X0 = cA.x0, Y0 = cA.y0, W1 = cB.x1 - cA.x0 + 1, H1 = cB.y1 - cA.y0 + 1;
         const bool staged = deep && count > 0 && W1 <= kFinW1 && H1 <= kFinH1;  // tile-uniform
         const int nslots = min(__popc(mask), kFinSlots);
-        // B
+        // B: the staged region in a fixed kFinH1 x kFinW1 layout per slot
+        // (constant divisors; elements beyond W1 x H1 are skipped).  Every
+        // load of a thread is issued before any is used, so they are all
+        // in flight together.
         if (staged) {
-            const int n1 = W1 * H1;
-            for (int idx = tid; idx < (nslots + 1) * n1; idx += 256) {
-                const int sl = idx / n1, q = idx - sl * n1;
-                const int ty1 = Y0 + q / W1, tx1 = X0 + q % W1;
-                const size_t t = static_cast<size_t>(ty1) * lw1 + tx1;
-                if (sl == nslots) {
-                    M r[4];
-                    ld4(ch.recon[1], t, r, static_cast<M*>(nullptr));
-                    sR[q][0] = r[0]; sR[q][1] = r[1]; sR[q][2] = r[2];
+            constexpr int kSlotN = kFinH1 * kFinW1;
+            constexpr int kIt = ((kFinSlots + 1) * kSlotN + 255) / 256;
+            M a[kIt][4];
+            int qq[kIt], sl[kIt], tx1[kIt], ty1[kIt];
+            bool use[kIt];
+#pragma unroll
+            for (int k = 0; k < kIt; ++k) {
+                const int idx = tid + 256 * k;
+                sl[k] = idx / kSlotN;
+                qq[k] = idx - sl[k] * kSlotN;
+                const int qy = qq[k] / kFinW1, qx = qq[k] - qy * kFinW1;
+                use[k] = sl[k] <= nslots && qy < H1 && qx < W1;
+                ty1[k] = Y0 + qy;
+                tx1[k] = X0 + qx;
+                const size_t t = use[k] ? static_cast<size_t>(ty1[k]) * lw1 + tx1[k] : 0;
+                a[k][0] = a[k][1] = a[k][2] = a[k][3] = M(0);
+                if (use[k]) {
+                    if (sl[k] == nslots) {
+                        ld4(ch.recon[1], t, a[k], static_cast<M*>(nullptr));
+                    } else {
+                        // contributor of slot sl: the sl-th set bit of the mask
+                        uint32_t m = mask;
+                        for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
+                        ld4(sit[__ffs(m) - 1].nd1, t, a[k], static_cast<S*>(nullptr));
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kIt; ++k) {
+                if (!use[k]) continue;
+                const int q = qq[k];
+                if (sl[k] == nslots) {
+                    sR[q][0] = a[k][0]; sR[q][1] = a[k][1]; sR[q][2] = a[k][2];
                 } else {
-                    // contributor of slot sl: the sl-th set bit of the mask
                     uint32_t m = mask;
-                    for (int j = 0; j < sl; ++j) m &= m - 1u;
+                    for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
                     const FinalItem& it = sit[__ffs(m) - 1];
-                    M a[4];
-                    ld4(it.nd1, t, a, static_cast<S*>(nullptr));
-                    const bool in = tx1 >= it.box[0] && tx1 <= it.box[2] && ty1 >= it.box[1] && ty1 <= it.box[3];
-                    const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
-                    sG[sl][q][0] = in ? a[0] * inv : M(0);
-                    sG[sl][q][1] = in ? a[1] * inv : M(0);
-                    sG[sl][q][2] = in ? a[2] * inv : M(0);
+                    const bool in = tx1[k] >= it.box[0] && tx1[k] <= it.box[2] && ty1[k] >= it.box[1] &&
+                                    ty1[k] <= it.box[3];
+                    const M inv = (a[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][3]) : M(0);
+                    sG[sl[k]][q][0] = in ? a[k][0] * inv : M(0);
+                    sG[sl[k]][q][1] = in ? a[k][1] * inv : M(0);
+                    sG[sl[k]][q][2] = in ? a[k][2] * inv : M(0);
                 }
             }
         }
@@ -1512,7 +1541,7 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                     if (staged) {
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int q = (ys[j] - Y0) * W1 + (xs[j] - X0);
+                            const int q = (ys[j] - Y0) * kFinW1 + (xs[j] - X0);
 #pragma unroll
                             for (int c3 = 0; c3 < 3; ++c3) r[j][c3] = sR[q][c3];
                         }
@@ -1528,7 +1557,7 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                     if (staged && slot < kFinSlots) {
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int q = (ys[j] - Y0) * W1 + (xs[j] - X0);
+                            const int q = (ys[j] - Y0) * kFinW1 + (xs[j] - X0);
 #pragma unroll
                             for (int c3 = 0; c3 < 3; ++c3) g[j][c3] = sG[slot][q][c3];
                         }
@@ -1711,6 +1740,10 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         }
         if (cc.width > 16384) {
             st.set(I360HIP_INVALID_PARAM, "chunk %d wider than 16384 px", ci);
+            return false;
+        }
+        if (static_cast<uint64_t>(cc.width) * static_cast<uint64_t>(cc.height) >= (1ull << 31)) {
+            st.set(I360HIP_INVALID_PARAM, "chunk %d has 2^31 pixels or more", ci);
             return false;
         }
         key = mix(key, static_cast<uint64_t>(cc.x) * 1000003ull + cc.y);

@@ -3,7 +3,6 @@ symbol include/*.h declares, and context creation without a device fails
 with NO_GPU rather than crashing."""
 import ctypes as C
 import os
-import subprocess
 
 import pytest
 
@@ -19,12 +18,40 @@ def test_library_exports_every_declared_symbol():
     assert set(_abi.SIGNATURES) == declared
 
 
+def _offload_bundles(blob):
+    """Entries (triple, code object bytes) of every clang offload bundle in
+    `blob`: magic, uint64 count, then per entry uint64 offset, size and
+    triple length and the triple (offsets from the bundle's start)."""
+    import struct
+
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    out, pos = [], blob.find(magic)
+    while pos >= 0:
+        (n,) = struct.unpack_from("<Q", blob, pos + len(magic))
+        q = pos + len(magic) + 8
+        for _ in range(n):
+            off, size, tlen = struct.unpack_from("<QQQ", blob, q)
+            q += 24
+            triple = blob[q:q + tlen].decode()
+            q += tlen
+            out.append((triple, blob[pos + off: pos + off + size]))
+        pos = blob.find(magic, pos + 1)
+    return out
+
+
 def test_library_is_gfx950_only():
-    out = subprocess.run(["/opt/rocm/llvm/bin/llvm-readelf", "-n", _abi.LIB_PATH], capture_output=True, text=True)
-    if out.returncode != 0:
-        pytest.skip("llvm-readelf unavailable")
-    # the offload bundle is embedded in .hip_fatbin; the host ELF must load
-    assert os.path.getsize(_abi.LIB_PATH) > 0
+    # every device code object embedded in the library (one bundle per
+    # translation unit) targets gfx950 and nothing else
+    with open(_abi.LIB_PATH, "rb") as f:
+        blob = f.read()
+    entries = _offload_bundles(blob)
+    device = [(t, co) for t, co in entries if not t.startswith("host-")]
+    assert len(device) >= 5, [t for t, _ in entries]
+    for triple, co in device:
+        assert triple == "hipv4-amdgcn-amd-amdhsa--gfx950", triple
+        # an AMDGPU ELF code object (e_machine 0xE0) whose note names gfx950
+        assert co[:4] == b"\x7fELF" and int.from_bytes(co[18:20], "little") == 0xE0, triple
+        assert b"amdgcn-amd-amdhsa--gfx950" in co
 
 
 def test_version_string():
