@@ -1412,7 +1412,7 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
     constexpr int K = 4;
     constexpr int kRowStep = kWarpTH / K;
     __shared__ FinalItem sit[I360HIP_MAX_CONTRIBUTORS];
-    __shared__ M sR[kFinH1 * kFinW1][3];
+    // per slot: the level-1 difference R1 - G1 of the slot's contributor
     __shared__ M sG[kFinSlots][kFinH1 * kFinW1][3];
     __shared__ uint32_t smask;
     const int tid = static_cast<int>(threadIdx.x);
@@ -1459,14 +1459,15 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         const int X0 = cA.x0, Y0 = cA.y0, W1 = cB.x1 - cA.x0 + 1, H1 = cB.y1 - cA.y0 + 1;
         const bool staged = deep && count > 0 && W1 <= kFinW1 && H1 <= kFinH1;  // tile-uniform
         const int nslots = min(__popc(mask), kFinSlots);
-        // B: the staged region in a fixed kFinH1 x kFinW1 layout per slot
-        // (constant divisors; elements beyond W1 x H1 are skipped).  Every
-        // load of a thread is issued before any is used, so they are all
-        // in flight together.
+        // B: per slot, the level-1 difference R1 - G1 of the slot's
+        // contributor over the tile's level-1 region, in a fixed
+        // kFinH1 x kFinW1 layout (constant divisors; elements beyond
+        // W1 x H1 are skipped).  Every load of a thread is issued before any
+        // is used, so they are all in flight together.
         if (staged) {
             constexpr int kSlotN = kFinH1 * kFinW1;
-            constexpr int kIt = ((kFinSlots + 1) * kSlotN + 255) / 256;
-            M a[kIt][4];
+            constexpr int kIt = (kFinSlots * kSlotN + 255) / 256;
+            M a[kIt][4], r[kIt][4];
             int qq[kIt], sl[kIt], tx1[kIt], ty1[kIt];
             bool use[kIt];
 #pragma unroll
@@ -1475,39 +1476,31 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                 sl[k] = idx / kSlotN;
                 qq[k] = idx - sl[k] * kSlotN;
                 const int qy = qq[k] / kFinW1, qx = qq[k] - qy * kFinW1;
-                use[k] = sl[k] <= nslots && qy < H1 && qx < W1;
+                use[k] = sl[k] < nslots && qy < H1 && qx < W1;
                 ty1[k] = Y0 + qy;
                 tx1[k] = X0 + qx;
                 const size_t t = use[k] ? static_cast<size_t>(ty1[k]) * lw1 + tx1[k] : 0;
                 a[k][0] = a[k][1] = a[k][2] = a[k][3] = M(0);
+                r[k][0] = r[k][1] = r[k][2] = r[k][3] = M(0);
                 if (use[k]) {
-                    if (sl[k] == nslots) {
-                        ld4(ch.recon[1], t, a[k], static_cast<M*>(nullptr));
-                    } else {
-                        // contributor of slot sl: the sl-th set bit of the mask
-                        uint32_t m = mask;
-                        for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
-                        ld4(sit[__ffs(m) - 1].nd1, t, a[k], static_cast<S*>(nullptr));
-                    }
+                    // contributor of slot sl: the sl-th set bit of the mask
+                    uint32_t m = mask;
+                    for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
+                    ld4(sit[__ffs(m) - 1].nd1, t, a[k], static_cast<S*>(nullptr));
+                    ld4(ch.recon[1], t, r[k], static_cast<M*>(nullptr));
                 }
             }
 #pragma unroll
             for (int k = 0; k < kIt; ++k) {
                 if (!use[k]) continue;
-                const int q = qq[k];
-                if (sl[k] == nslots) {
-                    sR[q][0] = a[k][0]; sR[q][1] = a[k][1]; sR[q][2] = a[k][2];
-                } else {
-                    uint32_t m = mask;
-                    for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
-                    const FinalItem& it = sit[__ffs(m) - 1];
-                    const bool in = tx1[k] >= it.box[0] && tx1[k] <= it.box[2] && ty1[k] >= it.box[1] &&
-                                    ty1[k] <= it.box[3];
-                    const M inv = (a[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][3]) : M(0);
-                    sG[sl[k]][q][0] = in ? a[k][0] * inv : M(0);
-                    sG[sl[k]][q][1] = in ? a[k][1] * inv : M(0);
-                    sG[sl[k]][q][2] = in ? a[k][2] * inv : M(0);
-                }
+                uint32_t m = mask;
+                for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
+                const FinalItem& it = sit[__ffs(m) - 1];
+                const bool in = tx1[k] >= it.box[0] && tx1[k] <= it.box[2] && ty1[k] >= it.box[1] &&
+                                ty1[k] <= it.box[3];
+                const M inv = (a[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][3]) : M(0);
+#pragma unroll
+                for (int c3 = 0; c3 < 3; ++c3) sG[sl[k]][qq[k]][c3] = r[k][c3] - (in ? a[k][c3] * inv : M(0));
             }
         }
         // the warped values (in flight across the barrier)
@@ -1535,50 +1528,35 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                     const int y = tl.ty * kWarpTH + ry + kRowStep * k;
                     const Coord<M> c = expand_at_s<M>(x, y, sx, sy, lw1, lh1);
                     const int slot = __popc(mask & ((1u << o[k]) - 1u));
-                    M g[4][3], r[4][3];
+                    // level 0 = G0 - Up(G1) + Up(R1) = G0 + Up(R1 - G1): the
+                    // bilinear upsample is linear, so one interpolation of
+                    // the level-1 difference (equal up to rounding)
+                    M dq[4][3];
                     const int xs[4] = {c.x0, c.x1, c.x0, c.x1};
                     const int ys[4] = {c.y0, c.y0, c.y1, c.y1};
-                    if (staged) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const int q = (ys[j] - Y0) * kFinW1 + (xs[j] - X0);
-#pragma unroll
-                            for (int c3 = 0; c3 < 3; ++c3) r[j][c3] = sR[q][c3];
-                        }
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            M rr[4];
-                            ld4(ch.recon[1], static_cast<size_t>(ys[j]) * lw1 + xs[j], rr, static_cast<M*>(nullptr));
-#pragma unroll
-                            for (int c3 = 0; c3 < 3; ++c3) r[j][c3] = rr[c3];
-                        }
-                    }
                     if (staged && slot < kFinSlots) {
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const int q = (ys[j] - Y0) * kFinW1 + (xs[j] - X0);
 #pragma unroll
-                            for (int c3 = 0; c3 < 3; ++c3) g[j][c3] = sG[slot][q][c3];
+                            for (int c3 = 0; c3 < 3; ++c3) dq[j][c3] = sG[slot][q][c3];
                         }
                     } else {
                         const FinalItem& it = sit[o[k]];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            M a[4];
-                            ld4(it.nd1, static_cast<size_t>(ys[j]) * lw1 + xs[j], a, static_cast<S*>(nullptr));
+                            M a[4], rr[4];
+                            const size_t t = static_cast<size_t>(ys[j]) * lw1 + xs[j];
+                            ld4(it.nd1, t, a, static_cast<S*>(nullptr));
+                            ld4(ch.recon[1], t, rr, static_cast<M*>(nullptr));
                             const bool in = xs[j] >= it.box[0] && xs[j] <= it.box[2] && ys[j] >= it.box[1] && ys[j] <= it.box[3];
                             const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
 #pragma unroll
-                            for (int c3 = 0; c3 < 3; ++c3) g[j][c3] = in ? a[c3] * inv : M(0);
+                            for (int c3 = 0; c3 < 3; ++c3) dq[j][c3] = rr[c3] - (in ? a[c3] * inv : M(0));
                         }
                     }
 #pragma unroll
-                    for (int c3 = 0; c3 < 3; ++c3) {
-                        const M up = lerp2m<M>(c, g[0][c3], g[1][c3], g[2][c3], g[3][c3]);
-                        const M ur = lerp2m<M>(c, r[0][c3], r[1][c3], r[2][c3], r[3][c3]);
-                        v[c3] = (v[c3] - up) + ur;
-                    }
+                    for (int c3 = 0; c3 < 3; ++c3) v[c3] = v[c3] + lerp2m<M>(c, dq[0][c3], dq[1][c3], dq[2][c3], dq[3][c3]);
                 }
             }
             if (ok[k]) {
