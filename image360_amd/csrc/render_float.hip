@@ -1462,45 +1462,41 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         // B: per slot, the level-1 difference R1 - G1 of the slot's
         // contributor over the tile's level-1 region, in a fixed
         // kFinH1 x kFinW1 layout (constant divisors; elements beyond
-        // W1 x H1 are skipped).  Every load of a thread is issued before any
-        // is used, so they are all in flight together.
+        // W1 x H1 are skipped).  A slot's loads of a thread are issued
+        // before any is used, so they are in flight together.
         if (staged) {
             constexpr int kSlotN = kFinH1 * kFinW1;
-            constexpr int kIt = (kFinSlots * kSlotN + 255) / 256;
-            M a[kIt][4], r[kIt][4];
-            int qq[kIt], sl[kIt], tx1[kIt], ty1[kIt];
-            bool use[kIt];
-#pragma unroll
-            for (int k = 0; k < kIt; ++k) {
-                const int idx = tid + 256 * k;
-                sl[k] = idx / kSlotN;
-                qq[k] = idx - sl[k] * kSlotN;
-                const int qy = qq[k] / kFinW1, qx = qq[k] - qy * kFinW1;
-                use[k] = sl[k] < nslots && qy < H1 && qx < W1;
-                ty1[k] = Y0 + qy;
-                tx1[k] = X0 + qx;
-                const size_t t = use[k] ? static_cast<size_t>(ty1[k]) * lw1 + tx1[k] : 0;
-                a[k][0] = a[k][1] = a[k][2] = a[k][3] = M(0);
-                r[k][0] = r[k][1] = r[k][2] = r[k][3] = M(0);
-                if (use[k]) {
-                    // contributor of slot sl: the sl-th set bit of the mask
-                    uint32_t m = mask;
-                    for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
-                    ld4(sit[__ffs(m) - 1].nd1, t, a[k], static_cast<S*>(nullptr));
-                    ld4(ch.recon[1], t, r[k], static_cast<M*>(nullptr));
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kIt; ++k) {
-                if (!use[k]) continue;
-                uint32_t m = mask;
-                for (int j = 0; j < sl[k]; ++j) m &= m - 1u;
+            constexpr int kIt = (kSlotN + 255) / 256;
+            uint32_t m = mask;
+            for (int sl = 0; sl < nslots; ++sl, m &= m - 1u) {
                 const FinalItem& it = sit[__ffs(m) - 1];
-                const bool in = tx1[k] >= it.box[0] && tx1[k] <= it.box[2] && ty1[k] >= it.box[1] &&
-                                ty1[k] <= it.box[3];
-                const M inv = (a[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][3]) : M(0);
+                M a[kIt][4], r[kIt][4];
+                int qq[kIt], tx1[kIt], ty1[kIt];
+                bool use[kIt];
 #pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) sG[sl[k]][qq[k]][c3] = r[k][c3] - (in ? a[k][c3] * inv : M(0));
+                for (int k = 0; k < kIt; ++k) {
+                    qq[k] = tid + 256 * k;
+                    const int qy = qq[k] / kFinW1, qx = qq[k] - qy * kFinW1;
+                    use[k] = qq[k] < kSlotN && qy < H1 && qx < W1;
+                    ty1[k] = Y0 + qy;
+                    tx1[k] = X0 + qx;
+                    const size_t t = use[k] ? static_cast<size_t>(ty1[k]) * lw1 + tx1[k] : 0;
+                    a[k][0] = a[k][1] = a[k][2] = a[k][3] = M(0);
+                    r[k][0] = r[k][1] = r[k][2] = r[k][3] = M(0);
+                    if (use[k]) {
+                        ld4(it.nd1, t, a[k], static_cast<S*>(nullptr));
+                        ld4(ch.recon[1], t, r[k], static_cast<M*>(nullptr));
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kIt; ++k) {
+                    if (!use[k]) continue;
+                    const bool in = tx1[k] >= it.box[0] && tx1[k] <= it.box[2] && ty1[k] >= it.box[1] &&
+                                    ty1[k] <= it.box[3];
+                    const M inv = (a[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][3]) : M(0);
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; ++c3) sG[sl][qq[k]][c3] = r[k][c3] - (in ? a[k][c3] * inv : M(0));
+                }
             }
         }
         // the warped values (in flight across the barrier)
