@@ -2606,6 +2606,14 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
         return;
     }
     (void)hipGraphDestroy(g);
+    // Upload the executable graph's device-side state now: replays of an
+    // uploaded graph reach the queues sooner (measured: four-group
+    // gigapixel step 36.8 -> 36.3 ms).
+    if (hipError_t e = hipGraphUpload(ge, s); e != hipSuccess) {
+        (void)hipGraphExecDestroy(ge);
+        st.set(CodeOf(e), "hipGraphUpload failed: %s", HipName(e));
+        return;
+    }
     rc.graph = ge;
     rc.call = ck;
     if (hipGraphLaunch(ge, s) != hipSuccess) st.set(I360HIP_KERNEL_LAUNCH_FAILED, "graph launch failed");
