@@ -1,8 +1,10 @@
 """The chamfer kernel's systolic schedule (image360_amd/csrc/chamfer.hpp),
 emulated step by step on the CPU, reproduces the serial two-pass chamfer
-of the oracle bit for bit (SeamBlend.cpp:206-242): every thread reads the
-neighbour strips' boundary values that were written at earlier steps,
-processes its rows at step 2r + lag, and prefetches its rows P ahead."""
+of the oracle bit for bit (SeamBlend.cpp:206-242): strip t handles its row
+r at step r + t + 1, reads the neighbour strips' values published at the
+previous step, recomputes the one neighbour that is finished only in the
+same step (the downstream strip's first column of the row above), and
+prefetches its rows ahead."""
 import numpy as np
 import pytest
 
@@ -21,11 +23,15 @@ def pitch(w):
 
 
 def shear_rows(w, h):
-    return h + (((w - 1) // min_strip(w)) >> 1) + 1
+    return h + (pitch(w) - 1) // min_strip(w) + 1
+
+
+def lg(S):
+    return {4: 2, 8: 3, 16: 4}[S]
 
 
 def shear_at(x, y, S, p):
-    return (y + ((x // S) >> 1)) * p + x
+    return (y + (x >> lg(S))) * p + x
 
 
 class Flat:
@@ -45,7 +51,7 @@ class Flat:
 
 def shear_cov(cov, scov, box, w, h, S, box_only=False):
     """chamfer_shear_kernel: the box's rows, columns widened to 16-column
-    groups, copied 8 bytes at a time to the sheared plane.  box_only: only
+    groups, copied min(S, 8) bytes at a time to the sheared plane.  box_only: only
     the box's own pixels are written (the render's warp writes its
     footprint, which contains the box), the rest holds garbage."""
     x0, y0, x1, y1 = box
@@ -56,23 +62,25 @@ def shear_cov(cov, scov, box, w, h, S, box_only=False):
             for x in range(x0, x1 + 1):
                 scov.store(shear_at(x, y, S, cp), np.array([cov[y, x]], np.uint8))
         return
+    G = min(S, 8)
     ga, gb = x0 & ~15, min((x1 + 16) & ~15, cp)
     for y in range(y0, y1 + 1):
-        for x in range(ga, gb, 8):
-            v = np.array([cov[y, x + k] if x + k < w else 0 for k in range(8)], np.uint8)
-            rows = {(xx // S) >> 1 for xx in range(x, x + 8)}
-            assert len(rows) == 1  # an 8-byte group never straddles two shear rows
+        for x in range(ga, gb, G):
+            v = np.array([cov[y, x + k] if x + k < w else 0 for k in range(G)], np.uint8)
+            rows = {xx >> lg(S) for xx in range(x, x + G)}
+            assert len(rows) == 1  # a G-byte group never straddles two shear rows
             scov.store(shear_at(x, y, S, cp), v)
 
 
 INF = np.float32(np.inf)
 
 
-def kernel(scov, dist, box, w, h, S, P, FWD, nt, access_rows):
+def kernel(scov, dist, box, w, h, S, P, FWD, nt, acc, urs, fast_supersteps):
     """chamfer_kernel: neighbours beyond the processed area are constants
-    (0 for an existing pixel outside the box, +inf for no pixel) taken from
-    sentinel LDS slots and per-thread constants; columns past the chunk
-    edge hold +inf."""
+    (0 for an existing pixel outside the box, +inf for no pixel) from the
+    upstream sentinel LDS slot and per-thread constants; columns past the
+    chunk edge hold +inf.  LDS is two buffers by step parity; -7 marks a
+    slot no strip has written.  urs collects every recomputed neighbour."""
     x0, y0, x1, y1 = box
     if x1 < 0: return
     dp = pitch(w)
@@ -80,107 +88,135 @@ def kernel(scov, dist, box, w, h, S, P, FWD, nt, access_rows):
     nact = (x1 + 1 - xa + S - 1) // S
     rows = y1 - y0 + 1
     kInf = f32(w + h)
-    # slot t + 1 holds strip t; slots 0 / nact + 1 the sentinels
-    firstv = np.full((2, nt + 2), -7, np.float32); lastv = np.full((2, nt + 2), -7, np.float32)
+    edge = np.full((2, nt + 2), -7, np.float32); part = np.full((2, nt + 2), -7, np.float32)
     sl = f32(0) if xa > 0 else INF
     sr = f32(0) if xa + nact * S < w else INF
-    lastv[:, 0] = sl; firstv[:, nact + 1] = sr
+    edge[:, 0 if FWD else nact + 1] = sl if FWD else sr
     edgePrev = (y0 > 0) if FWD else (y1 < h - 1)
+    idx = (lambda i: i) if FWD else (lambda i: S - 1 - i)
     th = []
     for t in range(nt):
-        lag = t if FWD else nact - 1 - t
         xs = xa + t * S
-        th.append(dict(t=t, active=t < nact, xs=xs, xl=xa + min(t, nact - 1) * S, lag=lag,
-                       base=lag >> 1, par=lag & 1,
-                       inbox=[x0 <= xs + s <= x1 for s in range(S)], exists=[xs + s < w for s in range(S)],
-                       pL0=f32(0) if (edgePrev and xs > 0) else INF,
-                       pR0=f32(0) if (edgePrev and xs + S < w) else INF,
-                       ring=None,
-                       prev=np.array([f32(0) if (edgePrev and xs + s < w) else INF for s in range(S)], np.float32)))
+        pL0 = f32(0) if (edgePrev and xs > 0) else INF
+        pR0 = f32(0) if (edgePrev and xs + S < w) else INF
+        T = dict(t=t, active=t < nact, xs=xs, xl=xa + min(t, nact - 1) * S,
+                 lag=t if FWD else nact - 1 - t,
+                 up=t if FWD else t + 2, dn=t + 2 if FWD else t,
+                 last=(t == nact - 1) if FWD else (t == 0),
+                 pUp0=pL0 if FWD else pR0, pDn0=pR0 if FWD else pL0, sDn=sr if FWD else sl,
+                 inbox=[x0 <= xs + s <= x1 for s in range(S)], exists=[xs + s < w for s in range(S)],
+                 prev=np.array([f32(0) if (edgePrev and xs + s < w) else INF for s in range(S)], np.float32))
+        T['prevEdge'] = T['pUp0']; T['pprevLast'] = None; T['initCur'] = None
+        th.append(T)
 
-    def load_row(T, r):
-        # unconditional, for active and inactive threads alike (strip clamped)
-        y = y0 + r if FWD else y1 - r
-        if FWD:
-            return scov.load(shear_at(T['xl'], y, S, dp), S)
+    def load_row(T, q):
+        y = y0 + q if FWD else y1 - q
+        if FWD: return scov.load(shear_at(T['xl'], y, S, dp), S)
         return dist.load(shear_at(T['xl'], y, S, dp), S)
 
-    def fetch(T, j):
-        return [load_row(T, min(max(P * j + k - T['base'], 0), rows - 1)) for k in range(P)]
+    def fetch(T, j):  # superstep j: steps P*j + k handle next-row q = P*j + k - lag
+        return [load_row(T, min(max(P * j + k - T['lag'], 0), rows - 1)) for k in range(P)]
 
     def init_of(T, raw, s):
         if not T['inbox'][s]: return f32(0)
         if not FWD: return raw[s]
-        return f32(f32(raw[s]) * kInf)  # coverage byte 0 or 1
+        return f32(f32(raw[s]) * kInf)
 
+    for T in th: T['ring'] = fetch(T, 0)
+    tmax = rows + nact - 1
+    # steady-state supersteps: a wavefront of interior strips whose rows
+    # stay in 1 .. rows - 2 (the kernel then drops every edge case)
     for T in th:
-        T['ring'] = fetch(T, 0)
-    tmax = 2 * (rows - 1) + nact - 1
+        T['interior'] = T['active'] and not T['last'] and all(T['inbox']) and all(T['exists'])
+    waves = [th[i:i + 64] for i in range(0, nt, 64)]
     T0 = 0
     while T0 <= tmax:
-        for T in th:
-            T['next'] = fetch(T, T0 // (2 * P) + 1)
-        for s2 in range(2 * P):
-            if T0 + s2 > tmax: break
-            # all threads "in parallel": reads of LDS happen before writes of this step
-            snap_first = firstv.copy(); snap_last = lastv.copy()
+        for wv, lanes in enumerate(waves):
+            lagMin = 64 * wv if FWD else nact - 64 - 64 * wv
+            lagMax = lagMin + 63
+            if all(T['interior'] for T in lanes) and T0 - lagMax - 1 >= 1 and T0 + P - 1 - lagMin - 1 <= rows - 2:
+                assert T0 + P - 1 <= tmax
+                for T in lanes:
+                    assert lagMin <= T['lag'] <= lagMax
+                    assert all(1 <= T0 + k - T['lag'] - 1 <= rows - 2 for k in range(P))
+                fast_supersteps.append((FWD, T0, wv))
+        for T in th: T['next'] = fetch(T, T0 // P + 1)
+        for k in range(P):
+            st = T0 + k
+            if st > tmax: break
+            se, sp = edge.copy(), part.copy()
+            rb, wb = (st - 1) & 1, st & 1
             for T in th:
-                t = T['t']
-                if not (T['active'] and (s2 & 1) == T['par']): continue
-                k = s2 >> 1
-                r = (T0 >> 1) + k - T['base']
-                if not (0 <= r < rows): continue
-                y = y0 + r if FWD else y1 - r
-                access_rows.setdefault((FWD, T0 + s2, t & 1), set()).add((y + (T['xs'] // S >> 1)))
-                pb, cb = (r - 1) & 1, r & 1
-                lCur, rCur = snap_last[cb][t], snap_first[cb][t + 2]
-                used = [lCur if FWD else rCur] + ([snap_last[pb][t], snap_first[pb][t + 2]] if r > 0 else [])
-                assert all(v != -7 for v in used)  # every neighbour used was written at an earlier step
-                prevL = T['pL0'] if r == 0 else snap_last[pb][t]
-                prevR = T['pR0'] if r == 0 else snap_first[pb][t + 2]
-                cur = np.zeros(S, np.float32)
-                xs = T['xs']; prev = T['prev']; raw = T['ring'][k]
-                order = range(S) if FWD else range(S - 1, -1, -1)
-                for s in order:
-                    b = init_of(T, raw, s)
-                    if FWD:
-                        b = min(b, f32((lCur if s == 0 else cur[s - 1]) + f32(1)))
+                if not T['active']: continue
+                q = st - T['lag']
+                r = q - 1
+                if r < -1 or r >= rows: continue
+                raw = T['ring'][k]
+                initNext = np.array([init_of(T, raw, s) for s in range(S)], np.float32)
+                prev = T['prev']
+                if r >= 0:
+                    y = y0 + r if FWD else y1 - r
+                    acc.setdefault((FWD, st, T['t'] // 64), set()).add(y + (T['xs'] >> lg(S)))
+                    eUp = se[rb][T['up']]
+                    assert eUp != -7
+                    diagUp = T['pUp0'] if r == 0 else T['prevEdge']
+                    if r == 0: ur = T['pDn0']
+                    elif T['last']: ur = T['sDn']
                     else:
-                        b = min(b, f32((rCur if s == S - 1 else cur[s + 1]) + f32(1)))
-                    b = min(b, f32(prev[s] + f32(1)))
-                    b = min(b, f32((prevL if s == 0 else prev[s - 1]) + D))
-                    b = min(b, f32((prevR if s == S - 1 else prev[s + 1]) + D))
-                    cur[s] = b if T['exists'][s] else INF
-                assert xs + S <= dp  # the strip stays inside its row
-                dist.store(shear_at(xs, y, S, dp), cur)  # the whole strip, as the kernel does
-                T['prev'] = cur.copy()
-                firstv[cb][t + 1] = cur[0]; lastv[cb][t + 1] = cur[S - 1]
-        for T in th:
-            T['ring'] = T['next']
-        T0 += 2 * P
-
+                        pDn = sp[rb][T['dn']]
+                        assert pDn != -7
+                        ur = min(pDn, f32(prev[idx(S - 1)] + f32(1)), f32(T['pprevLast'] + D))
+                        xr = T['xs'] + S if FWD else T['xs'] - 1
+                        urs.append((FWD, xr, (y - 1) if FWD else (y + 1), ur))
+                    cur = np.zeros(S, np.float32)
+                    ic = T['initCur']
+                    for i in range(S):
+                        s = idx(i)
+                        b = ic[s]
+                        b = min(b, f32((eUp if i == 0 else cur[idx(i - 1)]) + f32(1)))
+                        b = min(b, f32(prev[s] + f32(1)))
+                        b = min(b, f32((diagUp if i == 0 else prev[idx(i - 1)]) + D))
+                        b = min(b, f32((ur if i == S - 1 else prev[idx(i + 1)]) + D))
+                        cur[s] = b if T['exists'][s] else INF
+                    assert T['xs'] + S <= dp
+                    dist.store(shear_at(T['xs'], y, S, dp), cur)
+                    T['pprevLast'] = prev[idx(S - 1)]
+                    T['prev'] = prev = cur
+                    T['prevEdge'] = eUp
+                edge[wb][T['t'] + 1] = prev[idx(S - 1)]
+                part[wb][T['t'] + 1] = min(initNext[idx(0)], f32(prev[idx(0)] + f32(1)), f32(prev[idx(1)] + D))
+                T['initCur'] = initNext
+        for T in th: T['ring'] = T['next']
+        T0 += P
 
 def run_chamfer(cov, box, S, P, box_only=False):
     """Shear pre-pass, forward, backward; returns the distances read back
-    at every (x, y) in the box as the ownership kernels read them."""
+    at every (x, y) in the box as the ownership kernels read them, the
+    sheared rows each wavefront touched per step, and the number of
+    recomputed neighbours (each checked against the value its own strip
+    wrote) and the steady-state supersteps taken."""
     h, w = cov.shape
-    nt = ((w + S - 1) // S + 127) // 128 * 128
+    nt = ((w + S - 1) // S + 63) // 64 * 64
     rows = shear_rows(w, h)
     scov = Flat(rows * pitch(w), np.uint8, 0x5A)
     dist = Flat(rows * pitch(w), np.float32, -5)
     shear_cov(cov, scov, box, w, h, S, box_only)
-    acc = {}
-    kernel(scov, dist, box, w, h, S, P, True, nt, acc)
-    kernel(scov, dist, box, w, h, S, P, False, nt, acc)
+    acc, urs, fast = {}, [], []
+    kernel(scov, dist, box, w, h, S, P, True, nt, acc, urs, fast)
+    fwd = dist.a.copy()
+    kernel(scov, dist, box, w, h, S, P, False, nt, acc, urs, fast)
     out = np.zeros((h, w), np.float32)
     for y in range(box[1], box[3] + 1):
         for x in range(box[0], box[2] + 1):
             out[y, x] = dist.a[shear_at(x, y, S, pitch(w))]
-    return out, acc
+    for FWD, x, y, v in urs:
+        plane = fwd if FWD else dist.a
+        assert plane[shear_at(x, y, S, pitch(w))] == v, (FWD, x, y, v)
+    return out, acc, len(urs), fast
 
 
 @pytest.mark.parametrize("seed", range(10))
-@pytest.mark.parametrize("S,P", [(4, 8), (8, 4), (16, 2)])
+@pytest.mark.parametrize("S,P", [(4, 8), (4, 4), (8, 4), (8, 2), (16, 2), (16, 1)])
 def test_schedule_matches_serial_chamfer(seed, S, P):
     rng = np.random.default_rng(seed)
     h = int(rng.integers(3, 40))
@@ -198,12 +234,14 @@ def test_schedule_matches_serial_chamfer(seed, S, P):
         return
     box = (xs.min(), ys.min(), xs.max(), ys.max())
     want = po.coverage_distance(cov)
-    got, acc = run_chamfer(cov, box, S, P)
+    got, acc, nrec, _ = run_chamfer(cov, box, S, P)
     assert np.array_equal(got, want)
-    got, _ = run_chamfer(cov, box, S, P, box_only=True)
+    got, _, _, _ = run_chamfer(cov, box, S, P, box_only=True)
     assert np.array_equal(got, want)
-    # at every step the strips of one parity (one wavefront) touch one sheared row
+    # at every step the strips of one wavefront touch one sheared row
     assert all(len(rows) == 1 for rows in acc.values())
+    if box[2] - box[0] >= S and box[3] > box[1]:
+        assert nrec > 0  # more than one strip and row: neighbours were recomputed
 
 
 def test_sheared_plane_bounds_at_maximum_shear():
@@ -217,3 +255,20 @@ def test_sheared_plane_bounds_at_maximum_shear():
             xs = ((w - 1) // S) * S
             assert xs + S <= pitch(w)
             assert shear_at(xs + S - 1, h - 1, S, pitch(w)) < shear_rows(w, h) * pitch(w)
+
+
+@pytest.mark.parametrize("S,P", [(4, 8), (4, 4), (8, 2)])
+def test_steady_state_supersteps(S, P):
+    """A box wide enough for several wavefronts of interior strips takes the
+    steady-state supersteps, in both passes, and still equals the serial
+    chamfer."""
+    rng = np.random.default_rng(S * 10 + P)
+    w, h = 64 * S * 3 + 5, 150
+    cov = np.ones((h, w), np.uint8)
+    cov[rng.random((h, w)) < 0.02] = 0
+    cov[:, :3] = 0; cov[:4] = 0
+    ys, xs = np.nonzero(cov)
+    box = (xs.min(), ys.min(), xs.max(), ys.max())
+    got, _, nrec, fast = run_chamfer(cov, box, S, P)
+    assert np.array_equal(got, po.coverage_distance(cov))
+    assert {f for f, _, _ in fast} == {True, False} and nrec > 0
