@@ -2069,13 +2069,16 @@ static unsigned persistent_grid(i360hip_ctx* ctx, int n, int blocks_per_cu = 8) 
     return static_cast<unsigned>(std::max(1, std::min(n, cap)));
 }
 
-// Enqueues the stages of chunk `c` (or of every chunk when c < 0).
+// Which stages of a chunk range enqueue() launches: all of them, the warp
+// phase (box init, tile map, warp, box reduce) or the stages after it.
+enum EnqueuePart { kPartAll, kPartWarp, kPartRest };
+
 // Launches the render of chunks [c0, c1) on stream s.  Every stage's tiles
 // of a chunk range are contiguous, so each stage is one launch over the
 // range.
 template <int MODE>
 static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0, int c1, hipStream_t s, Status& st,
-                    const SurrRun* sr = nullptr) {
+                    const SurrRun* sr = nullptr, EnqueuePart part = kPartAll) {
     constexpr int MM = MODE == kModeBatched ? kModePrecision : MODE;
     using M = typename Form<MM>::M;
     using S = typename Form<MM>::S;
@@ -2086,28 +2089,31 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
         for (int k = c0; k < c1; ++k) n += v[k].n;
     };
     int off, n;
-    {
-        const int i0 = P.chunks[c0].first;
-        const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
-        if (i1 > i0) RL(st, "box_init", 0.0, rbox_init_kernel<<<(i1 - i0 + 63) / 64, 64, 0, s>>>(T.items + i0, T.chunks, i1 - i0));
-    }
-    // warp
-    span(P.warp, off, n);
     double area0 = 0;
     for (int k = c0; k < c1; ++k) area0 += static_cast<double>(P.chunks[k].w) * P.chunks[k].h;
-    if (MM == kModePrecision && n > 0)
-        RL(st, "tile_map", 0.0, rtilemap_kernel<<<(n + 255) / 256, 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, T.tmaps + off));
-    RL(st, MODE == kModeFP64 ? "warp_fp64" : (MODE == kModePlain ? "warp_f32" : "warp_f16"),
-       static_cast<double>(n) * kWarpTW * kWarpTH * (6.0 + 4 * sS + 1),
-       rwarp_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
-                                                                    T.tmaps + off, T.wbox + 4 * static_cast<size_t>(off),
-                                                                    ChamferShear(P.max_w, ctx->tune_strip)));
-    {
-        const int i0 = P.chunks[c0].first;
-        const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
-        if (i1 > i0)
-            RL(st, "box_reduce", 0.0, rbox_reduce_kernel<<<std::min(i1 - i0, 1024), 256, 0, s>>>(T.items + i0, i1 - i0, T.wbox));
+    if (part != kPartRest) {
+        {
+            const int i0 = P.chunks[c0].first;
+            const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
+            if (i1 > i0) RL(st, "box_init", 0.0, rbox_init_kernel<<<(i1 - i0 + 63) / 64, 64, 0, s>>>(T.items + i0, T.chunks, i1 - i0));
+        }
+        // warp
+        span(P.warp, off, n);
+        if (MM == kModePrecision && n > 0)
+            RL(st, "tile_map", 0.0, rtilemap_kernel<<<(n + 255) / 256, 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, T.tmaps + off));
+        RL(st, MODE == kModeFP64 ? "warp_fp64" : (MODE == kModePlain ? "warp_f32" : "warp_f16"),
+           static_cast<double>(n) * kWarpTW * kWarpTH * (6.0 + 4 * sS + 1),
+           rwarp_kernel<MODE><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
+                                                                        T.tmaps + off, T.wbox + 4 * static_cast<size_t>(off),
+                                                                        ChamferShear(P.max_w, ctx->tune_strip)));
+        {
+            const int i0 = P.chunks[c0].first;
+            const int i1 = P.chunks[c1 - 1].first + P.chunks[c1 - 1].count;
+            if (i1 > i0)
+                RL(st, "box_reduce", 0.0, rbox_reduce_kernel<<<std::min(i1 - i0, 1024), 256, 0, s>>>(T.items + i0, i1 - i0, T.wbox));
+        }
     }
+    if (part == kPartWarp) return;
     // chamfer over the items of the chunk range
     {
         const int i0 = P.chunks[c0].first;
@@ -2279,6 +2285,7 @@ struct RenderCache {
     // chunk groups of the batched form: auxiliary streams and their events
     hipEvent_t fork = nullptr;
     std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> forks;  // per auxiliary stream (staggered groups)
     std::vector<hipEvent_t> join;
     // recorded after the last enqueued render (any stream): the workspace
     // and tables are not rewritten before it has completed
@@ -2299,6 +2306,7 @@ void render_release(i360hip_ctx* ctx) {
         (void)hipStreamDestroy(a);
     }
     for (auto e : rc->join) (void)hipEventDestroy(e);
+    for (auto e : rc->forks) (void)hipEventDestroy(e);
     if (rc->fork) (void)hipEventDestroy(rc->fork);
     if (rc->done) {
         (void)hipEventSynchronize(rc->done);
@@ -2399,15 +2407,33 @@ static bool aux_streams(i360hip_ctx* ctx, int n) {
     }
     while (static_cast<int>(rc.aux.size()) < n) {
         hipStream_t st = nullptr;
-        hipEvent_t ev = nullptr;
+        hipEvent_t ev = nullptr, fk = nullptr;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return false;
         if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
             (void)hipStreamDestroy(st);
             return false;
         }
+        if (hipEventCreateWithFlags(&fk, hipEventDisableTiming) != hipSuccess) {
+            (void)hipEventDestroy(ev);
+            (void)hipStreamDestroy(st);
+            return false;
+        }
         rc.aux.push_back(st);
         rc.join.push_back(ev);
+        rc.forks.push_back(fk);
     }
+    return true;
+}
+
+// Staggered chunk groups: the groups' warp phases run one after another on
+// the origin stream, and each group's remaining stages fork off onto its
+// own stream as soon as its warp phase is done, so the groups reach their
+// latency-bound stages (chamfer, seams) at different times and each one's
+// serial chain overlaps the others' bandwidth-bound stages.  Off: every
+// group forks at the start and all groups run their stages side by side.
+// I360HIP_RENDER_STAGGER=0/1 overrides the default (on).
+static bool render_stagger() {
+    if (const char* e = std::getenv("I360HIP_RENDER_STAGGER")) return std::atoi(e) != 0;
     return true;
 }
 
@@ -2431,6 +2457,34 @@ static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, 
     }
     cut.push_back(nch);
     const int ng = static_cast<int>(cut.size()) - 1;
+    int share = ctx->tune_share > 0 ? ctx->tune_share : 1;
+    if (const char* e = std::getenv("I360HIP_GRID_SHARE")) share = std::max(1, std::atoi(e));
+    if (render_stagger()) {
+        // Each fork event is waited on right after it is recorded, before
+        // anything more is enqueued on the origin stream.
+        g_grid_share = share;
+        int forked = 0;
+        for (int g = 0; g < ng && st.ok(); ++g) {
+            enqueue<kModeBatched>(ctx, P, T, cut[g], cut[g + 1], s, st, sr, kPartWarp);
+            if (!st.ok()) break;
+            if (g + 1 == ng) {
+                enqueue<kModeBatched>(ctx, P, T, cut[g], cut[g + 1], s, st, sr, kPartRest);
+                break;
+            }
+            if (hipEventRecord(rc.forks[g], s) != hipSuccess || hipStreamWaitEvent(rc.aux[g], rc.forks[g], 0) != hipSuccess) {
+                st.set(I360HIP_DEVICE_ERROR, "render fork failed");
+                break;
+            }
+            ++forked;
+            enqueue<kModeBatched>(ctx, P, T, cut[g], cut[g + 1], rc.aux[g], st, sr, kPartRest);
+        }
+        g_grid_share = 1;
+        for (int g = 0; g < forked; ++g)
+            if (hipEventRecord(rc.join[g], rc.aux[g]) != hipSuccess || hipStreamWaitEvent(s, rc.join[g], 0) != hipSuccess) {
+                if (st.ok()) st.set(I360HIP_DEVICE_ERROR, "render join failed");
+            }
+        return;
+    }
     if (hipEventRecord(rc.fork, s) != hipSuccess) {
         st.set(I360HIP_DEVICE_ERROR, "render fork failed");
         return;
@@ -2440,8 +2494,6 @@ static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, 
             st.set(I360HIP_DEVICE_ERROR, "render fork failed");
             return;
         }
-    int share = ctx->tune_share > 0 ? ctx->tune_share : 1;
-    if (const char* e = std::getenv("I360HIP_GRID_SHARE")) share = std::max(1, std::atoi(e));
     g_grid_share = share;
     for (int g = 0; g < ng && st.ok(); ++g)
         enqueue<kModeBatched>(ctx, P, T, cut[g], cut[g + 1], g == 0 ? s : rc.aux[g - 1], st, sr);
