@@ -385,12 +385,13 @@ inline int ChamferThreads(int w, int S) {
 }
 
 // Strip width: `want` (4, 8 or 16; an autotuned choice) when a row of w
-// columns fits one 1024-thread block at that width, else 8 columns (the
-// measured best on 4096-wide chunks: fewer steps than 4 columns, shorter
-// per-step chains than 16), 16 where 8 does not fit.
+// columns fits one 1024-thread block at that width, else the narrowest
+// strip that fits (4 columns measured best for the four-group gigapixel
+// step on 4096-wide chunks: 8 columns take about as long on their own, but
+// their longer per-step chains overlap the other groups' stages worse).
 // I360HIP_CHAMFER_STRIP overrides both.
 inline int ChamferStrip(int w, int want = 0) {
-    int s = w <= 8192 ? 8 : 16;
+    int s = ChamferMinStrip(w);
     if ((want == 4 || want == 8 || want == 16) && w <= 1024 * want) s = want;
     if (const char* e = std::getenv("I360HIP_CHAMFER_STRIP")) {
         const int v = std::atoi(e);
