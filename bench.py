@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "image360_amd", "surrogate_weights.npz"))
     ap.add_argument("--only", choices=["burst", "uneven", "small"], default=None,
                     help="run only this workload's GPU measurement and print its figures (for profiling runs)")
+    ap.add_argument("--surrogate-max-error", type=float, default=0.0,
+                    help="per-pixel predicted squared error above which a surrogate tile also falls back (0 = off)")
     ap.add_argument("--surrogate-threshold", type=float, default=1e-5,
                     help="mean predicted squared error per tile above which the surrogate falls back (1e-5 = 50 dB)")
     return ap.parse_args()
@@ -460,6 +462,7 @@ def load_surrogate(hip, args, log):
     bt = torch.from_numpy(biases).cuda()
     torch.cuda.synchronize()
     hip.check(hip.surrogate_set(wt.data_ptr(), bt.data_ptr(), args.surrogate_threshold), "surrogate_set")
+    hip.check(hip.surrogate_set_max_error(getattr(args, "surrogate_max_error", 0.0)), "surrogate_set_max_error")
     return src
 
 
@@ -688,7 +691,9 @@ def main():
                        "steps": steps}
         if name == "surrogate":
             tiles, fb = hip.surrogate_stats()
-            forms[name].update({"tiles": tiles, "fallback_tiles": fb, "weights": surr_src})
+            forms[name].update({"tiles": tiles, "fallback_tiles": fb, "weights": surr_src,
+                                "threshold_mse": args.surrogate_threshold,
+                                "max_error_bound": args.surrogate_max_error})
             log(f"surrogate : {fb}/{tiles} level-K tiles fell back to the plain blend")
         log(f"{name:10s}: {ms / steps:9.2f} ms/step  {forms[name]['mp_s']:9.1f} MP/s"
             f"  (host {host_ms.get(mode, 0.0):.2f} ms per call)")

@@ -97,6 +97,7 @@ SIGNATURES = {
     "i360hip_surrogate_train_step": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, C.c_float, i32, vp, vp]),
     "i360hip_surrogate_round_bf16": (i32, [vp, vp, vp, vp]),
     "i360hip_surrogate_set": (i32, [vp, vp, vp, C.c_float]),
+    "i360hip_surrogate_set_max_error": (i32, [vp, C.c_float]),
     "i360hip_surrogate_chunk_level": (i32, [i32, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     "i360hip_surrogate_features": (i32, [vp, C.POINTER(Frame), i32, C.POINTER(ChunkDesc), i32, C.POINTER(i32), i32,
                                          vp, vp, vp]),

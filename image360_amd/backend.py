@@ -295,6 +295,9 @@ class HipComputeBackend:
     def surrogate_set(self, weights, biases, threshold_mse=1e-5) -> int:
         return lib().i360hip_surrogate_set(self._ctx, weights, biases, threshold_mse)
 
+    def surrogate_set_max_error(self, max_sq_err) -> int:
+        return lib().i360hip_surrogate_set_max_error(self._ctx, max_sq_err)
+
     @staticmethod
     def surrogate_chunk_level(w, h, num_bands):
         """(K, level width, level height); K = 0 when the chunk has < 4 levels."""

@@ -90,6 +90,7 @@ struct i360hip_ctx {
     const uint16_t* surr_weights = nullptr;  // device copy (a scratch slot)
     const float* surr_biases = nullptr;
     float surr_threshold = 1e-5f;
+    float surr_max_threshold = 0.0f;  // per-pixel bound of the fallback test (0: off)
     int surr_tiles = 0, surr_fallback = -1;  // last surrogate render
     // launch configuration of the batched render (i360hip_render_autotune;
     // 0 = built-in default)

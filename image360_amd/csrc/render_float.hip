@@ -1303,11 +1303,13 @@ __global__ void rscflag_init_kernel(const RChunk* __restrict__ chunks, int nch, 
 }
 
 // One block per 16x16 surrogate tile: fall back when the mean predicted
-// squared error of the low band (10^out[3]) exceeds `thr`.
+// squared error of the low band (10^out[3]) exceeds `thr`, or (thrMax > 0)
+// when any pixel's exceeds `thrMax`.
 __global__ void __launch_bounds__(256) rsflag_kernel(const SurrImg* __restrict__ imgs,
                                                      const SurrTile* __restrict__ tiles, int t0, SurrBufs sb,
-                                                     float thr) {
+                                                     float thr, float thrMax) {
     __shared__ float ssum[256];
+    __shared__ float smax[256];
     __shared__ int scnt[256];
     const SurrTile tl = tiles[t0 + blockIdx.x];
     const SurrImg im = imgs[tl.img];
@@ -1320,17 +1322,21 @@ __global__ void __launch_bounds__(256) rsflag_kernel(const SurrImg* __restrict__
         n = 1;
     }
     ssum[threadIdx.x] = e;
+    smax[threadIdx.x] = e;
     scnt[threadIdx.x] = n;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if (threadIdx.x < k) {
             ssum[threadIdx.x] += ssum[threadIdx.x + k];
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + k]);
             scnt[threadIdx.x] += scnt[threadIdx.x + k];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const int f = ssum[0] > thr * static_cast<float>(max(scnt[0], 1)) ? 1 : 0;
+        const bool overMean = ssum[0] > thr * static_cast<float>(max(scnt[0], 1));
+        const bool overMax = thrMax > 0.f && smax[0] > thrMax;
+        const int f = (overMean || overMax) ? 1 : 0;
         sb.tflag[t0 + blockIdx.x] = f;
         if (f) atomicOr(&sb.cflag[im.tag], 1);
     }
@@ -1600,6 +1606,7 @@ struct SurrRun {
     const uint16_t* weights = nullptr;
     const float* biases = nullptr;
     float threshold = 0.0f;  // mean predicted squared error above which a tile falls back
+    float max_threshold = 0.0f;  // per-pixel predicted squared error above which it falls back (0: off)
 };
 
 struct Plan {
@@ -2251,7 +2258,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
                 }
             }
             RL(st, "surrogate_flags", px * 16.0,
-               rsflag_kernel<<<sn, 256, 0, s>>>(T.simgs, T.stiles, so, sr->bufs, sr->threshold));
+               rsflag_kernel<<<sn, 256, 0, s>>>(T.simgs, T.stiles, so, sr->bufs, sr->threshold, sr->max_threshold));
         }
     }
     if (surr) {
@@ -2569,6 +2576,7 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
     sr.weights = ctx->surr_weights;
     sr.biases = ctx->surr_biases;
     sr.threshold = ctx->surr_threshold;
+    sr.max_threshold = ctx->surr_max_threshold;
     P.key = mix(P.key, static_cast<uint64_t>(ctx->tune_groups * 1000003 + ctx->tune_strip * 101 + ctx->tune_share));
     const bool reuse = !ctx->profiling && rc.graph && rc.key == P.key && rc.mode == api_mode && graphable;
     if (reuse) {  // other arguments, the same plan
@@ -2784,6 +2792,28 @@ int i360hip_surrogate_set(i360hip_ctx* ctx, const uint16_t* weights, const float
         if (rc.graph) (void)hipGraphExecDestroy(rc.graph);
         rc.graph = nullptr;
         ctx->surr_threshold = threshold_mse;
+    }
+    return I360HIP_SUCCESS;
+}
+
+int i360hip_surrogate_set_max_error(i360hip_ctx* ctx, float max_sq_err) {
+    if (!ctx) return I360HIP_DEVICE_ERROR;
+    if (!(max_sq_err >= 0.0f) || !std::isfinite(max_sq_err)) {
+        set_error(ctx, "Surrogate max error bound must be finite and non-negative.");
+        return I360HIP_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->surr_max_threshold != max_sq_err) {
+        // the bound is a kernel argument of a captured graph
+        if (hipSetDevice(ctx->device) != hipSuccess) return I360HIP_DEVICE_ERROR;
+        RenderCache& rc = cache_of(ctx);
+        if (!wait_renders(rc)) {
+            set_error(ctx, "waiting for the previous render failed");
+            return I360HIP_DEVICE_ERROR;
+        }
+        if (rc.graph) (void)hipGraphExecDestroy(rc.graph);
+        rc.graph = nullptr;
+        ctx->surr_max_threshold = max_sq_err;
     }
     return I360HIP_SUCCESS;
 }

@@ -300,6 +300,13 @@ int i360hip_surrogate_round_bf16(i360hip_ctx* ctx, const float* master_weights, 
  * threshold is a mean squared error per 16x16 level-K tile: 1e-5 is the
  * 50 dB PSNR equivalent on [0,1] RGB). */
 int i360hip_surrogate_set(i360hip_ctx* ctx, const uint16_t* weights, const float* biases, float threshold_mse);
+/* Per-pixel bound of the SURROGATE form's fallback test: a 16x16 level-K
+ * tile also keeps the plain blend when any of its pixels' predicted squared
+ * error exceeds max_sq_err (1e-6 aims at 1e-3 max abs on [0,1] RGB; the
+ * prediction is the network's, not a guarantee).  0 (the default) turns the
+ * test off.  No counterpart in the reference (the surrogate is this
+ * backend's). */
+int i360hip_surrogate_set_max_error(i360hip_ctx* ctx, float max_sq_err);
 /* Level K the surrogate works at for a chunk of width x height at
  * num_bands (0 when the chunk has fewer than 4 levels) and its size. */
 int i360hip_surrogate_chunk_level(int width, int height, int num_bands, int* level, int* level_width,

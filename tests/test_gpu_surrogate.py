@@ -147,6 +147,36 @@ def test_surrogate_form_all_fallback_equals_batched(hip):
     assert tiles > 0 and fb == tiles
 
 
+def test_surrogate_max_error_bound(hip):
+    # the per-pixel bound on the predicted squared error: more tiles fall
+    # back as it tightens (the graph is re-captured on every change), and a
+    # bound below every prediction renders the batched form's bytes even
+    # though the mean test alone keeps tiles on the network
+    import os
+
+    from image360_amd import _abi
+
+    g, frames = _small_pano(13)
+    bits, b = sg.load_params(os.path.join(os.path.dirname(sg.__file__), "surrogate_weights.npz"))
+    _set_params(hip, bits, b, 1e-5)
+    want = _render(hip, g, frames, _abi.RENDER_BATCHED)
+    fbs, tiles = [], 0
+    try:
+        for bound in (0.0, 1e-5, 1e-6, 1e-8, 1e-30):
+            hip.check(hip.surrogate_set_max_error(bound), "surrogate_set_max_error")
+            got = _render(hip, g, frames, _abi.RENDER_SURROGATE)
+            tiles, fb = hip.surrogate_stats()
+            fbs.append(fb)
+        note(f"surrogate max-error bound: fallback tiles {fbs} of {tiles}")
+        assert tiles > 0 and fbs[0] < tiles  # the mean test alone keeps tiles on the network
+        assert fbs[0] <= fbs[1] <= fbs[2] <= fbs[3] <= fbs[4] == tiles
+        assert bool((got == want).all())
+        for bad in (-1.0, float("nan"), float("inf")):
+            assert hip.surrogate_set_max_error(bad) == _abi.INVALID_PARAM
+    finally:
+        hip.check(hip.surrogate_set_max_error(0.0), "surrogate_set_max_error")
+
+
 def test_surrogate_form_composite_only(hip):
     # a network that predicts no residual and no error never falls back: the
     # low band is the single-level composite, which stays close to the blend
