@@ -433,6 +433,57 @@ def bench_uneven(hip, torch, geom, fdesc, args, log, count=512):
     return res, g
 
 
+def bench_surrogate_tiles(hip, torch, geom, fdesc, args, log):
+    """The learned surrogate at its stated inference shape: the gigapixel
+    canvas in 8192x8192 tiles (15 chunks), batched form against the
+    surrogate form, and both against the FP64 form on the tile with the
+    most contributors."""
+    from image360_amd import _abi, synth
+
+    g = synth.gigapixel(chunk=8192)
+    chunks = g.chunks
+    area = sum(c.area for c in chunks)
+    offs = np.cumsum([0] + [c.area * 3 for c in chunks])
+    out = torch.empty(int(offs[-1]), dtype=torch.float32, device="cuda")
+    contrib, descs = [], []
+    for k, c in enumerate(chunks):
+        descs.append({"x": c.x, "y": c.y, "w": c.w, "h": c.h, "first": len(contrib), "count": len(c.contributors),
+                      "out": out.data_ptr() + 4 * int(offs[k]), "owner": None})
+        contrib += c.contributors
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    kb = max(range(len(chunks)), key=lambda k: (len(chunks[k].contributors), chunks[k].area))
+    cb = chunks[kb]
+    ref = torch.empty(cb.area * 3, dtype=torch.float32, device="cuda")
+    d = dict(descs[kb], first=0, out=ref.data_ptr())
+    with torch.cuda.stream(stream):
+        hip.check(hip.render_chunks(fdesc, [d], cb.contributors, args.bands, _abi.RENDER_FP64, sh), "tiles fp64")
+    torch.cuda.synchronize()
+    res = {"chunks": len(chunks), "tile": 8192, "output_mp": round(area / 1e6, 1), "bands": args.bands,
+           "accuracy_tile": {"x": cb.x, "y": cb.y, "w": cb.w, "h": cb.h, "contributors": len(cb.contributors)}}
+    for name, mode in (("batched", _abi.RENDER_BATCHED), ("surrogate", _abi.RENDER_SURROGATE)):
+        with torch.cuda.stream(stream):
+            hip.check(hip.render_chunks(fdesc, descs, contrib, args.bands, mode, sh), "tiles warmup")
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(3):
+                hip.check(hip.render_chunks(fdesc, descs, contrib, args.bands, mode, sh), "tiles")
+            e1.record(stream)
+            torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        e = out[int(offs[kb]): int(offs[kb + 1])].double() - ref.double()
+        r = {"ms": round(ms, 2), "mp_s": round(area / (ms / 1e3) / 1e6, 1),
+             "psnr_db_vs_fp64": round(10 * np.log10(1.0 / float((e * e).mean())), 2),
+             "max_abs_err_vs_fp64": float(e.abs().max())}
+        if name == "surrogate":
+            tiles, fb = hip.surrogate_stats()
+            r.update({"tiles": tiles, "fallback_tiles": fb})
+        res[name] = r
+    log(f"surrogate on 8192 tiles: {res}")
+    return res
+
+
 def load_surrogate(hip, args, log):
     """Sets the surrogate's parameters from the trained file, or trains a
     small network on synthetic panoramas when the file is absent.  Returns
@@ -829,6 +880,8 @@ def main():
         small, sgeom, scfas = bench_small(hip, torch, log)
         extra = {"burst_median_demosaic": burst, "uneven_chunks": uneven, "small_raw_chain": small,
                  "preview_export": exp, "colour_transfer": col}
+        if surr_src is not None:
+            extra["surrogate_8192_tiles"] = bench_surrogate_tiles(hip, torch, geom, fdesc, args, log)
 
     cpu = None
     if not args.no_cpu:
