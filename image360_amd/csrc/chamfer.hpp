@@ -43,8 +43,8 @@
 namespace i360 {
 
 struct ChamferWork {
-    const uint8_t* cov;  // w*h coverage (0/1), row pitch w
-    uint8_t* scov;       // sheared copy of the coverage box (written by chamfer_shear_kernel)
+    const uint8_t* cov;  // w*h coverage (0/1), row pitch w; read only by chamfer_shear_kernel
+    uint8_t* scov;       // sheared coverage over the box (chamfer_shear_kernel's copy, or the producer's)
     float* dist;         // sheared distances, written inside the box only
     const int* box;      // x0, y0, x1, y1 (x1 < 0: nothing covered)
     int w, h;

@@ -30,8 +30,8 @@ struct RItem {
     int local;        // index within the chunk's contributor list
     int wtile0, wtilen;  // this item's warp tiles (global tile indices)
     void* warped;     // chunk-sized RGBA (coverage in A), storage type of the form
-    uint8_t* cov;     // chunk-sized coverage bits
-    uint8_t* scov;    // sheared coverage (chamfer input), written by the warp over its footprint
+    uint8_t* scov;    // coverage (0/1) in the chamfer's sheared plane, written by the warp over its
+                      // footprint; read by the chamfer and the seam scan
     float* dist;      // sheared chamfer distances (inside the coverage box)
     int* box;         // coverage box x0,y0,x1,y1 (chunk-local, inclusive)
     int lbox[kMaxLevels][4];  // conservative support box per level (footprint dilated)

@@ -487,7 +487,6 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                     const int y = ty0 + wave * R + r;
                     const size_t p = static_cast<size_t>(y) * ch.w + x;
                     st4(it.warped, p, v, static_cast<S*>(nullptr));
-                    gptr(it.cov)[p] = 1;
                     gptr(it.scov)[ShearAt(x, y, shift, DistPitch(ch.w))] = 1;
                 }
                 if (lx == 0) {
@@ -594,8 +593,8 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                 bx0 = min(bx0, x); by0 = min(by0, y); bx1 = max(bx1, x); by1 = max(by1, y);
             }
             st4(it.warped, p, v, static_cast<S*>(nullptr));
-            gptr(it.cov)[p] = ok[r] ? 1 : 0;
-            gptr(it.scov)[ShearAt(x, y, shift, DistPitch(ch.w))] = ok[r] ? 1 : 0;  // the chamfer's input
+            // coverage, for the chamfer and the seam scan
+            gptr(it.scov)[ShearAt(x, y, shift, DistPitch(ch.w))] = ok[r] ? 1 : 0;
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -1707,7 +1706,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     // layout offsets are relative; absolute pointers are patched after allocation
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
-    struct Pending { size_t warped, cov, scov, dist, nd[kMaxLevels], wt[kMaxLevels]; };
+    struct Pending { size_t warped, scov, dist, nd[kMaxLevels], wt[kMaxLevels]; };
     struct PendingC { size_t boxes, owner, recon[kMaxLevels], lohi, prevl, chosen, back, costs, counters; };
     std::vector<Pending> pit;
     std::vector<PendingC> pch;
@@ -1798,7 +1797,6 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             it.local = static_cast<int>(used.size());
             Pending pd{};
             pd.warped = take(n * 4 * sS);
-            pd.cov = take(n);
             const size_t sheared = static_cast<size_t>(DistPitch(cc.width)) * ShearRows(cc.width, cc.height);
             pd.scov = take(sheared);
             pd.dist = take(sheared * 4);
@@ -1874,7 +1872,6 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     for (size_t i = 0; i < P.items.size(); ++i) {
         RItem& it = P.items[i];
         it.warped = reinterpret_cast<void*>(pit[i].warped);
-        it.cov = reinterpret_cast<uint8_t*>(pit[i].cov);
         it.scov = reinterpret_cast<uint8_t*>(pit[i].scov);
         it.dist = reinterpret_cast<float*>(pit[i].dist);
         for (int l = 1; l < kMaxLevels; ++l) {
@@ -1967,7 +1964,7 @@ static void patch_plan(Plan& P, uint8_t* base) {
         fix(P.sbufs.tflag); fix(P.sbufs.cflag);
     }
     for (auto& it : P.items) {
-        fix(it.warped); fix(it.cov); fix(it.scov); fix(it.dist);
+        fix(it.warped); fix(it.scov); fix(it.dist);
         for (int l = 1; l < kMaxLevels; ++l) { fix(it.nd[l]); fix(it.wt[l]); }
     }
     for (auto& ch : P.chunks) {
@@ -1978,7 +1975,8 @@ static void patch_plan(Plan& P, uint8_t* base) {
     for (size_t i = 0; i < P.items.size(); ++i) {
         const RItem& it = P.items[i];
         const RChunk& ch = P.chunks[it.chunk];
-        P.chamfer[i] = ChamferWork{it.cov, it.scov, it.dist, it.box, ch.w, ch.h};
+        // the warp writes the sheared coverage: no row-major copy
+        P.chamfer[i] = ChamferWork{nullptr, it.scov, it.dist, it.box, ch.w, ch.h};
     }
 }
 
@@ -2021,8 +2019,10 @@ MapImg make_img<MapImg>(const RItem& it, const RChunk& ch) {
     return m;
 }
 
+// covShift: the shear of the coverage planes the warp writes for this plan
+// (the seam scan reads the sheared coverage).
 template <typename Img>
-static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables& T) {
+static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables& T, int covShift) {
     size_t rounds = 0;
     for (auto& pr : P.pairs)
         for (const SeamPair& p : pr) rounds = std::max(rounds, static_cast<size_t>(p.round) + 1);
@@ -2043,7 +2043,8 @@ static void seam_table(const Plan& P, std::vector<SeamWork<Img>>& out, DevTables
                 SeamWork<Img> s{};
                 s.imgI = make_img<Img>(a, ch);
                 s.imgJ = make_img<Img>(b, ch);
-                s.covI = a.cov; s.covJ = b.cov; s.boxes = ch.boxes; s.ci = p.i; s.cj = p.j; s.w = ch.w;
+                s.covI = a.scov; s.covJ = b.scov; s.covShift = covShift;
+                s.boxes = ch.boxes; s.ci = p.i; s.cj = p.j; s.w = ch.w;
                 s.owner = ch.owner;
                 s.lohi = ch.lohi + 2 * sl; s.prevl = ch.prevl + sl; s.chosen = ch.chosen + sl; s.back = ch.back + 64 * sl;
                 s.costs = ch.costs + 64 * 8 * sl;
@@ -2343,9 +2344,10 @@ static void upload_tables(i360hip_ctx* ctx, Plan& P, DevTables& T, hipStream_t s
     std::vector<SeamWork<RgbaImg<float, float>>> s32;
     std::vector<SeamWork<MapImg>> s16;
     T = DevTables{};
-    if (P.mode == kModeFP64) seam_table(P, s64, T);
-    else if (P.mode == kModePlain) seam_table(P, s32, T);
-    else seam_table(P, s16, T);
+    const int covShift = ChamferShear(P.max_w, ctx->tune_strip);  // as the warp's launch
+    if (P.mode == kModeFP64) seam_table(P, s64, T, covShift);
+    else if (P.mode == kModePlain) seam_table(P, s32, T, covShift);
+    else seam_table(P, s16, T, covShift);
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += A256(b); return o; };
     const size_t oI = take(sizeof(RItem) * std::max<size_t>(1, P.items.size()));
@@ -2722,7 +2724,7 @@ int i360hip_render_map_error(i360hip_ctx* ctx, const i360hip_frame* frames, int 
         return st.code;
     }
     // only the item table and warp tiles are used; no workspace needed
-    for (auto& it : P.items) { it.warped = nullptr; it.cov = nullptr; it.scov = nullptr; it.dist = nullptr; }
+    for (auto& it : P.items) { it.warped = nullptr; it.scov = nullptr; it.dist = nullptr; }
     DevTables T;
     upload_tables(ctx, P, T, ctx->stream, st, kSlotRender + 3);
     auto* bits = static_cast<unsigned int*>(ctx_scratch(ctx, kSlotRender + 2, 256, st));

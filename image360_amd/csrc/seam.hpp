@@ -18,6 +18,7 @@
 // movement (2 in RGB48 units, SeamBlend.cpp:309; 2/65535 for [0,1] images).
 #pragma once
 
+#include "chamfer.hpp"
 #include "common.hpp"
 
 namespace i360 {
@@ -89,9 +90,12 @@ __device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w,
 // pairs of a round) scan in parallel.
 // The band's pixel costs sum_c |I_c - J_c| (SeamBlend.cpp:371-378) go to
 // costs[line * 64 + k] for band position lo + k.
+// covI / covJ: the contributors' coverage bytes, row-major (covShift < 0)
+// or in the chamfer's sheared planes (covShift = the launch's shear).
 template <class Img, typename M, int U = 4>
 __device__ __forceinline__ void seam_scan_body(Img imgI, Img imgJ, const uint8_t* __restrict__ covI,
-                                               const uint8_t* __restrict__ covJ, const int* __restrict__ boxes,
+                                               const uint8_t* __restrict__ covJ, int covShift,
+                                               const int* __restrict__ boxes,
                                                int ci, int cj, int w, const int32_t* __restrict__ owner,
                                                int* __restrict__ lohi, int* __restrict__ chosen,
                                                int* __restrict__ contested, M* __restrict__ costs, int line) {
@@ -142,7 +146,9 @@ __device__ __forceinline__ void seam_scan_body(Img imgI, Img imgJ, const uint8_t
         const int t = a + lane;
         bool both = false;
         if (t <= b) {
-            const size_t p = seam_at(rowScan, ox0, oy0, w, line, t);
+            const size_t p = covShift < 0 ? seam_at(rowScan, ox0, oy0, w, line, t)
+                                          : ShearAt(rowScan ? ox0 + t : ox0 + line, rowScan ? oy0 + line : oy0 + t,
+                                                    covShift, DistPitch(w));
             both = gptr(covI)[p] && gptr(covJ)[p];
         }
         const unsigned long long m = __ballot(both);
@@ -180,7 +186,7 @@ __global__ void __launch_bounds__(256) seam_scan_kernel(Img imgI, Img imgJ, cons
                                                         int* __restrict__ chosen, int* __restrict__ contested,
                                                         M* __restrict__ costs) {
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body<Img, M>(imgI, imgJ, covI, covJ, boxes, ci, cj, w, owner, lohi, chosen, contested, costs, line);
+    seam_scan_body<Img, M>(imgI, imgJ, covI, covJ, -1, boxes, ci, cj, w, owner, lohi, chosen, contested, costs, line);
 }
 
 // Overlap of the pair's coverage boxes and its scan direction (the
@@ -476,6 +482,7 @@ struct SeamWork {
     Img imgI, imgJ;
     const uint8_t* covI;
     const uint8_t* covJ;
+    int covShift;      // coverage layout: < 0 row-major, else sheared (seam_scan_body)
     const int* boxes;  // the chunk's contributor boxes
     int ci, cj, w;
     int32_t* owner;
@@ -491,7 +498,7 @@ template <class Img, typename M>
 __global__ void __launch_bounds__(256) seam_scan_table_kernel(const SeamWork<Img>* __restrict__ work) {
     const SeamWork<Img> wk = work[blockIdx.y];
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body<Img, M>(wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi,
+    seam_scan_body<Img, M>(wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.covShift, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi,
                            wk.chosen, wk.contested, static_cast<M*>(wk.costs), line);
 }
 
