@@ -166,7 +166,17 @@ struct TileMap {
     // position at least twice the margin inside the frame, denominator
     // above 1/2): the warp may skip every per-pixel test
     int interior;
+    // interior tiles: the frame texels [sx0, sx0+sw) x [sy0, sy0+sh) hold
+    // every sample of the tile; sw > 0: the warp stages them in LDS
+    int sx0, sy0, sw, sh;
 };
+
+// LDS staging of an interior warp tile's source texels: at most kStageRows
+// frame rows of kStageWords 4-byte words (FP16 RGB, 6 bytes a texel,
+// copied as 16-byte chunks from the 4-byte word at or below the box's
+// first byte).
+constexpr int kStageRows = 32;
+constexpr int kStageWords = 128;
 
 __device__ __forceinline__ TileMap tile_map(const double* h, double X0, double Y0) {
     TileMap t{};
@@ -407,6 +417,35 @@ __global__ void __launch_bounds__(256) rtilemap_kernel(const RItem* __restrict__
     const RChunk& ch = chunks[it.chunk];
     TileMap t = tile_map(it.hinv, static_cast<double>(ch.x + tl.tx * kWarpTW), static_cast<double>(ch.y + tl.ty * kWarpTH));
     t.interior = tile_interior(it, ch, tl, t) ? 1 : 0;
+    if (t.interior) {
+        // The exact positions of the tile's pixels lie in the convex hull of
+        // its corners' (positive denominator), and every FP32 position is
+        // within the margin of its exact one: the corners' FP32 box widened
+        // by 1 + 2 * margin texels holds every pixel's (floor, floor + 1)
+        // sample pair.  Staged only when every chunk load (16 bytes from the
+        // word at or below a row's first byte, through the 4 words past its
+        // last) stays inside the frame and the box fits the LDS tile.
+        float mnx = 3e38f, mny = 3e38f, mxx = -3e38f, mxy = -3e38f;
+        for (int k = 0; k < 4; ++k) {
+            const int dx = (k & 1) ? kWarpTW - 1 : 0, dy = (k & 2) ? kWarpTH - 1 : 0;
+            float px, py;
+            tile_pos(t, dx, dy, tile_den(t, dx, dy), px, py);
+            mnx = fminf(mnx, px); mny = fminf(mny, py);
+            mxx = fmaxf(mxx, px); mxy = fmaxf(mxy, py);
+        }
+        const int pad = 1 + static_cast<int>(ceilf(2.0f * t.margin));
+        const int bx0 = max(t.ix + static_cast<int>(floorf(mnx)) - pad, 0);
+        const int by0 = max(t.iy + static_cast<int>(floorf(mny)) - pad, 0);
+        const int bx1 = min(t.ix + static_cast<int>(floorf(mxx)) + 1 + pad, it.fw - 1);
+        const int by1 = min(t.iy + static_cast<int>(floorf(mxy)) + 1 + pad, it.fh - 1);
+        const long long w0 = (6ll * bx0) >> 2;
+        const long long nw = ((6ll * (bx1 + 1) + 3) >> 2) - w0;
+        const long long lastEnd = 6ll * by1 * it.fw + 4 * (w0 + 4 * ((nw + 3) >> 2));  // bytes
+        if (bx1 >= bx0 && by1 >= by0 && by1 - by0 + 1 <= kStageRows && 4 * ((nw + 3) >> 2) <= kStageWords &&
+            lastEnd <= 6ll * it.fw * it.fh) {
+            t.sx0 = bx0; t.sy0 = by0; t.sw = bx1 - bx0 + 1; t.sh = by1 - by0 + 1;
+        }
+    }
     maps[e] = t;
 }
 
@@ -447,6 +486,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                 const int lx = threadIdx.x & 63;
                 const int x = tx0 + lx;
                 size_t i0[R];
+                int sxa[R], sya[R];
                 float fx[R], fy[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -454,25 +494,17 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                     float px, py;
                     tile_pos(tm, lx, ly, tile_den(tm, lx, ly), px, py);
                     const float flx = floorf(px), fly = floorf(py);
-                    const int sx0 = tm.ix + static_cast<int>(flx), sy0 = tm.iy + static_cast<int>(fly);
+                    sxa[r] = tm.ix + static_cast<int>(flx);
+                    sya[r] = tm.iy + static_cast<int>(fly);
                     fx[r] = px - flx;
                     fy[r] = py - fly;
-                    i0[r] = static_cast<size_t>(sy0) * fw + sx0;
+                    i0[r] = static_cast<size_t>(sya[r]) * fw + sxa[r];
                 }
-                u4a q0[R], q1[R];
-                uint32_t s0[R], s1[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    q0[r] = load_pair_f16(src, i0[r], nbytes, s0[r]);
-                    q1[r] = load_pair_f16(src, i0[r] + fw, nbytes, s1[r]);
-                }
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
+                // blend of row r's texel pairs (a, b) above (c, d), stores
+                auto finish = [&](int r, const _Float16 (&a)[3], const _Float16 (&b)[3], const _Float16 (&c)[3],
+                                  const _Float16 (&d)[3]) {
                     const float omx = 1.0f - fx[r], omy = 1.0f - fy[r];
                     const float wq[4] = {omx * omy, fx[r] * omy, omx * fy[r], fx[r] * fy[r]};
-                    _Float16 a[3], b[3], c[3], d[3];
-                    decode_pair_h(q0[r], s0[r], src, i0[r], a, b);
-                    decode_pair_h(q1[r], s1[r], src, i0[r] + fw, c, d);
                     float v[4];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
@@ -488,6 +520,76 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                     const size_t p = static_cast<size_t>(y) * ch.w + x;
                     st4(it.warped, p, v, static_cast<S*>(nullptr));
                     gptr(it.scov)[ShearAt(x, y, shift, DistPitch(ch.w))] = 1;
+                };
+                if (tm.sw > 0) {  // tile-uniform
+                    // Stage the tile's source box: half a wavefront per box
+                    // row, 16-byte chunks from the word at or below the
+                    // row's first byte (every chunk inside the frame: the
+                    // tile map checked the last one).
+                    __shared__ __attribute__((aligned(16))) uint32_t stage[kStageRows * kStageWords + 4];
+                    const auto* src8 = static_cast<const uint8_t*>(src);
+                    const size_t rowBytes = static_cast<size_t>(fw) * 6;
+                    const int w0 = (6 * tm.sx0) >> 2;
+                    const int nc = (((6 * (tm.sx0 + tm.sw) + 3) >> 2) - w0 + 3) >> 2;  // chunks per row, <= 32
+                    const int hl = lx >> 5, cl = lx & 31;
+                    u4a ldv[kStageRows / 8];
+#pragma unroll
+                    for (int k = 0; k < kStageRows / 8; ++k) {
+                        const int r = 8 * k + 2 * wave + hl;
+                        const bool act = r < tm.sh && cl < nc;
+                        const size_t byte = static_cast<size_t>(tm.sy0 + (act ? r : 0)) * rowBytes +
+                                            4 * static_cast<size_t>(w0 + (act ? 4 * cl : 0));
+                        ldv[k] = *(const I360_GLOBAL u4a*)(src8 + byte);
+                    }
+                    lds_barrier();  // the previous staged tile's samples are read
+#pragma unroll
+                    for (int k = 0; k < kStageRows / 8; ++k) {
+                        const int r = 8 * k + 2 * wave + hl;
+                        if (r < tm.sh && cl < nc) {
+                            typedef uint32_t u4s __attribute__((ext_vector_type(4)));
+                            u4s q;
+                            q.x = ldv[k].x; q.y = ldv[k].y; q.z = ldv[k].z; q.w = ldv[k].w;
+                            *reinterpret_cast<u4s*>(stage + r * kStageWords + 4 * cl) = q;
+                        }
+                    }
+                    lds_barrier();
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int lr = sya[r] - tm.sy0, lc = sxa[r] - tm.sx0;
+                        _Float16 a[3], b[3], c[3], d[3];
+                        if (lr >= 0 && lr + 1 < tm.sh && lc >= 0 && lc + 1 < tm.sw) {
+                            // the pair's 12 bytes start 0 or 2 bytes into word q
+                            const int off = 6 * sxa[r] - 4 * w0;
+                            const uint32_t* t0 = stage + lr * kStageWords + (off >> 2);
+                            const uint32_t* t1 = t0 + kStageWords;
+                            const u4a q0 = {t0[0], t0[1], t0[2], t0[3]};
+                            const u4a q1 = {t1[0], t1[1], t1[2], t1[3]};
+                            decode_pair_h(q0, static_cast<uint32_t>(off & 3), src, i0[r], a, b);
+                            decode_pair_h(q1, static_cast<uint32_t>(off & 3), src, i0[r] + fw, c, d);
+                        } else {  // outside the box (rounding beyond the margin): from the frame
+                            uint32_t s0, s1;
+                            const u4a q0 = load_pair_f16(src, i0[r], nbytes, s0);
+                            const u4a q1 = load_pair_f16(src, i0[r] + fw, nbytes, s1);
+                            decode_pair_h(q0, s0, src, i0[r], a, b);
+                            decode_pair_h(q1, s1, src, i0[r] + fw, c, d);
+                        }
+                        finish(r, a, b, c, d);
+                    }
+                } else {
+                    u4a q0[R], q1[R];
+                    uint32_t s0[R], s1[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        q0[r] = load_pair_f16(src, i0[r], nbytes, s0[r]);
+                        q1[r] = load_pair_f16(src, i0[r] + fw, nbytes, s1[r]);
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        _Float16 a[3], b[3], c[3], d[3];
+                        decode_pair_h(q0[r], s0[r], src, i0[r], a, b);
+                        decode_pair_h(q1[r], s1[r], src, i0[r] + fw, c, d);
+                        finish(r, a, b, c, d);
+                    }
                 }
                 if (lx == 0) {
                     typedef int i4v __attribute__((ext_vector_type(4)));
