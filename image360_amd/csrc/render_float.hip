@@ -403,6 +403,43 @@ __device__ __forceinline__ bool tile_interior(const RItem& it, const RChunk& ch,
     return true;
 }
 
+// Byte offset in the sheared coverage plane of dword t (0..255) of a
+// kWarpTW x kWarpTH warp tile's coverage.  Pixel (x, y) sits at sheared row
+// y + (x >> shift), so the tile's 4-, 8- or 16-byte column groups form a
+// parallelogram; its (row, group) pairs are numbered by sheared row, then
+// group, so a wavefront's 64 dwords form a few contiguous runs along
+// sheared rows (one cache line each) instead of one line per column group.
+// Offsets of the kWarpTH * kWarpTW / 4 dwords are exactly the tile's
+// ShearAt bytes, each once.
+__device__ __forceinline__ size_t shear_tile_dword(int t, int tx0, int ty0, int shift, int pitch) {
+    static_assert(kWarpTW == 64 && kWarpTH == 16, "tile shape of the numbering");
+    const int S = 1 << shift, G = kWarpTW >> shift;  // column groups of the tile
+    const int q = t >> (shift - 2), k = t & ((S >> 2) - 1);  // (row, group) pair, dword in its group
+    // largest d with d (d + 1) / 2 <= v
+    auto tri = [](int v) {
+        int d = static_cast<int>((sqrtf(8.0f * static_cast<float>(v) + 1.0f) - 1.0f) * 0.5f);
+        if ((d + 1) * (d + 2) / 2 <= v) ++d;
+        if (d * (d + 1) / 2 > v) --d;
+        return d;
+    };
+    const int T1 = G * (G + 1) / 2, T3 = G * (G - 1) / 2;  // pairs on sheared rows d < G, d >= kWarpTH
+    int d, gl;
+    if (q < T1) {  // rows d < G: groups 0..d
+        d = tri(q);
+        gl = q - d * (d + 1) / 2;
+    } else if (q < kWarpTH * G - T3) {  // rows G <= d < kWarpTH: all G groups
+        const int q2 = q - T1;
+        d = G + q2 / G;
+        gl = q2 - (d - G) * G;
+    } else {  // rows d >= kWarpTH: groups d - kWarpTH + 1 .. G - 1, counted from the end
+        const int q3 = kWarpTH * G - 1 - q;
+        const int dd = tri(q3);
+        d = kWarpTH + G - 2 - dd;
+        gl = G - 1 - (q3 - dd * (dd + 1) / 2);
+    }
+    return static_cast<size_t>(ty0 + (tx0 >> shift) + d) * pitch + tx0 + gl * S + 4 * k;
+}
+
 // Per-tile FP32 maps of the precision forms, one thread per warp tile
 // (the FP64 reduction runs fully parallel here instead of serially inside
 // the warp's tile loop).
@@ -519,7 +556,6 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                     const int y = ty0 + wave * R + r;
                     const size_t p = static_cast<size_t>(y) * ch.w + x;
                     st4(it.warped, p, v, static_cast<S*>(nullptr));
-                    gptr(it.scov)[ShearAt(x, y, shift, DistPitch(ch.w))] = 1;
                 };
                 if (tm.sw > 0) {  // tile-uniform
                     // Stage the tile's source box: half a wavefront per box
@@ -591,6 +627,10 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                         finish(r, a, b, c, d);
                     }
                 }
+                // the tile's coverage (all 1): one dword per thread, runs
+                // along the sheared rows
+                *gptr(reinterpret_cast<uint32_t*>(it.scov + shear_tile_dword(static_cast<int>(threadIdx.x), tx0, ty0,
+                                                                             shift, DistPitch(ch.w)))) = 0x01010101u;
                 if (lx == 0) {
                     typedef int i4v __attribute__((ext_vector_type(4)));
                     i4v bv;
