@@ -45,7 +45,7 @@ struct RChunk {
     int levels;
     int lw[kMaxLevels], lh[kMaxLevels];
     int* boxes;          // count * 4 (the items' box pointers point here)
-    int32_t* owner;      // w*h
+    int8_t* owner;       // w*h contributor index (-1: none); the caller's owner_out is int32
     float* out;          // FP32 RGB, w*h*3 (caller's); RGB48 when out16
     int out16;           // out holds uint16 RGB48 (the reference's output rounding of v * 65535)
     int32_t* owner_out;  // optional (caller's)

@@ -403,18 +403,19 @@ __device__ __forceinline__ bool tile_interior(const RItem& it, const RChunk& ch,
     return true;
 }
 
-// Byte offset in the sheared coverage plane of dword t (0..255) of a
-// kWarpTW x kWarpTH warp tile's coverage.  Pixel (x, y) sits at sheared row
-// y + (x >> shift), so the tile's 4-, 8- or 16-byte column groups form a
-// parallelogram; its (row, group) pairs are numbered by sheared row, then
-// group, so a wavefront's 64 dwords form a few contiguous runs along
-// sheared rows (one cache line each) instead of one line per column group.
-// Offsets of the kWarpTH * kWarpTW / 4 dwords are exactly the tile's
-// ShearAt bytes, each once.
-__device__ __forceinline__ size_t shear_tile_dword(int t, int tx0, int ty0, int shift, int pitch) {
+// Tile-local first pixel (dx, dy) of the t-th (0..255) run of four pixels
+// x, x+1, x+2, x+3 (x a multiple of 4) of a kWarpTW x kWarpTH tile, in the
+// order that suits the chamfer's sheared planes.  Pixel (x, y) sits at
+// sheared row y + (x >> shift), so the tile's 4-, 8- or 16-column groups
+// form a parallelogram there; its (row, group) pairs are numbered by
+// sheared row, then group, so a wavefront's 64 runs (64 dwords of a byte
+// plane, 64 x 16 bytes of a float plane) lie in a few contiguous stretches
+// of sheared rows instead of one stretch per column group.  The 256 runs
+// are the tile's pixels, each once.
+__device__ __forceinline__ void shear_tile_px(int t, int shift, int& dx, int& dy) {
     static_assert(kWarpTW == 64 && kWarpTH == 16, "tile shape of the numbering");
     const int S = 1 << shift, G = kWarpTW >> shift;  // column groups of the tile
-    const int q = t >> (shift - 2), k = t & ((S >> 2) - 1);  // (row, group) pair, dword in its group
+    const int q = t >> (shift - 2), k = t & ((S >> 2) - 1);  // (row, group) pair, run in its group
     // largest d with d (d + 1) / 2 <= v
     auto tri = [](int v) {
         int d = static_cast<int>((sqrtf(8.0f * static_cast<float>(v) + 1.0f) - 1.0f) * 0.5f);
@@ -437,7 +438,8 @@ __device__ __forceinline__ size_t shear_tile_dword(int t, int tx0, int ty0, int 
         d = kWarpTH + G - 2 - dd;
         gl = G - 1 - (q3 - dd * (dd + 1) / 2);
     }
-    return static_cast<size_t>(ty0 + (tx0 >> shift) + d) * pitch + tx0 + gl * S + 4 * k;
+    dx = gl * S + 4 * k;
+    dy = d - gl;
 }
 
 // Per-tile FP32 maps of the precision forms, one thread per warp tile
@@ -629,8 +631,12 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                 }
                 // the tile's coverage (all 1): one dword per thread, runs
                 // along the sheared rows
-                *gptr(reinterpret_cast<uint32_t*>(it.scov + shear_tile_dword(static_cast<int>(threadIdx.x), tx0, ty0,
-                                                                             shift, DistPitch(ch.w)))) = 0x01010101u;
+                {
+                    int dx, dy;
+                    shear_tile_px(static_cast<int>(threadIdx.x), shift, dx, dy);
+                    *gptr(reinterpret_cast<uint32_t*>(it.scov + ShearAt(tx0 + dx, ty0 + dy, shift, DistPitch(ch.w)))) =
+                        0x01010101u;
+                }
                 if (lx == 0) {
                     typedef int i4v __attribute__((ext_vector_type(4)));
                     i4v bv;
@@ -836,26 +842,33 @@ __global__ void __launch_bounds__(256) rmap_error_kernel(const RItem* __restrict
 // Owner = the contributor with the largest distance, the first on ties
 // (distances are read from the chamfer's sheared planes, shift = the
 // launch's ChamferShear).  A 256-thread block takes one 64x16 tile per
-// iteration, each thread four pixels of one column; the chunk's distance
-// planes and coverage boxes are staged in LDS once per tile (double
-// buffered by iteration, one barrier per tile).  Every contributor's
-// distance load is issued unconditionally (a pixel outside the box or the
-// chunk reads its plane's first element, and the value is discarded), so
-// the loads of several contributors and pixels are in flight together.
+// iteration; the chunk's distance planes and coverage boxes are staged in
+// LDS once per tile (double buffered by iteration, one barrier per tile).
+// Every contributor's distance load is issued unconditionally (a run
+// outside the box or the chunk reads its plane's first elements, and the
+// values are discarded), so the loads of several contributors are in
+// flight together.
 struct OwnerItem {
     const float* dist;
     int box[4];
 };
 constexpr int kOwnerPix = 4;
 
+// Ownership of one 64x16 tile of a chunk: argmax over the contributors of
+// the chamfer distance, > 0, ties to the lowest index (SeamBlend.cpp:244-261).
+// Each thread owns a run of four pixels in the sheared-row order of
+// shear_tile_px: its distances are one 16-byte load per contributor from
+// the sheared plane, and a wavefront's loads lie in a few stretches of
+// sheared rows.  The owner map is written as bytes (a dword when the row
+// pitch keeps the run 4-byte aligned).
 __global__ void __launch_bounds__(kWarpTW * kWarpTH / kOwnerPix) rowner_kernel(const RItem* __restrict__ items,
                                                                               const RChunk* __restrict__ chunks,
                                                                               const RTile* __restrict__ tiles,
                                                                               int ntiles, int shift) {
-    constexpr int K = kOwnerPix;
-    constexpr int kRowStep = kWarpTH / K;
+    static_assert(kOwnerPix == 4 && kWarpTW * kWarpTH / kOwnerPix == 256, "one four-pixel run per thread");
     __shared__ OwnerItem sit[2][I360HIP_MAX_CONTRIBUTORS];
-    const int lx = threadIdx.x & 63, ry = threadIdx.x >> 6;
+    int dx, dy;
+    shear_tile_px(static_cast<int>(threadIdx.x), shift, dx, dy);
     int buf = 0;
     for (int e = blockIdx.x; e < ntiles; e += gridDim.x, buf ^= 1) {
         const RTile tl = tiles[e];
@@ -870,33 +883,38 @@ __global__ void __launch_bounds__(kWarpTW * kWarpTH / kOwnerPix) rowner_kernel(c
             sit[buf][threadIdx.x] = f;
         }
         lds_barrier();
-        const int x = tl.tx * kWarpTW + lx;
-        const int dp = DistPitch(w);
-        int y[K], best[K];
-        float bd[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            y[k] = tl.ty * kWarpTH + ry + kRowStep * k;
-            best[k] = -1;
-            bd[k] = 0.0f;
-        }
-#pragma unroll 2
+        const int x = tl.tx * kWarpTW + dx, y = tl.ty * kWarpTH + dy;
+        const bool live = x < w && y < h;  // the run's first pixel exists (x < w: the run lies in the pitch)
+        const size_t so = live ? ShearAt(x, y, shift, DistPitch(w)) : 0;
+        int best[4] = {-1, -1, -1, -1};
+        float bd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
         for (int i = 0; i < count; ++i) {
-            const OwnerItem& it = sit[buf][i];
-            bool in[K];
-            float v[K];
+            const OwnerItem it = sit[buf][i];  // one LDS read; the box test below is branch-free
+            // bit k: pixel x + k exists and lies in the item's box
+            // (lo in [0, 4], hi in [-1, 3]; the mask is empty when lo > hi)
+            const int lo = min(max(x, it.box[0]) - x, 4), hi = max(min(min(x + 3, it.box[2]), w - 1) - x, -1);
+            const bool rowIn = live & (y >= it.box[1]) & (y <= it.box[3]);
+            const uint32_t in = rowIn ? (((1u << (hi + 1)) - 1u) & ~((1u << lo) - 1u)) : 0u;
+            const f4v v = ld_f4(it.dist + (in ? so : 0));  // distances exist inside the box only
+            const float vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                in[k] = x < w && y[k] < h && x >= it.box[0] && x <= it.box[2] && y[k] >= it.box[1] && y[k] <= it.box[3];
-                v[k] = gptr(it.dist)[in[k] ? ShearAt(x, y[k], shift, dp) : 0];
-            }
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (in[k] && v[k] > bd[k]) { bd[k] = v[k]; best[k] = i; }
+            for (int k = 0; k < 4; ++k)
+                if (((in >> k) & 1u) && vv[k] > bd[k]) { bd[k] = vv[k]; best[k] = i; }
         }
+        if (live) {
+            int8_t* o = ch.owner + static_cast<size_t>(y) * w + x;
+            if ((w & 3) == 0) {  // tile-uniform; the run is whole and 4-byte aligned
+                const uint32_t q = (static_cast<uint32_t>(best[0]) & 0xffu) | ((static_cast<uint32_t>(best[1]) & 0xffu) << 8) |
+                                   ((static_cast<uint32_t>(best[2]) & 0xffu) << 16) |
+                                   ((static_cast<uint32_t>(best[3]) & 0xffu) << 24);
+                *gptr(reinterpret_cast<uint32_t*>(o)) = q;
+            } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (x < w && y[k] < h) gptr(ch.owner)[static_cast<size_t>(y[k]) * w + x] = best[k];
+                for (int k = 0; k < 4; ++k)
+                    if (x + k < w) gptr(o)[k] = static_cast<int8_t>(best[k]);
+            }
+        }
     }
 }
 
@@ -1954,7 +1972,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
         ch.count = static_cast<int>(used.size());
         PendingC pc{};
         pc.boxes = take(16 * static_cast<size_t>(std::max(1, ch.count)));
-        pc.owner = take(n * 4);
+        pc.owner = take(n);
         for (int l = 1; l < ch.levels; ++l) pc.recon[l] = take(static_cast<size_t>(ch.lw[l]) * ch.lh[l] * 4 * sM);
         // pairs in the reference's order (SeamBlend.cpp:573-584); pairs whose
         // footprints are disjoint have disjoint coverage boxes, which the
@@ -2024,7 +2042,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     for (size_t c = 0; c < P.chunks.size(); ++c) {
         RChunk& ch = P.chunks[c];
         ch.boxes = reinterpret_cast<int*>(pch[c].boxes);
-        ch.owner = reinterpret_cast<int32_t*>(pch[c].owner);
+        ch.owner = reinterpret_cast<int8_t*>(pch[c].owner);
         for (int l = 1; l < kMaxLevels; ++l) ch.recon[l] = reinterpret_cast<void*>(pch[c].recon[l]);
         ch.lohi = reinterpret_cast<int*>(pch[c].lohi);
         ch.prevl = reinterpret_cast<int*>(pch[c].prevl);
@@ -2284,7 +2302,14 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
         }
     }
     span(P.own, off, n);
-    RL(st, "ownership", area0 * 8.0, rowner_kernel<<<persistent_grid(ctx, n, 8), kWarpTW * kWarpTH / kOwnerPix, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
+    // algorithmic bytes: each item's distances over its support box (4 B
+    // per pixel), the owner byte of every chunk pixel
+    double own_box_px = 0;
+    for (int i = P.chunks[c0].first; i < P.chunks[c1 - 1].first + P.chunks[c1 - 1].count; ++i) {
+        const int* b = P.items[i].lbox[0];
+        if (b[2] >= b[0] && b[3] >= b[1]) own_box_px += static_cast<double>(b[2] - b[0] + 1) * (b[3] - b[1] + 1);
+    }
+    RL(st, "ownership", 4.0 * own_box_px + area0, rowner_kernel<<<persistent_grid(ctx, n, 8), kWarpTW * kWarpTH / kOwnerPix, 0, s>>>(T.items, T.chunks, T.tiles + off, n,
                                                                                    ChamferShear(P.max_w, ctx->tune_strip)));
     // seams: the k-th pair of every chunk in range, in pair order
     {
@@ -2346,7 +2371,10 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
             n = 0;
             for (int k = c0; k < c1; ++k) n += P.red[k][l].n;
             if (n == 0) continue;
-            const double bytes = static_cast<double>(n) * kRedTW * kRedTH * (4.0 * (4 * sS + sS) + (4 * sS + sS));
+            // per level-(l+1) tile pixel: four level-l pixels (RGBA, and the
+            // weight: at level 0 the owner byte) in, RGBA and weight out
+            const double bytes = static_cast<double>(n) * kRedTW * kRedTH *
+                                 (4.0 * (4 * sS + (l == 0 ? 1.0 : sS)) + (4 * sS + sS));
             if (l == 0 && MM == kModePrecision)
                 RL(st, "reduce_l0", bytes, rreduce_s_kernel<MODE, true><<<persistent_grid(ctx, n), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n, 0, phase, cflag));
             else if (MM == kModePrecision)
@@ -2420,7 +2448,7 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
     if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
     const double out_bytes = (c1 > c0 && P.chunks[c0].out16) ? 6.0 : 12.0;  // RGB48 or FP32 RGB per pixel
-    RL(st, "final_level0", area0 * (4.0 + 4 * sS + out_bytes),
+    RL(st, "final_level0", area0 * (1.0 + 4 * sS + out_bytes),  // owner byte, owner's RGBA, output
        rfinal_kernel<MODE><<<persistent_grid(ctx, n, 8), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
 }
 

@@ -92,11 +92,11 @@ __device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w,
 // costs[line * 64 + k] for band position lo + k.
 // covI / covJ: the contributors' coverage bytes, row-major (covShift < 0)
 // or in the chamfer's sheared planes (covShift = the launch's shear).
-template <class Img, typename M, int U = 4>
+template <class Img, typename M, typename Own, int U = 4>
 __device__ __forceinline__ void seam_scan_body(Img imgI, Img imgJ, const uint8_t* __restrict__ covI,
                                                const uint8_t* __restrict__ covJ, int covShift,
                                                const int* __restrict__ boxes,
-                                               int ci, int cj, int w, const int32_t* __restrict__ owner,
+                                               int ci, int cj, int w, const Own* __restrict__ owner,
                                                int* __restrict__ lohi, int* __restrict__ chosen,
                                                int* __restrict__ contested, M* __restrict__ costs, int line) {
     const int* bI = boxes + 4 * ci;
@@ -186,7 +186,8 @@ __global__ void __launch_bounds__(256) seam_scan_kernel(Img imgI, Img imgJ, cons
                                                         int* __restrict__ chosen, int* __restrict__ contested,
                                                         M* __restrict__ costs) {
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body<Img, M>(imgI, imgJ, covI, covJ, -1, boxes, ci, cj, w, owner, lohi, chosen, contested, costs, line);
+    seam_scan_body<Img, M, int32_t>(imgI, imgJ, covI, covJ, -1, boxes, ci, cj, w, owner, lohi, chosen, contested, costs,
+                                    line);
 }
 
 // Overlap of the pair's coverage boxes and its scan direction (the
@@ -438,8 +439,9 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
 
 // Phase 4 for one line (one wavefront): reassign I/J-owned pixels inside
 // the contested band to the side of the chosen seam (SeamBlend.cpp:409-437).
+template <typename Own>
 __device__ __forceinline__ void seam_reassign_body(const int* __restrict__ boxes, int ci, int cj, int w,
-                                                   int32_t* __restrict__ owner, const int* __restrict__ lohi,
+                                                   Own* __restrict__ owner, const int* __restrict__ lohi,
                                                    const int* __restrict__ chosen, const int* __restrict__ contested,
                                                    int line) {
     SeamGeom g;
@@ -453,7 +455,7 @@ __device__ __forceinline__ void seam_reassign_body(const int* __restrict__ boxes
     const size_t p = seam_at(g.rowScan, g.ox0, g.oy0, w, line, t);
     const int o = gptr(owner)[p];
     if (o != ci && o != cj) return;
-    gptr(owner)[p] = ((t < ch) == g.iLow) ? ci : cj;
+    gptr(owner)[p] = static_cast<Own>(((t < ch) == g.iLow) ? ci : cj);
 }
 
 template <class Img, typename M>
@@ -485,7 +487,7 @@ struct SeamWork {
     int covShift;      // coverage layout: < 0 row-major, else sheared (seam_scan_body)
     const int* boxes;  // the chunk's contributor boxes
     int ci, cj, w;
-    int32_t* owner;
+    int8_t* owner;  // the float render's owner map (contributor index, -1: none)
     int* lohi;
     int* prevl;
     int* chosen;
@@ -498,7 +500,8 @@ template <class Img, typename M>
 __global__ void __launch_bounds__(256) seam_scan_table_kernel(const SeamWork<Img>* __restrict__ work) {
     const SeamWork<Img> wk = work[blockIdx.y];
     const int line = blockIdx.x * kScanLinesPerBlock + (threadIdx.x >> 6);
-    seam_scan_body<Img, M>(wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.covShift, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner, wk.lohi,
+    seam_scan_body<Img, M, int8_t>(wk.imgI, wk.imgJ, wk.covI, wk.covJ, wk.covShift, wk.boxes, wk.ci, wk.cj, wk.w, wk.owner,
+                                   wk.lohi,
                            wk.chosen, wk.contested, static_cast<M*>(wk.costs), line);
 }
 

@@ -1,10 +1,11 @@
-"""The warp's cooperative coverage store for interior tiles
-(shear_tile_dword in image360_amd/csrc/render_float.hip), emulated on the
-CPU: the 256 dwords of a 64x16 tile land exactly on the tile's bytes of the
-chamfer's sheared plane (pixel (x, y) at row y + (x >> log2 S), pitch a
-multiple of 16), each byte once, for every strip width S the chamfer uses;
-and a wavefront's 64 dwords touch few 128-byte lines, where a per-pixel
-byte store touches one line per column group."""
+"""The sheared-row numbering of a 64x16 tile's four-pixel runs
+(shear_tile_px in image360_amd/csrc/render_float.hip), used by the warp's
+coverage store for interior tiles and by the ownership kernel's distance
+loads, emulated on the CPU: the 256 runs are the tile's pixels, each once,
+for every strip width S the chamfer uses (pixel (x, y) at sheared row
+y + (x >> log2 S), pitch a multiple of 16); and a wavefront's 64 runs touch
+few 128-byte lines of a byte plane, where a per-pixel store touches one line
+per column group."""
 import math
 
 import pytest
@@ -12,7 +13,7 @@ import pytest
 TW, TH = 64, 16
 
 
-def shear_tile_dword(t, tx0, ty0, shift, pitch):
+def shear_tile_px(t, shift):
     S, G = 1 << shift, TW >> shift
     q, k = t >> (shift - 2), t & ((S >> 2) - 1)
 
@@ -37,7 +38,17 @@ def shear_tile_dword(t, tx0, ty0, shift, pitch):
         dd = tri(q3)
         d = TH + G - 2 - dd
         gl = G - 1 - (q3 - dd * (dd + 1) // 2)
-    return (ty0 + (tx0 >> shift) + d) * pitch + tx0 + gl * S + 4 * k
+    return gl * S + 4 * k, d - gl
+
+
+def shear_at(x, y, shift, pitch):
+    return (y + (x >> shift)) * pitch + x
+
+
+def shear_tile_dword(t, tx0, ty0, shift, pitch):
+    dx, dy = shear_tile_px(t, shift)
+    assert dx % 4 == 0 and 0 <= dx < TW and 0 <= dy < TH
+    return shear_at(tx0 + dx, ty0 + dy, shift, pitch)
 
 
 @pytest.mark.parametrize("shift", [2, 3, 4])
