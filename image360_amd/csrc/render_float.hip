@@ -987,6 +987,14 @@ __device__ __forceinline__ M blur5m(M a, M b, M m, M d, M e) {
     return (((ta + tb) + tm) + td) + te;
 }
 
+// The same 5-tap binomial blur in symmetric form, k0 (a + e) + k1 (b + d) +
+// k2 m with two fused multiply-adds (5 instructions instead of 9): the
+// FP16-storage forms' reduce (FP32 arithmetic; its rounding differs from
+// blur5m's within FP32, far inside the FP16 storage error).
+__device__ __forceinline__ float blur5s(float a, float b, float m, float d, float e) {
+    return __builtin_fmaf(1.0f / 16.0f, a + e, __builtin_fmaf(4.0f / 16.0f, b + d, (6.0f / 16.0f) * m));
+}
+
 // One tile of level l+1 for one item: level-l samples outside the item's
 // support read as zero.
 // Surrogate phases of the pyramid stages: 0 = every level; 1 = the levels
@@ -1146,44 +1154,55 @@ __global__ void __launch_bounds__(256) rreduce_s_kernel(const RItem* __restrict_
         lds_barrier();  // window staged; the previous tile's vertical pass is done with tmp
         const int nx = next_tile(e + gridDim.x);
         if (nx < ntiles) fetch(nx);
-        for (int idx = threadIdx.x; idx < kRedRows * kRedTW; idx += 256) {
-            const int rr = idx / kRedTW, xx = idx % kRedTW;
+        // horizontal pass: each thread blurs two neighbouring outputs of a
+        // window row (taps 2 xx .. 2 xx + 6, shared)
+        for (int idx = threadIdx.x; idx < kRedRows * (kRedTW / 2); idx += 256) {
+            const int rr = idx / (kRedTW / 2), xx = 2 * (idx % (kRedTW / 2));
             if (ox0 + xx >= ow) continue;
-            M v[5][5];
+            M v[7][5];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
+            for (int k = 0; k < 7; ++k) {
                 const int j = 2 * xx + k;
                 const h4 q = sq[rr * kRed0Cols + j];
-                const M q0 = static_cast<M>(q.x), q1 = static_cast<M>(q.y), q2 = static_cast<M>(q.z), q3 = static_cast<M>(q.w);
-                if (L0) {
-                    v[k][0] = q0 * q3;
-                    v[k][1] = q1 * q3;
-                    v[k][2] = q2 * q3;
-                } else {
-                    v[k][0] = q0;
-                    v[k][1] = q1;
-                    v[k][2] = q2;
-                }
-                v[k][3] = q3;
+                // level 0: the warped plane is premultiplied already (its
+                // coverage is 0 or 1 and its RGB is 0 where uncovered)
+                v[k][0] = static_cast<M>(q.x);
+                v[k][1] = static_cast<M>(q.y);
+                v[k][2] = static_cast<M>(q.z);
+                v[k][3] = static_cast<M>(q.w);
                 v[k][4] = static_cast<M>(sw[rr * kRed0Cols + j]);
             }
 #pragma unroll
-            for (int c = 0; c < 5; ++c) tmp[c][rr][xx] = blur5m<M>(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c]);
+            for (int c = 0; c < 5; ++c) {
+                tmp[c][rr][xx] = blur5s(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c]);
+                tmp[c][rr][xx + 1] = blur5s(v[2][c], v[3][c], v[4][c], v[5][c], v[6][c]);
+            }
         }
         lds_barrier();  // tmp complete (and every read of sq / sw done)
-        for (int idx = threadIdx.x; idx < kRedTH * kRedTW; idx += 256) {
-            const int yy = idx / kRedTW, xx = idx % kRedTW;
+        // vertical pass: each thread blurs two output rows of a column
+        // (window rows 4 yy .. 4 yy + 6, shared)
+        for (int idx = threadIdx.x; idx < (kRedTH / 2) * kRedTW; idx += 256) {
+            const int yy = 2 * (idx / kRedTW), xx = idx % kRedTW;
             const int ox = ox0 + xx, oy = oy0 + yy;
             if (ox >= ow || oy >= oh) continue;
             const int r = 2 * yy + 2;
-            M o[5];
+            M o[2][5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c)
-                o[c] = blur5m<M>(tmp[c][r - 2][xx], tmp[c][r - 1][xx], tmp[c][r][xx], tmp[c][r + 1][xx], tmp[c][r + 2][xx]);
-            const size_t q = static_cast<size_t>(oy) * ow + ox;
-            const M nd[4] = {o[0], o[1], o[2], o[3]};
-            st4(it.nd[l + 1], q, nd, static_cast<_Float16*>(nullptr));
-            st1<_Float16, M>(it.wt[l + 1], q, o[4]);
+            for (int c = 0; c < 5; ++c) {
+                M t[7];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) t[k] = tmp[c][r - 2 + k][xx];
+                o[0][c] = blur5s(t[0], t[1], t[2], t[3], t[4]);
+                o[1][c] = blur5s(t[2], t[3], t[4], t[5], t[6]);
+            }
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+                if (oy + h2 >= oh) continue;
+                const size_t q = static_cast<size_t>(oy + h2) * ow + ox;
+                const M nd[4] = {o[h2][0], o[h2][1], o[h2][2], o[h2][3]};
+                st4(it.nd[l + 1], q, nd, static_cast<_Float16*>(nullptr));
+                st1<_Float16, M>(it.wt[l + 1], q, o[h2][4]);
+            }
         }
         e = nx;
     }
@@ -1682,9 +1701,10 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         for (int k = 0; k < K; ++k) {
             M v[3] = {M(0), M(0), M(0)};
             if (o[k] >= 0) {
-                const M inv0 = (qv[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(qv[k][3]) : M(0);
+                // the owner covers the pixel: its coverage (alpha) is exactly 1
+                // and its hole-filled level-0 Gaussian is the warped value
 #pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) v[c3] = (qv[k][c3] * qv[k][3]) * inv0;
+                for (int c3 = 0; c3 < 3; ++c3) v[c3] = qv[k][c3];
                 if (deep) {
                     const int y = tl.ty * kWarpTH + ry + kRowStep * k;
                     const Coord<M> c = expand_at_s<M>(x, y, sx, sy, lw1, lh1);
