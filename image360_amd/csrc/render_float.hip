@@ -16,6 +16,7 @@
 // and skip the contributor there (a zero weight adds nothing to a band).
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -1819,23 +1820,44 @@ inline size_t A256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 // Groups a chunk's seam pairs into rounds.  A pair (i, j) reads only which
 // of i / j owns a pixel (owners outside {i, j} count as "neither") and
 // rewrites only pixels owned by i or j, to i or j (SeamBlend.cpp:336-343,
-// 437-442); its bands, costs and sides come from coverage, images and
-// boxes, which no pair changes.  Two pairs with no contributor in common
-// therefore commute, and any order that keeps the reference order among
-// pairs sharing a contributor gives the reference's owner map.  A pair's
-// round is one past the latest round of an earlier pair it shares a
-// contributor with; pairs of one round run side by side, each in its own
-// scratch slot.  Returns the number of slots (largest round).
-int schedule_pairs(std::vector<SeamPair>& pr) {
-    std::vector<int> last(I360HIP_MAX_CONTRIBUTORS, -1);  // latest round touching each contributor
+// 437-442), and it reads and rewrites owners only inside the overlap of
+// i's and j's coverage boxes (SeamBlend.cpp:313-325, 429-442); its bands,
+// costs and sides come from coverage, images and boxes, which no pair
+// changes.  Two pairs commute, then, when they have no contributor in
+// common, or when the overlaps of their support boxes (which hold the
+// coverage boxes) are disjoint.  Any order that keeps the reference order
+// among the other pairs gives the reference's owner map: a pair's round is
+// one past the latest round of an earlier pair it does not commute with,
+// and the pairs of one round run side by side, each in its own scratch
+// slot.  `boxes` holds each contributor's support box.  Returns the
+// number of slots (largest round).
+int schedule_pairs(std::vector<SeamPair>& pr, const std::vector<std::array<int, 4>>& boxes) {
     std::vector<int> fill;
     int slots = 0;
-    for (SeamPair& p : pr) {
-        p.round = std::max(last[p.i], last[p.j]) + 1;
-        last[p.i] = last[p.j] = p.round;
-        if (static_cast<int>(fill.size()) <= p.round) fill.resize(p.round + 1, 0);
-        p.slot = fill[p.round]++;
-        slots = std::max(slots, fill[p.round]);
+    auto overlap = [&](const SeamPair& p, int (&o)[4]) {
+        const auto& a = boxes[p.i];
+        const auto& b = boxes[p.j];
+        o[0] = std::max(a[0], b[0]); o[1] = std::max(a[1], b[1]);
+        o[2] = std::min(a[2], b[2]); o[3] = std::min(a[3], b[3]);
+    };
+    for (size_t k = 0; k < pr.size(); ++k) {
+        SeamPair& p = pr[k];
+        int op[4];
+        overlap(p, op);
+        int r = 0;
+        for (size_t q = 0; q < k; ++q) {
+            const SeamPair& e = pr[q];
+            if (e.i != p.i && e.i != p.j && e.j != p.i && e.j != p.j) continue;  // no contributor in common
+            int oe[4];
+            overlap(e, oe);
+            if (std::max(op[0], oe[0]) > std::min(op[2], oe[2]) || std::max(op[1], oe[1]) > std::min(op[3], oe[3]))
+                continue;  // disjoint overlaps
+            r = std::max(r, e.round + 1);
+        }
+        p.round = r;
+        if (static_cast<int>(fill.size()) <= r) fill.resize(r + 1, 0);
+        p.slot = fill[r]++;
+        slots = std::max(slots, fill[r]);
     }
     return slots;
 }
@@ -2005,7 +2027,12 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
                 if (std::max(a[0], b[0]) <= std::min(a[2], b[2]) && std::max(a[1], b[1]) <= std::min(a[3], b[3]))
                     pr.push_back({i, j, 0, 0});
             }
-        const int nslots = schedule_pairs(pr);
+        std::vector<std::array<int, 4>> sboxes(ch.count);
+        for (int i = 0; i < ch.count; ++i) {
+            const int* a = P.items[ch.first + i].lbox[0];
+            sboxes[i] = {a[0], a[1], a[2], a[3]};
+        }
+        const int nslots = schedule_pairs(pr, sboxes);
         const size_t lines = static_cast<size_t>(std::max(cc.width, cc.height)) * std::max(1, nslots);
         pc.lohi = take(8 * lines); pc.prevl = take(4 * lines); pc.chosen = take(4 * lines); pc.back = take(64 * lines);
         pc.costs = take(64 * 8 * lines);

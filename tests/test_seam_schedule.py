@@ -1,10 +1,14 @@
 """The GPU render runs a chunk's seam pairs in rounds: a pair's round is one
-past the latest round of an earlier pair that shares a contributor, and the
-pairs of one round run side by side.  That equals the reference's
-sequential pair order (SeamBlend.cpp:573-584) because pairs with no
-contributor in common commute.  Checked here on the oracle (CPU): the
-round-grouped order gives the reference order's owner map on seeded grid
-scenes, while an order that breaks the shared-contributor order does not."""
+past the latest round of an earlier pair it does not commute with, and the
+pairs of one round run side by side.  Two pairs commute when they share no
+contributor, or when their overlap boxes are disjoint: a pair reads and
+rewrites owners only of its own two contributors and only inside its
+overlap box.  The rounds therefore give the reference's sequential pair
+order's result (SeamBlend.cpp:573-584).  Checked here on the oracle (CPU):
+the round-grouped order gives the reference order's owner map on seeded
+grid scenes, with fewer rounds than the shared-contributor rule alone,
+while an order that breaks the order of pairs that do not commute does
+not."""
 import numpy as np
 import pytest
 
@@ -12,18 +16,30 @@ from oracle import pyoracle as po
 from tests.render_cases import grid_case
 
 
-def schedule(pairs):
-    last = {}
+def _disjoint(a, b):
+    return max(a[0], b[0]) > min(a[2], b[2]) or max(a[1], b[1]) > min(a[3], b[3])
+
+
+def schedule(pairs, spatial=True):
+    """Round of each (i, j, overlap box) pair, as schedule_pairs in
+    image360_amd/csrc/render_float.hip (spatial=False: the shared-contributor
+    rule alone)."""
     rounds = []
-    for i, j in pairs:
-        r = max(last.get(i, -1), last.get(j, -1)) + 1
-        last[i] = last[j] = r
+    for k, (i, j, ov) in enumerate(pairs):
+        r = 0
+        for q in range(k):
+            a, b, oq = pairs[q]
+            if not ({a, b} & {i, j}):
+                continue
+            if spatial and _disjoint(ov, oq):
+                continue
+            r = max(r, rounds[q] + 1)
         rounds.append(r)
     return rounds
 
 
-def _scene(seed):
-    g, frames = grid_case(seed=seed, nx=3, ny=2, fw=120, fh=90)
+def _scene(seed, nx=3, ny=2):
+    g, frames = grid_case(seed=seed, nx=nx, ny=ny, fw=120, fh=90)
     c = g.chunks[0]
     warped = []
     for i in c.contributors:
@@ -52,16 +68,27 @@ def _run(warped, dists, boxes, order):
     return owner
 
 
-@pytest.mark.parametrize("seed", [7, 8, 9])
-def test_round_order_equals_reference_order(seed):
-    warped, dists, boxes, pairs = _scene(seed)
-    rounds = schedule([(i, j) for i, j, _ in pairs])
-    assert max(rounds) + 1 < len(pairs)  # some pairs do share a round
+@pytest.mark.parametrize("seed,nx,ny", [(7, 3, 2), (8, 3, 2), (9, 3, 2), (7, 4, 1), (8, 4, 1)])
+@pytest.mark.parametrize("spatial", [False, True])
+def test_round_order_equals_reference_order(seed, nx, ny, spatial):
+    warped, dists, boxes, pairs = _scene(seed, nx, ny)
+    rounds = schedule(pairs, spatial)
+    if spatial or ny > 1:
+        assert max(rounds) + 1 < len(pairs)  # some pairs do share a round
     want = _run(warped, dists, boxes, pairs)
     # pairs of one round in reverse order: still the same owner map
     grouped = sorted(range(len(pairs)), key=lambda k: (rounds[k], -k))
     got = _run(warped, dists, boxes, [pairs[k] for k in grouped])
     assert np.array_equal(got, want)
+
+
+def test_disjoint_overlaps_cut_rounds():
+    # a strip of four frames: a frame's pairs with its left and right
+    # neighbours overlap on opposite sides of it, so they share a round
+    # although they share a contributor
+    for seed in (7, 8):
+        _, _, _, pairs = _scene(seed, 4, 1)
+        assert max(schedule(pairs, True)) < max(schedule(pairs, False))
 
 
 def test_breaking_shared_order_changes_the_result():
