@@ -2674,18 +2674,34 @@ static void enqueue_groups(i360hip_ctx* ctx, const Plan& P, const DevTables& T, 
     RenderCache& rc = cache_of(ctx);
     double total = 0;
     for (const auto& c : P.chunks) total += static_cast<double>(c.w) * c.h;
-    // split points: equal shares of the output area
+    // Split points by output area.  Staggered, the step ends with the last
+    // group's pyramids (after every warp phase and its own serial chain),
+    // and no group's pyramids start before the first group's chain ends
+    // (after the first warp phase): so the first and the last group get two
+    // shares of the area and every other group three (20/30/30/20 % for
+    // four groups: gigapixel step 30.45-31.27 -> 29.35-29.66 ms against
+    // equal shares, three runs each on one box).  Side by side, equal
+    // shares.
+    const bool stagger = render_stagger();
+    std::vector<double> share_of(G, 1.0);
+    if (stagger && G > 2)
+        for (int g = 0; g < G; ++g) share_of[g] = (g == 0 || g == G - 1) ? 2.0 : 3.0;
+    double wsum = 0;
+    for (double v : share_of) wsum += v;
     std::vector<int> cut{0};
-    double acc = 0;
+    double acc = 0, bound = total * share_of[0] / wsum;
     for (int c = 0; c < nch && static_cast<int>(cut.size()) < G; ++c) {
         acc += static_cast<double>(P.chunks[c].w) * P.chunks[c].h;
-        if (acc >= total * static_cast<double>(cut.size()) / G && c + 1 < nch) cut.push_back(c + 1);
+        if (acc >= bound && c + 1 < nch) {
+            cut.push_back(c + 1);
+            if (static_cast<int>(cut.size()) < G) bound += total * share_of[cut.size() - 1] / wsum;
+        }
     }
     cut.push_back(nch);
     const int ng = static_cast<int>(cut.size()) - 1;
     int share = ctx->tune_share > 0 ? ctx->tune_share : 1;
     if (const char* e = std::getenv("I360HIP_GRID_SHARE")) share = std::max(1, std::atoi(e));
-    if (render_stagger()) {
+    if (stagger) {
         // Each fork event is waited on right after it is recorded, before
         // anything more is enqueued on the origin stream.
         g_grid_share = share;
