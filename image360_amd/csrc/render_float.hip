@@ -1893,8 +1893,11 @@ uint64_t mix(uint64_t h, uint64_t v) {
 }  // namespace
 
 // Builds the plan (host tables + workspace layout); returns false on error.
+// `caller`: the caller's index of each planned chunk, for messages (null:
+// the planned order is the caller's).
 static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk* chunks, int nc, const int* contrib,
-                       int bands, int mode, Plan& P, Status& st, int surr = kSurrNone, bool out16 = false) {
+                       int bands, int mode, Plan& P, Status& st, int surr = kSurrNone, bool out16 = false,
+                       const int* caller = nullptr) {
     P = Plan{};
     P.mode = mode;
     P.surr = (mode == kModeBatched) ? surr : kSurrNone;
@@ -1917,15 +1920,15 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
     for (int ci = 0; ci < nc; ++ci) {
         const i360hip_chunk& cc = chunks[ci];
         if (cc.width <= 0 || cc.height <= 0 || !cc.out || cc.count < 0 || cc.first < 0) {
-            st.set(I360HIP_INVALID_PARAM, "chunk %d: invalid rectangle, output or contributor range", ci);
+            st.set(I360HIP_INVALID_PARAM, "chunk %d: invalid rectangle, output or contributor range", caller ? caller[ci] : ci);
             return false;
         }
         if (cc.width > 16384) {
-            st.set(I360HIP_INVALID_PARAM, "chunk %d wider than 16384 px", ci);
+            st.set(I360HIP_INVALID_PARAM, "chunk %d wider than 16384 px", caller ? caller[ci] : ci);
             return false;
         }
         if (static_cast<uint64_t>(cc.width) * static_cast<uint64_t>(cc.height) >= (1ull << 31)) {
-            st.set(I360HIP_INVALID_PARAM, "chunk %d has 2^31 pixels or more", ci);
+            st.set(I360HIP_INVALID_PARAM, "chunk %d has 2^31 pixels or more", caller ? caller[ci] : ci);
             return false;
         }
         key = mix(key, static_cast<uint64_t>(cc.x) * 1000003ull + cc.y);
@@ -1950,7 +1953,7 @@ static bool build_plan(const i360hip_frame* frames, int nf, const i360hip_chunk*
             const int f = contrib[cc.first + k];
             key = mix(key, static_cast<uint64_t>(f));
             if (f < 0 || f >= nf) {
-                st.set(I360HIP_INVALID_PARAM, "chunk %d: contributor %d out of range", ci, f);
+                st.set(I360HIP_INVALID_PARAM, "chunk %d: contributor %d out of range", caller ? caller[ci] : ci, f);
                 return false;
             }
             const i360hip_frame& fr = frames[f];
@@ -2805,7 +2808,31 @@ static void run_render(i360hip_ctx* ctx, const i360hip_frame* frames, int nf, co
         return;
     }
     Plan P;
-    if (!build_plan(frames, nf, chunks, nc, contrib, bands, mode, P, st, surr, out16)) return;
+    // The batched form plans the chunks lightest first (fewest contributors,
+    // then smallest area; canvas order on ties).  Each chunk group's serial
+    // chain (chamfer, seam rounds) is as long as its heaviest chunk's, and
+    // no group's pyramids start before the first group's chain ends: with
+    // the heavy chunks gathered in the last group, the earlier groups' chains
+    // end sooner (gigapixel step 29.6-30.3 -> 28.8-29.1 ms, canvas order
+    // against this order, three runs each).  Outputs go to each chunk's own
+    // buffers, so the order changes no result.
+    std::vector<i360hip_chunk> order;
+    std::vector<int> idx;
+    const i360hip_chunk* plan_chunks = chunks;
+    if (graphable && surr != kSurrExport && nc > 2) {
+        idx.resize(nc);
+        for (int c = 0; c < nc; ++c) idx[c] = c;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+            if (chunks[a].count != chunks[b].count) return chunks[a].count < chunks[b].count;
+            return static_cast<int64_t>(chunks[a].width) * chunks[a].height <
+                   static_cast<int64_t>(chunks[b].width) * chunks[b].height;
+        });
+        order.reserve(nc);
+        for (int c : idx) order.push_back(chunks[c]);
+        plan_chunks = order.data();
+    }
+    if (!build_plan(frames, nf, plan_chunks, nc, contrib, bands, mode, P, st, surr, out16, idx.empty() ? nullptr : idx.data()))
+        return;
     SurrRun sr;
     sr.mode = surr;
     sr.weights = ctx->surr_weights;
