@@ -2498,7 +2498,10 @@ static void enqueue(i360hip_ctx* ctx, const Plan& P, const DevTables& T, int c0,
     if (exporting) return;  // the export writes no chunk output
     span(P.fin, off, n);
     const double out_bytes = (c1 > c0 && P.chunks[c0].out16) ? 6.0 : 12.0;  // RGB48 or FP32 RGB per pixel
-    RL(st, "final_level0", area0 * (1.0 + 4 * sS + out_bytes),  // owner byte, owner's RGBA, output
+    // per chunk pixel: the owner byte, the owner's RGBA, a quarter texel of
+    // the level-1 reconstruction (RGBA, maths type) and of the owner's
+    // level-1 numerator / denominator, the output
+    RL(st, "final_level0", area0 * (1.0 + 4 * sS + (4 * sM + 4 * sS) / 4.0 + out_bytes),
        rfinal_kernel<MODE><<<persistent_grid(ctx, n, 8), 256, 0, s>>>(T.items, T.chunks, T.tiles + off, n));
 }
 
