@@ -301,65 +301,73 @@ __device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, 
         const bool more = b0 + kDpLines < lines;
         if (more) fetch(b0 + kDpLines);
         const int nb = min(kDpLines, lines - b0);
-        for (int l = 0; l < nb; ++l) {
-            const int lo = __builtin_amdgcn_readlane(bcur, 2 * l);
-            const int hi = __builtin_amdgcn_readlane(bcur, 2 * l + 1);
-            if (hi < lo) continue;  // block-uniform
-            const int lw = hi - lo + 1;
-            const M c = sh.cost[buf][l][lane];
-            M v = kOut;
-            int bp = -1;
-            if (last < 0) {
-                v = (lane < lw) ? c : kOut;
-            } else {
-                if (lo - lastLo != pshift) {  // uniform; the seam band mostly keeps its offset
-                    pshift = lo - lastLo;
-                    const M d0 = static_cast<M>(pshift + lane - P0);
+        for (int l = 0; l < nb;) {
+            // Lines run in stretches of one band shift lo - lastLo: the
+            // penalties are formed at the head of a stretch and stay fixed
+            // (loop-invariant, so they are not copied line to line) until a
+            // contested line's shift differs.
+            const int lo0 = __builtin_amdgcn_readlane(bcur, 2 * l);
+            if (last >= 0 && lo0 - lastLo != pshift) {  // uniform
+                pshift = lo0 - lastLo;
+                const M d0 = static_cast<M>(pshift + lane - P0);
 #pragma unroll
-                    for (int i = 0; i < kSpan; ++i) pen[i] = Img::smooth() * fabs(d0 - static_cast<M>(i));
-                }
-                M cv[kSpan];
-                int ck[kSpan];
+                for (int i = 0; i < kSpan; ++i) pen[i] = Img::smooth() * fabs(d0 - static_cast<M>(i));
+            }
+            for (; l < nb; ++l) {
+                const int lo = __builtin_amdgcn_readlane(bcur, 2 * l);
+                const int hi = __builtin_amdgcn_readlane(bcur, 2 * l + 1);
+                if (hi < lo) continue;  // block-uniform
+                if (last >= 0 && lo - lastLo != pshift) break;  // uniform: a new stretch
+                const int lw = hi - lo + 1;
+                const M c = sh.cost[buf][l][lane];
+                M v = kOut;
+                int bp = -1;
+                if (last < 0) {
+                    v = (lane < lw) ? c : kOut;
+                } else {
+                    M cv[kSpan];
+                    int ck[kSpan];
 #pragma unroll
-                for (int i = 0; i < kSpan; ++i) {
-                    cv[i] = lane_value(dpv, P0 + i) + pen[i];
-                    ck[i] = i;
-                }
-#pragma unroll
-                for (int st = 1; st < kSpan; st *= 2)
-#pragma unroll
-                    for (int i = 0; i + st < kSpan; i += 2 * st) {
-                        ck[i] = (cv[i + st] < cv[i]) ? ck[i + st] : ck[i];
-                        cv[i] = __builtin_fmin(cv[i], cv[i + st]);  // no NaN; a tie keeps the same bits
+                    for (int i = 0; i < kSpan; ++i) {
+                        cv[i] = lane_value(dpv, P0 + i) + pen[i];
+                        ck[i] = i;
                     }
-                const int pb = ord & 1;
-                ++ord;
-                sh.part[pb][W][lane] = Best::pack(cv[0], P0 + ck[0]);
-                lds_barrier();
+#pragma unroll
+                    for (int st = 1; st < kSpan; st *= 2)
+#pragma unroll
+                        for (int i = 0; i + st < kSpan; i += 2 * st) {
+                            ck[i] = (cv[i + st] < cv[i]) ? ck[i + st] : ck[i];
+                            cv[i] = __builtin_fmin(cv[i], cv[i + st]);  // no NaN; a tie keeps the same bits
+                        }
+                    const int pb = ord & 1;
+                    ++ord;
+                    sh.part[pb][W][lane] = Best::pack(cv[0], P0 + ck[0]);
+                    lds_barrier();
+                    if (own) {
+                        typename Best::T r[kDpWaves];  // all read before any is used
+#pragma unroll
+                        for (int q = 0; q < kDpWaves; ++q) r[q] = sh.part[pb][q][lane];
+                        M best = Best::value(r[0]);
+                        bp = Best::index(r[0]);
+#pragma unroll
+                        for (int q = 1; q < kDpWaves; ++q) {
+                            const M bq = Best::value(r[q]);
+                            const bool lt = bq < best;
+                            bp = lt ? Best::index(r[q]) : bp;
+                            best = lt ? bq : best;
+                        }
+                        v = (lane < lw) ? c + best : kOut;
+                    }
+                }
                 if (own) {
-                    typename Best::T r[kDpWaves];  // all read before any is used
-#pragma unroll
-                    for (int q = 0; q < kDpWaves; ++q) r[q] = sh.part[pb][q][lane];
-                    M best = Best::value(r[0]);
-                    bp = Best::index(r[0]);
-#pragma unroll
-                    for (int q = 1; q < kDpWaves; ++q) {
-                        const M bq = Best::value(r[q]);
-                        const bool lt = bq < best;
-                        bp = lt ? Best::index(r[q]) : bp;
-                        best = lt ? bq : best;
-                    }
-                    v = (lane < lw) ? c + best : kOut;
+                    sh.bk[buf][l][lane] = static_cast<uint8_t>(bp & 0xFF);
+                    dpv = v;
                 }
+                if (W == 0 && lane == 0) sh.prv[buf][l] = last;
+                last = b0 + l;
+                lastLo = lo;
+                lastW = lw;
             }
-            if (own) {
-                sh.bk[buf][l][lane] = static_cast<uint8_t>(bp & 0xFF);
-                dpv = v;
-            }
-            if (W == 0 && lane == 0) sh.prv[buf][l] = last;
-            last = b0 + l;
-            lastLo = lo;
-            lastW = lw;
         }
         if (more) commit(buf ^ 1);
         bcur = bnext;
