@@ -53,7 +53,6 @@ constexpr int kSeamRadius = 24;  // SeamBlend.cpp:319
 constexpr int kDpWaves = 4;      // wavefronts of the DP kernel (one per SIMD)
 constexpr int kDpThreads = 64 * kDpWaves;
 constexpr int kDpLines = 16;     // cost rows staged in LDS per batch
-constexpr int kBackBatch = 256;
 
 // A wavefront's partial minimum for one lane, value bits and candidate
 // index packed in one vector, written and read as one LDS access (separate
@@ -404,44 +403,61 @@ __device__ __forceinline__ void seam_dp_body(const int* __restrict__ contested, 
         default: seam_dp_forward<Img, M, 3>(sh, lines, costs, lohi, prevl, back, last, lastW); break;
     }
     if (last < 0) return;  // no contested pixel in this overlap (block-uniform)
-    __syncthreads();
-    M* s_dpfinal = sh.dpfinal;
+    __syncthreads();  // the last line's dp row in LDS, back-pointer rows and predecessors in memory
+    if (wave != 0) return;  // the backtrack is one wavefront's: no barrier follows
 
-    // Phase 3: backtrack through back-pointers staged in LDS, from the end.
-    __shared__ uint8_t sback[kBackBatch][64];
-    __shared__ int sprev[kBackBatch], slo2[kBackBatch];
-    __shared__ int s_line, s_k;
-    if (threadIdx.x == 0) {
+    // Phase 3: backtrack from the best end position.  The wavefront holds the
+    // back-pointer bytes of 64 lines in registers (register j: line b0 + j,
+    // lane: position) and each line's predecessor and band start in lane j,
+    // and follows the chain with readlane, so a step costs no memory round
+    // trip; the next 64 lines load while a batch is walked.
+    const int lane = static_cast<int>(threadIdx.x);
+    int k = 0;
+    {
+        const M f = lane < lastW ? sh.dpfinal[lane] : M(0);
         M best = SeamMax<M>::value;
-        int bk = 0;
-        for (int k = 0; k < lastW; ++k)
-            if (s_dpfinal[k] < best) { best = s_dpfinal[k]; bk = k; }
-        s_line = last;
-        s_k = bk;
-    }
-    __syncthreads();
-    for (int b1 = last + 1; b1 > 0; b1 -= kBackBatch) {
-        const int b0 = max(0, b1 - kBackBatch);
-        for (int idx = threadIdx.x; idx < (b1 - b0) * 64; idx += blockDim.x)
-            sback[idx >> 6][idx & 63] = gptr(back)[static_cast<size_t>(b0 + (idx >> 6)) * 64 + (idx & 63)];
-        for (int idx = threadIdx.x; idx < b1 - b0; idx += blockDim.x) {
-            sprev[idx] = gptr(prevl)[b0 + idx];
-            slo2[idx] = gptr(lohi)[2 * (b0 + idx)];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int line = s_line, k = s_k;
-            while (line >= b0) {
-                gptr(chosen)[line] = slo2[line - b0] + k;
-                const int pk = static_cast<int>(sback[line - b0][k]);
-                line = sprev[line - b0];
-                k = pk;
+        for (int q = 0; q < lastW; ++q) {  // the first minimum (strict <)
+            const M v = lane_value(f, q);
+            if (v < best) {
+                best = v;
+                k = q;
             }
-            s_line = line;
-            s_k = k;
         }
-        __syncthreads();
-        if (s_line < 0) break;
+    }
+    int bkr[64], nbkr[64];
+    int prv = 0, lo0 = 0, nprv = 0, nlo0 = 0;
+    auto load = [&](int b0, int (&r)[64], int& pv, int& lv) {  // rows b0.. (<= last)
+#pragma unroll
+        for (int j = 0; j < 64; ++j) r[j] = gptr(back)[static_cast<size_t>(min(b0 + j, last)) * 64 + lane];
+        const int l = min(b0 + lane, last);
+        pv = gptr(prevl)[l];
+        lv = gptr(lohi)[2 * l];
+    };
+    int line = last;
+    int b0 = line & ~63;
+    load(b0, nbkr, nprv, nlo0);
+    while (true) {  // uniform; line >= 0 and in [b0, b0 + 64), whose rows are in nbkr
+        // the walked batch is copied out of the load targets first, so no
+        // load is in flight for it while the next batch loads
+#pragma unroll
+        for (int j = 0; j < 64; ++j) bkr[j] = nbkr[j];
+        prv = nprv;
+        lo0 = nlo0;
+        const int nb0 = b0 - 64;
+        if (nb0 >= 0) load(nb0, nbkr, nprv, nlo0);
+        int ch = -1;
+#pragma unroll
+        for (int j = 63; j >= 0; --j) {
+            if (line == b0 + j) {  // uniform: line b0 + j is on the chain
+                ch = lane == j ? __builtin_amdgcn_readlane(lo0, j) + k : ch;
+                k = __builtin_amdgcn_readlane(bkr[j], k & 63);
+                line = __builtin_amdgcn_readlane(prv, j);
+            }
+        }
+        if (ch >= 0) gptr(chosen)[b0 + lane] = ch;
+        if (line < 0) break;
+        b0 = line & ~63;
+        if (b0 != nb0) load(b0, nbkr, nprv, nlo0);  // the chain skipped a batch of uncontested lines
     }
 }
 
