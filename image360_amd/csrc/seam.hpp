@@ -5,13 +5,13 @@
 //  scan     one wavefront per line: the crossover of the current owner map
 //           and the contested band around it (ballots), then the band's
 //           pixel costs |I - J|, written to a per-line cost row;
-//  dp       one block per pair: the line-to-line DP, lane k = candidate
-//           position lo+k; the previous line's candidates (at most 49) are
-//           split into kDpWaves ranges, one wavefront each (readlane
-//           broadcasts, a tree minimum), and the partial minima are
-//           combined in range order with a strict <, so the first minimum
-//           wins as in the reference's serial scan; then the backtrack
-//           through back-pointers staged in LDS;
+//  dp       one block per pair: the line-to-line DP over band positions
+//           lo+k, four lanes per position, each lane a quarter of the
+//           previous line's candidates (at most 49, read from LDS, a tree
+//           minimum), the quarters' minima joined across lanes with a
+//           strict <, so the first minimum wins as in the reference's
+//           serial scan; then the backtrack through back-pointer rows held
+//           in registers;
 //  reassign one wavefront per line: I/J-owned pixels inside each contested
 //           band go to the side of the chosen seam.
 // Img supplies rgb(p, M[3]) and the smoothness weight per pixel of seam
@@ -53,29 +53,6 @@ constexpr int kSeamRadius = 24;  // SeamBlend.cpp:319
 constexpr int kDpWaves = 4;      // wavefronts of the DP kernel (one per SIMD)
 constexpr int kDpThreads = 64 * kDpWaves;
 constexpr int kDpLines = 16;     // cost rows staged in LDS per batch
-
-// A wavefront's partial minimum for one lane, value bits and candidate
-// index packed in one vector, written and read as one LDS access (separate
-// scalar loads let the compiler load the index from a selected address,
-// after the compares).
-template <typename M> struct SeamBest;
-template <> struct SeamBest<double> {
-    typedef int T __attribute__((ext_vector_type(4)));
-    __device__ static T pack(double v, int k) {
-        const long long b = __double_as_longlong(v);
-        return T{static_cast<int>(b), static_cast<int>(b >> 32), k, 0};
-    }
-    __device__ static double value(T t) {
-        return __longlong_as_double((static_cast<long long>(t.y) << 32) | static_cast<unsigned int>(t.x));
-    }
-    __device__ static int index(T t) { return t.z; }
-};
-template <> struct SeamBest<float> {
-    typedef int T __attribute__((ext_vector_type(2)));
-    __device__ static T pack(float v, int k) { return T{__float_as_int(v), k}; }
-    __device__ static float value(T t) { return __int_as_float(t.x); }
-    __device__ static int index(T t) { return t.y; }
-};
 
 __device__ __forceinline__ size_t seam_at(bool rowScan, int ox0, int oy0, int w, int line, int t) {
     return rowScan ? static_cast<size_t>(oy0 + line) * w + (ox0 + t) : static_cast<size_t>(oy0 + t) * w + (ox0 + line);
@@ -216,29 +193,73 @@ __device__ __forceinline__ bool seam_geom(const int* __restrict__ boxes, int ci,
 // LDS of the DP kernel.
 template <typename M>
 struct SeamDpShared {
-    static constexpr int kSpan = (2 * kSeamRadius + 1 + kDpWaves - 1) / kDpWaves;
-    M cost[2][kDpLines][64];                               // cost rows of a batch (double buffered)
-    typename SeamBest<M>::T part[2][kDpWaves][64];          // per-wave partial minima of a line
-    __align__(16) uint8_t bk[2][kDpLines][64];              // back-pointer rows of a batch
-    int prv[2][kDpLines];                                   // predecessor line of each line of a batch
-    M dpfinal[64];                                          // dp row of the last contested line
+    static constexpr int kSpan = (2 * kSeamRadius + 1 + 3) / 4;  // candidates per lane: 4 groups cover a band
+    M cost[2][kDpLines][64];                   // cost rows of a batch (double buffered)
+    M dpv[2][64];                              // dp row of the last two contested lines (by position)
+    __align__(16) uint8_t bk[2][kDpLines][64];  // back-pointer rows of a batch
+    int prv[2][kDpLines];                      // predecessor line of each line of a batch
+    M dpfinal[64];                             // dp row of the last contested line
 };
+
+// Lanes l and l ^ S (S = 16: adjacent rows, 32: halves of the wavefront)
+// exchange a dword: lo gets the lower lane's value, hi the upper lane's, in
+// both lanes (gfx950 permlane swaps, no LDS).
+template <int S>
+__device__ __forceinline__ void lane_pair(int x, int& lo, int& hi) {
+    if constexpr (S == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        lo = static_cast<int>(r[0]);
+        hi = static_cast<int>(r[1]);
+    } else {
+        static_assert(S == 32, "rows or halves");
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        lo = static_cast<int>(r[0]);
+        hi = static_cast<int>(r[1]);
+    }
+}
+template <int S>
+__device__ __forceinline__ void lane_pair(float x, float& lo, float& hi) {
+    int a, b;
+    lane_pair<S>(__float_as_int(x), a, b);
+    lo = __int_as_float(a);
+    hi = __int_as_float(b);
+}
+template <int S>
+__device__ __forceinline__ void lane_pair(double x, double& lo, double& hi) {
+    const long long v = __double_as_longlong(x);
+    int l0, h0, l1, h1;
+    lane_pair<S>(static_cast<int>(v & 0xffffffffll), l0, h0);
+    lane_pair<S>(static_cast<int>(v >> 32), l1, h1);
+    lo = __longlong_as_double((static_cast<long long>(l1) << 32) | static_cast<unsigned int>(l0));
+    hi = __longlong_as_double((static_cast<long long>(h1) << 32) | static_cast<unsigned int>(h0));
+}
+// The minimum of lanes l and l ^ S, in both; the lower lane holds the lower
+// candidate indices and wins ties (the upper one only on a strict <).
+template <int S, typename M>
+__device__ __forceinline__ void lane_pair_min(M& v, int& k) {
+    M vl, vh;
+    int kl, kh;
+    lane_pair<S>(v, vl, vh);
+    lane_pair<S>(k, kl, kh);
+    const bool up = vh < vl;
+    v = up ? vh : vl;
+    k = up ? kh : kl;
+}
 
 // Phase 2 for wavefront W: the line-to-line DP (SeamBlend.cpp:362-399).
 //
-// A band is at most 2 * kSeamRadius + 1 wide, so the previous line's
-// candidates split into kDpWaves ranges of kSpan; wavefront W evaluates
-// candidates W * kSpan + i for every target lane t = lo + lane, with no
-// branch: a candidate outside the last line's band holds +inf, which never
-// wins a strict <.  The penalty smooth * |t - (lastLo + pk)| is formed from
-// (t - lastLo - W * kSpan) - i, exact in M for these small integers, so
-// each candidate's value is the reference's bit for bit.  The kSpan values
-// are reduced as a tree whose right operand (higher candidates) wins only
-// on a strict <, and the wavefronts' partial minima are combined in
-// wavefront order the same way, so the first minimum wins as in the
-// reference's serial scan.  Wavefront W combines only the lanes of its own
-// candidate range: it needs the new dp values there (readlane broadcasts
-// on the next line) and writes those lanes' back-pointers.
+// A band is at most 2 * kSeamRadius + 1 wide.  Wavefront W holds target
+// positions t = 16 W + (lane & 15), four lanes each: row g = lane >> 4 of
+// the wavefront evaluates the previous line's candidates pk = kSpan g + i,
+// read from LDS, with no branch: a candidate outside the last line's band
+// holds +inf, which never wins a strict <.  The penalty
+// smooth * |t - (lastLo + pk)| is formed from (t - lastLo - kSpan g) - i,
+// exact in M for these small integers, so each candidate's value is the
+// reference's bit for bit.  A lane's kSpan values are reduced as a tree
+// whose right operand (higher candidates) wins only on a strict <, and the
+// four rows' minima likewise across lane pairs (rows, then halves), so the
+// first minimum wins as in the reference's serial scan.  Row 0 writes the
+// new dp value and back-pointer of its position; one barrier per line.
 //
 // Cost rows of kDpLines lines are staged in LDS and their bands held in
 // registers (lane 2l / 2l+1: lo / hi of line l), double buffered: the next
@@ -249,15 +270,16 @@ __device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, 
                                                 const int* __restrict__ lohi, int* __restrict__ prevl,
                                                 uint8_t* __restrict__ back, int& last_out, int& lastW_out) {
     constexpr int kSpan = SeamDpShared<M>::kSpan;
-    constexpr int P0 = W * kSpan;
-    static_assert(kSpan * kDpWaves >= 2 * kSeamRadius + 1 && kSpan * kDpWaves <= 64, "candidate split");
+    static_assert(kDpWaves == 4, "16 target positions per wavefront");
+    static_assert(4 * kSpan >= 2 * kSeamRadius + 1 && 4 * kSpan <= 64, "candidate split");
     constexpr int kPer = kDpLines * 64 / kDpThreads;
     static_assert(2 * kDpLines <= 64, "bands of a batch fit one wavefront");
     static_assert(kDpLines * 16 <= kDpThreads, "one dword of back-pointers per thread");
-    using Best = SeamBest<M>;
     const int lane = threadIdx.x & 63;
-    const bool own = lane >= P0 && lane < P0 + kSpan;  // this wavefront's candidate lanes
-    const M kOut = __builtin_huge_val();  // dp value of a lane outside the band
+    const int t = 16 * W + (lane & 15);  // target position of this lane
+    const int g0 = kSpan * (lane >> 4);  // first candidate of this lane's row
+    const bool row0 = lane < 16;         // writes position t
+    const M kOut = __builtin_huge_val();  // dp value of a position outside the band
     M creg[kPer];
     int bnext = 0;
     auto fetch = [&](int b0) {
@@ -290,9 +312,10 @@ __device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, 
     int bcur = bnext;
     __syncthreads();
     int last = -1, lastLo = 0, lastW = 0, ord = 0;
-    M dpv = kOut;  // dp value of the last contested line at this lane (kept on own lanes)
-    // penalties smooth * |t - (lastLo + P0 + i)| of this lane's candidates,
-    // which depend on the line only through the band shift lo - lastLo
+    M dpv = kOut;  // dp value of the last contested line at position t
+    // penalties smooth * |(lo + t) - (lastLo + g0 + i)| of this lane's
+    // candidates, which depend on the line only through the band shift
+    // lo - lastLo
     M pen[kSpan] = {};
     int pshift = INT_MIN;
     for (int b0 = 0, bi = 0; b0 < lines; b0 += kDpLines, ++bi) {
@@ -308,7 +331,7 @@ __device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, 
             const int lo0 = __builtin_amdgcn_readlane(bcur, 2 * l);
             if (last >= 0 && lo0 - lastLo != pshift) {  // uniform
                 pshift = lo0 - lastLo;
-                const M d0 = static_cast<M>(pshift + lane - P0);
+                const M d0 = static_cast<M>(pshift + t - g0);
 #pragma unroll
                 for (int i = 0; i < kSpan; ++i) pen[i] = Img::smooth() * fabs(d0 - static_cast<M>(i));
             }
@@ -318,17 +341,20 @@ __device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, 
                 if (hi < lo) continue;  // block-uniform
                 if (last >= 0 && lo - lastLo != pshift) break;  // uniform: a new stretch
                 const int lw = hi - lo + 1;
-                const M c = sh.cost[buf][l][lane];
+                const M c = sh.cost[buf][l][t];
+                const int pb = ord & 1;
+                ++ord;
                 M v = kOut;
                 int bp = -1;
                 if (last < 0) {
-                    v = (lane < lw) ? c : kOut;
+                    v = (t < lw) ? c : kOut;
                 } else {
+                    const M* prev = &sh.dpv[pb ^ 1][g0];
                     M cv[kSpan];
                     int ck[kSpan];
 #pragma unroll
                     for (int i = 0; i < kSpan; ++i) {
-                        cv[i] = lane_value(dpv, P0 + i) + pen[i];
+                        cv[i] = prev[i] + pen[i];
                         ck[i] = i;
                     }
 #pragma unroll
@@ -338,31 +364,19 @@ __device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, 
                             ck[i] = (cv[i + st] < cv[i]) ? ck[i + st] : ck[i];
                             cv[i] = __builtin_fmin(cv[i], cv[i + st]);  // no NaN; a tie keeps the same bits
                         }
-                    const int pb = ord & 1;
-                    ++ord;
-                    sh.part[pb][W][lane] = Best::pack(cv[0], P0 + ck[0]);
-                    lds_barrier();
-                    if (own) {
-                        typename Best::T r[kDpWaves];  // all read before any is used
-#pragma unroll
-                        for (int q = 0; q < kDpWaves; ++q) r[q] = sh.part[pb][q][lane];
-                        M best = Best::value(r[0]);
-                        bp = Best::index(r[0]);
-#pragma unroll
-                        for (int q = 1; q < kDpWaves; ++q) {
-                            const M bq = Best::value(r[q]);
-                            const bool lt = bq < best;
-                            bp = lt ? Best::index(r[q]) : bp;
-                            best = lt ? bq : best;
-                        }
-                        v = (lane < lw) ? c + best : kOut;
-                    }
+                    M best = cv[0];
+                    bp = g0 + ck[0];
+                    lane_pair_min<16>(best, bp);
+                    lane_pair_min<32>(best, bp);
+                    v = (t < lw) ? c + best : kOut;
                 }
-                if (own) {
-                    sh.bk[buf][l][lane] = static_cast<uint8_t>(bp & 0xFF);
+                if (row0) {
+                    sh.dpv[pb][t] = v;
+                    sh.bk[buf][l][t] = static_cast<uint8_t>(bp & 0xFF);
                     dpv = v;
                 }
                 if (W == 0 && lane == 0) sh.prv[buf][l] = last;
+                lds_barrier();  // the new dp row is visible to every wavefront
                 last = b0 + l;
                 lastLo = lo;
                 lastW = lw;
@@ -373,7 +387,7 @@ __device__ __forceinline__ void seam_dp_forward(SeamDpShared<M>& sh, int lines, 
         __syncthreads();
         flush(buf, b0, nb);
     }
-    if (own) sh.dpfinal[lane] = dpv;
+    if (row0) sh.dpfinal[t] = dpv;
     last_out = last;
     lastW_out = lastW;
 }
