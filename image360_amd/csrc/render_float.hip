@@ -1598,7 +1598,8 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
     constexpr int kRowStep = kWarpTH / K;
     __shared__ FinalItem sit[I360HIP_MAX_CONTRIBUTORS];
     // per slot: the level-1 difference R1 - G1 of the slot's contributor
-    __shared__ M sG[kFinSlots][kFinH1 * kFinW1][3];
+    // (a fourth, unused component aligns an element for one 16-byte access)
+    __shared__ M sG[kFinSlots][kFinH1 * kFinW1][4];
     __shared__ uint32_t smask;
     const int tid = static_cast<int>(threadIdx.x);
     const int lx = tid & 63, ry = tid >> 6;
@@ -1634,6 +1635,13 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         if (bits) atomicOr(&smask, bits);
         lds_barrier();
         const uint32_t mask = smask;
+        // the warped values (in flight across the level-1 staging)
+        M qv[K][4];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const FinalItem& it = sit[o[k] >= 0 ? o[k] : 0];
+            if (count > 0) ld4(it.warped, p[k], qv[k], static_cast<S*>(nullptr));
+        }
         const bool deep = ch.levels > 1;  // tile-uniform
         const int lw1 = deep ? ch.lw[1] : 1, lh1 = deep ? ch.lh[1] : 1;
         const M sx = expand_scale<M>(lw1, w), sy = expand_scale<M>(lh1, h);
@@ -1681,15 +1689,9 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                     const M inv = (a[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][3]) : M(0);
 #pragma unroll
                     for (int c3 = 0; c3 < 3; ++c3) sG[sl][qq[k]][c3] = r[k][c3] - (in ? a[k][c3] * inv : M(0));
+                    sG[sl][qq[k]][3] = M(0);
                 }
             }
-        }
-        // the warped values (in flight across the barrier)
-        M qv[K][4];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const FinalItem& it = sit[o[k] >= 0 ? o[k] : 0];
-            if (count > 0) ld4(it.warped, p[k], qv[k], static_cast<S*>(nullptr));
         }
         lds_barrier();
         // C
@@ -1698,10 +1700,31 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
             for (int k = 0; k < K; ++k)
                 if (ok[k]) gptr(ch.owner_out)[p[k]] = o[k];
         }
+        // every owner of the tile has a staged slot (tile-uniform): no
+        // per-pixel branch; an unowned pixel's value is computed from slot 0
+        // and discarded (its reads stay in the staged region: the tile's
+        // coordinates, clamped to the level-1 grid, lie inside it)
+        const bool fast = staged && __popc(mask) <= kFinSlots;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             M v[3] = {M(0), M(0), M(0)};
-            if (o[k] >= 0) {
+            if (fast) {
+                const int y = tl.ty * kWarpTH + ry + kRowStep * k;
+                const Coord<M> c = expand_at_s<M>(x, y, sx, sy, lw1, lh1);
+                const int slot = __popc(mask & ((1u << max(o[k], 0)) - 1u));
+                // small non-negative operands: 24-bit multiplies (full rate)
+                const int r0 = __mul24(slot, kFinH1 * kFinW1) + __mul24(c.y0 - Y0, kFinW1) - X0;
+                const int r1 = r0 + __mul24(c.y1 - c.y0, kFinW1);
+                const int qs[4] = {r0 + c.x0, r0 + c.x1, r1 + c.x0, r1 + c.x1};
+                M dq[4][4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int c3 = 0; c3 < 4; ++c3) dq[j][c3] = sG[0][qs[j]][c3];
+#pragma unroll
+                for (int c3 = 0; c3 < 3; ++c3)
+                    v[c3] = qv[k][c3] + lerp2m<M>(c, dq[0][c3], dq[1][c3], dq[2][c3], dq[3][c3]);
+            } else if (o[k] >= 0) {
                 // the owner covers the pixel: its coverage (alpha) is exactly 1
                 // and its hole-filled level-0 Gaussian is the warped value
 #pragma unroll
