@@ -105,8 +105,8 @@ def test_breaking_shared_order_changes_the_result():
 def _line_min(dprev, last_lo, lo, smooth, dt, ranges):
     """Per target lane: minimum over the previous line's candidates of
     dprev + smooth * |t - q|, each range scanned in increasing order with a
-    strict <, the ranges' results combined in order with a strict < (the
-    DP kernel's split across wavefronts; one range is the serial scan)."""
+    strict <, the ranges' results combined in order with a strict < (one
+    range is the serial scan)."""
     out = []
     for lane in range(64):
         t = lo + lane
@@ -123,25 +123,32 @@ def _line_min(dprev, last_lo, lo, smooth, dt, ranges):
     return out
 
 
-SPAN, WAVES = 13, 4  # candidates per wavefront of the DP kernel, wavefronts
+SPAN, GROUPS = 13, 4  # candidates per lane of the DP kernel, lane groups (rows of a wavefront)
 
 
 def _line_min_kernel(dprev, last_lo, lo, smooth, dt, combine_le=False):
     """The DP kernel's candidate scan, step for step: the previous line's
-    row held in 64 lanes with +inf outside its band, WAVES ranges of SPAN
-    candidates each evaluated with no guard, the penalty formed from the
-    band shift (lo - last_lo + lane - p0) - i, each range reduced as a tree
-    (node i absorbs node i + st; the right node's index on a strict <, the
-    value by fmin), then the ranges combined in order with a strict <
-    (combine_le plants a <= there)."""
+    row held in 64 positions with +inf outside its band, GROUPS ranges of
+    SPAN candidates each evaluated with no guard, the penalty formed from
+    the band shift (lo - last_lo + t - p0) - i, each range reduced as a
+    tree (node i absorbs node i + st; the right node's index on a strict <,
+    the value by fmin), then the ranges joined in pairs, as the kernel's
+    lane swaps do (rows 0-1 and 2-3, then the two halves), the upper range
+    winning only on a strict < (combine_le plants a <= there)."""
     row = np.full(64, np.inf, dtype=dt)
     row[: len(dprev)] = dprev
+
+    def join(low, high):
+        if high[0] < low[0] or (combine_le and high[0] == low[0]):
+            return high
+        return low
+
     out = []
-    for lane in range(64):
-        best, bp = None, -1
-        for w in range(WAVES):
-            p0 = w * SPAN
-            d0 = dt(lo - last_lo + lane - p0)
+    for t in range(64):
+        parts = []
+        for g in range(GROUPS):
+            p0 = g * SPAN
+            d0 = dt(lo - last_lo + t - p0)
             cv = [dt(row[p0 + i] + dt(dt(smooth) * abs(dt(d0 - dt(i))))) for i in range(SPAN)]
             ck = list(range(SPAN))
             st = 1
@@ -150,9 +157,8 @@ def _line_min_kernel(dprev, last_lo, lo, smooth, dt, combine_le=False):
                     ck[i] = ck[i + st] if cv[i + st] < cv[i] else ck[i]
                     cv[i] = np.fmin(cv[i], cv[i + st])
                 st *= 2
-            if best is None or cv[0] < best or (combine_le and cv[0] == best):
-                best, bp = cv[0], p0 + ck[0]
-        out.append((best, bp))
+            parts.append((cv[0], p0 + ck[0]))
+        out.append(join(join(parts[0], parts[1]), join(parts[2], parts[3])))
     return out
 
 
@@ -160,10 +166,11 @@ def _line_min_kernel(dprev, last_lo, lo, smooth, dt, combine_le=False):
 @pytest.mark.parametrize("seed", range(6))
 def test_split_candidate_scan_keeps_first_minimum(seed, dt):
     """Splitting a line's candidates into 4 ranges of 13 (a band is at most
-    49 wide) and combining the partial minima in range order selects the
-    same value and the same (first) back-pointer as the reference's single
-    serial scan, ties included (integer costs make ties frequent)."""
-    assert SPAN * WAVES >= 2 * 24 + 1
+    49 wide) and joining the partial minima in pairs, lower ranges first,
+    selects the same value and the same (first) back-pointer as the
+    reference's single serial scan, ties included (integer costs make ties
+    frequent)."""
+    assert SPAN * GROUPS >= 2 * 24 + 1
     rng = np.random.default_rng(seed)
     for k in range(40):
         last_w = 49 if k < 4 else int(rng.integers(1, 50))
@@ -178,8 +185,8 @@ def test_split_candidate_scan_keeps_first_minimum(seed, dt):
 
 
 def test_kernel_scan_emulation_catches_a_wrong_tie_rule():
-    """The emulation is sharp: with a <= planted in the range combine, a row
-    of tied candidates takes its back-pointer from the last range instead of
+    """The emulation is sharp: with a <= planted in the range join, a row
+    of tied candidates takes its back-pointer from a later range instead of
     candidate 0."""
     dt = np.float64
     dprev = np.full(40, 4.0, dtype=dt)  # every candidate ties
