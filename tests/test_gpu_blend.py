@@ -94,3 +94,28 @@ def test_blend_ragged_width_full_rows(hip, monkeypatch, strip, cw):
     out, owner = _blend(hip, bufs, bands=4)
     assert np.array_equal(owner, want_owner)
     assert np.array_equal(out, want)
+
+
+@pytest.mark.parametrize("hole", [(0, 0), (100, 120), (64, 64), (190, 70), (130, 200)])
+def test_blend_seam_chain_with_uncontested_gap(hip, hole):
+    """A 400-line seam (overlap 500 x 400, scanned row by row) whose chain of
+    contested lines skips a run of uncontested rows: the second contributor
+    has a hole there, so those rows hold no crossover.  The DP's backtrack
+    walks 64-line batches and reloads when the chain jumps over a whole
+    batch (a hole of 64 rows or more); bands are clipped near the hole's
+    rims.  Owners and output equal the oracle's bit for bit."""
+    cw, ch = 800, 400
+    rng = np.random.Generator(np.random.PCG64(7 + hole[0]))
+    a = np.clip(rng.normal(30000, 6000, size=(ch, cw, 3)), 1, 65535).astype(np.uint16)
+    b = np.clip(rng.normal(26000, 6000, size=(ch, cw, 3)), 1, 65535).astype(np.uint16)
+    a[:, 600:] = 0
+    b[:, :100] = 0
+    y0, n = hole
+    if n:
+        b[y0:y0 + n, :] = 0
+    bufs = [a, b]
+    note(f"blend seam gap hole={hole} {cw}x{ch}")
+    want, want_owner = po.blend_chunk(bufs, 5, want_owner=True)
+    out, owner = _blend(hip, bufs)
+    assert np.array_equal(owner, want_owner)
+    assert np.array_equal(out, want)
