@@ -1635,12 +1635,17 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         if (bits) atomicOr(&smask, bits);
         lds_barrier();
         const uint32_t mask = smask;
-        // the warped values (in flight across the level-1 staging)
-        M qv[K][4];
+        // the warped values, in flight across the level-1 staging: loaded
+        // raw (storage type) and converted where they are used, so that no
+        // instruction needs them, and none of the four loads waits, before
+        // the blend
+        using Raw4 = S __attribute__((ext_vector_type(4)));
+        Raw4 qr[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const FinalItem& it = sit[o[k] >= 0 ? o[k] : 0];
-            if (count > 0) ld4(it.warped, p[k], qv[k], static_cast<S*>(nullptr));
+            qr[k] = Raw4{};
+            if (count > 0) qr[k] = gptr(static_cast<const Raw4*>(it.warped))[p[k]];
         }
         const bool deep = ch.levels > 1;  // tile-uniform
         const int lw1 = deep ? ch.lw[1] : 1, lh1 = deep ? ch.lh[1] : 1;
@@ -1723,12 +1728,12 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
                     for (int c3 = 0; c3 < 4; ++c3) dq[j][c3] = sG[0][qs[j]][c3];
 #pragma unroll
                 for (int c3 = 0; c3 < 3; ++c3)
-                    v[c3] = qv[k][c3] + lerp2m<M>(c, dq[0][c3], dq[1][c3], dq[2][c3], dq[3][c3]);
+                    v[c3] = static_cast<M>(qr[k][c3]) + lerp2m<M>(c, dq[0][c3], dq[1][c3], dq[2][c3], dq[3][c3]);
             } else if (o[k] >= 0) {
                 // the owner covers the pixel: its coverage (alpha) is exactly 1
                 // and its hole-filled level-0 Gaussian is the warped value
 #pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) v[c3] = qv[k][c3];
+                for (int c3 = 0; c3 < 3; ++c3) v[c3] = static_cast<M>(qr[k][c3]);
                 if (deep) {
                     const int y = tl.ty * kWarpTH + ry + kRowStep * k;
                     const Coord<M> c = expand_at_s<M>(x, y, sx, sy, lw1, lh1);
