@@ -1270,13 +1270,18 @@ __device__ __forceinline__ M lerp2m(const Coord<M>& c, M v00, M v10, M v01, M v1
 }
 
 // Hole-filled Gaussian at one level-l sample; zero outside the support box.
+// The plane spans the whole level, so the load is valid outside the box
+// too: it is issued with no branch (the loads of neighbouring samples are
+// then in flight together) and its bits are selected away there, never
+// used in arithmetic.
 template <typename M, typename S>
 __device__ __forceinline__ void gauss_m(const void* nd, const IBox& box, int w, int x, int y, M (&g)[3]) {
-    if (!in_box(box, x, y)) { g[0] = g[1] = g[2] = M(0); return; }
+    const bool in = in_box(box, x, y);
     M q[4];
     ld4(nd, static_cast<size_t>(y) * w + x, q, static_cast<S*>(nullptr));
-    const M inv = (q[3] > static_cast<M>(1e-6)) ? M(1) / q[3] : M(0);
-    g[0] = q[0] * inv; g[1] = q[1] * inv; g[2] = q[2] * inv;
+    const M inv = (in && q[3] > static_cast<M>(1e-6)) ? M(1) / q[3] : M(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g[k] = in ? q[k] * inv : M(0);
 }
 
 template <typename M, typename S>
@@ -1329,19 +1334,47 @@ __global__ void __launch_bounds__(256) rcompose_kernel(const RItem* __restrict__
         if (!top) c = expand_at<M>(x, y, nw, nh, w, h);
         M band[3] = {M(0), M(0), M(0)};
         const size_t p = static_cast<size_t>(y) * w + x;
+        // a hole-filled Gaussian from a raw numerator/denominator sample (the
+        // same arithmetic as gauss_m)
+        using Raw4 = S __attribute__((ext_vector_type(4)));
+        auto fill = [](const Raw4& r, bool in, M (&g)[3]) {
+            M q[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) q[j] = static_cast<M>(r[j]);
+            const M inv = (in && q[3] > static_cast<M>(1e-6)) ? M(1) / q[3] : M(0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) g[j] = in ? q[j] * inv : M(0);
+        };
         for (int k = 0; k < ch.count; ++k) {
             const RItem& it = items[ch.first + k];
             const IBox bl = box_of(it.lbox[l]);
             if (!in_box(bl, x, y)) continue;
             const M wt = ld1<M, S>(it.wt[l], p);
             if (wt == M(0)) continue;
-            M g[3];
-            gauss_m<M, S>(it.nd[l], bl, w, x, y, g);
+            // the sample and (below the top) the four upsample corners, all
+            // loaded raw before any is used: one memory latency, not two
+            // (the planes span their whole level, so every corner address
+            // is valid; a corner outside the box is selected away)
+            const Raw4 qc = gptr(static_cast<const Raw4*>(it.nd[l]))[p];
+            Raw4 qe[4] = {};
             if (!top) {
-                M up[3];
-                expand_gauss_m<M, S>(it.nd[l + 1], box_of(it.lbox[l + 1]), nw, c, up);
+                const Raw4* np = static_cast<const Raw4*>(it.nd[l + 1]);
+                qe[0] = gptr(np)[static_cast<size_t>(c.y0) * nw + c.x0];
+                qe[1] = gptr(np)[static_cast<size_t>(c.y0) * nw + c.x1];
+                qe[2] = gptr(np)[static_cast<size_t>(c.y1) * nw + c.x0];
+                qe[3] = gptr(np)[static_cast<size_t>(c.y1) * nw + c.x1];
+            }
+            M g[3];
+            fill(qc, true, g);
+            if (!top) {
+                const IBox bu = box_of(it.lbox[l + 1]);
+                M a[3], b[3], d[3], e[3];
+                fill(qe[0], in_box(bu, c.x0, c.y0), a);
+                fill(qe[1], in_box(bu, c.x1, c.y0), b);
+                fill(qe[2], in_box(bu, c.x0, c.y1), d);
+                fill(qe[3], in_box(bu, c.x1, c.y1), e);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) g[q] = g[q] - up[q];
+                for (int q = 0; q < 3; ++q) g[q] = g[q] - lerp2m<M>(c, a[q], b[q], d[q], e[q]);
             }
 #pragma unroll
             for (int q = 0; q < 3; ++q) band[q] = band[q] + wt * g[q];
