@@ -1698,35 +1698,46 @@ __global__ void __launch_bounds__(256) rfinal_kernel(const RItem* __restrict__ i
         if (staged) {
             constexpr int kSlotN = kFinH1 * kFinW1;
             constexpr int kIt = (kSlotN + 255) / 256;
+            // Loads are raw (storage type) and unconditional: an element
+            // outside the region reads texel 0 of the level-1 plane and is
+            // skipped at the write.  The reconstruction is the chunk's, the
+            // same for every slot, so it is loaded once.
+            using RawS = S __attribute__((ext_vector_type(4)));
+            using RawM = M __attribute__((ext_vector_type(4)));
+            int qq[kIt], tx1[kIt], ty1[kIt];
+            size_t t1[kIt];
+            bool use[kIt];
+            RawM rr[kIt];
+#pragma unroll
+            for (int k = 0; k < kIt; ++k) {
+                qq[k] = tid + 256 * k;
+                const int qy = qq[k] / kFinW1, qx = qq[k] - qy * kFinW1;
+                use[k] = qq[k] < kSlotN && qy < H1 && qx < W1;
+                ty1[k] = Y0 + qy;
+                tx1[k] = X0 + qx;
+                t1[k] = use[k] ? static_cast<size_t>(ty1[k]) * lw1 + tx1[k] : 0;
+                rr[k] = gptr(static_cast<const RawM*>(ch.recon[1]))[t1[k]];
+            }
             uint32_t m = mask;
             for (int sl = 0; sl < nslots; ++sl, m &= m - 1u) {
                 const FinalItem& it = sit[__ffs(m) - 1];
-                M a[kIt][4], r[kIt][4];
-                int qq[kIt], tx1[kIt], ty1[kIt];
-                bool use[kIt];
+                RawS ar[kIt];
 #pragma unroll
-                for (int k = 0; k < kIt; ++k) {
-                    qq[k] = tid + 256 * k;
-                    const int qy = qq[k] / kFinW1, qx = qq[k] - qy * kFinW1;
-                    use[k] = qq[k] < kSlotN && qy < H1 && qx < W1;
-                    ty1[k] = Y0 + qy;
-                    tx1[k] = X0 + qx;
-                    const size_t t = use[k] ? static_cast<size_t>(ty1[k]) * lw1 + tx1[k] : 0;
-                    a[k][0] = a[k][1] = a[k][2] = a[k][3] = M(0);
-                    r[k][0] = r[k][1] = r[k][2] = r[k][3] = M(0);
-                    if (use[k]) {
-                        ld4(it.nd1, t, a[k], static_cast<S*>(nullptr));
-                        ld4(ch.recon[1], t, r[k], static_cast<M*>(nullptr));
-                    }
-                }
+                for (int k = 0; k < kIt; ++k) ar[k] = gptr(static_cast<const RawS*>(it.nd1))[t1[k]];
 #pragma unroll
                 for (int k = 0; k < kIt; ++k) {
                     if (!use[k]) continue;
+                    M a[4], r[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        a[j] = static_cast<M>(ar[k][j]);
+                        r[j] = rr[k][j];
+                    }
                     const bool in = tx1[k] >= it.box[0] && tx1[k] <= it.box[2] && ty1[k] >= it.box[1] &&
                                     ty1[k] <= it.box[3];
-                    const M inv = (a[k][3] > static_cast<M>(1e-6)) ? recip<M, S>(a[k][3]) : M(0);
+                    const M inv = (a[3] > static_cast<M>(1e-6)) ? recip<M, S>(a[3]) : M(0);
 #pragma unroll
-                    for (int c3 = 0; c3 < 3; ++c3) sG[sl][qq[k]][c3] = r[k][c3] - (in ? a[k][c3] * inv : M(0));
+                    for (int c3 = 0; c3 < 3; ++c3) sG[sl][qq[k]][c3] = r[c3] - (in ? a[c3] * inv : M(0));
                     sG[sl][qq[k]][3] = M(0);
                 }
             }
