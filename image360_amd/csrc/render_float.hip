@@ -541,10 +541,10 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                     i0[r] = static_cast<size_t>(sya[r]) * fw + sxa[r];
                 }
                 // blend of row r's texel pairs (a, b) above (c, d), stores
-                auto finish = [&](int r, const _Float16 (&a)[3], const _Float16 (&b)[3], const _Float16 (&c)[3],
-                                  const _Float16 (&d)[3]) {
-                    const float omx = 1.0f - fx[r], omy = 1.0f - fy[r];
-                    const float wq[4] = {omx * omy, fx[r] * omy, omx * fy[r], fx[r] * fy[r]};
+                auto finish = [&](int r, float fxr, float fyr, const _Float16 (&a)[3], const _Float16 (&b)[3],
+                                  const _Float16 (&c)[3], const _Float16 (&d)[3]) {
+                    const float omx = 1.0f - fxr, omy = 1.0f - fyr;
+                    const float wq[4] = {omx * omy, fxr * omy, omx * fyr, fxr * fyr};
                     float v[4];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
@@ -591,28 +591,59 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                             *reinterpret_cast<u4s*>(stage + r * kStageWords + 4 * cl) = q;
                         }
                     }
+                    // the staged words count as read here on every path (a
+                    // thread with no row to write reads none): their loads
+                    // are then known complete when the registers are reused
+#pragma unroll
+                    for (int k = 0; k < kStageRows / 8; ++k)
+                        asm volatile("" ::"v"(ldv[k].x), "v"(ldv[k].y), "v"(ldv[k].z), "v"(ldv[k].w));
                     lds_barrier();
+                    // Rows whose pair lies in the staged box blend from LDS;
+                    // the others (rounding beyond the margin, rare) are done
+                    // afterwards from the frame, their position recomputed.
+                    // Kept apart, the rare rows' global loads share no
+                    // registers with the staged rows, whose LDS reads then
+                    // wait for no memory operation (they had waited for
+                    // every store before).
+                    uint32_t farm = 0u;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int lr = sya[r] - tm.sy0, lc = sxa[r] - tm.sx0;
-                        _Float16 a[3], b[3], c[3], d[3];
                         if (lr >= 0 && lr + 1 < tm.sh && lc >= 0 && lc + 1 < tm.sw) {
+                            _Float16 a[3], b[3], c[3], d[3];
                             // the pair's 12 bytes start 0 or 2 bytes into word q
+                            // (so the decode's odd-frame path, which reads the
+                            // frame at the index, is never taken here)
                             const int off = 6 * sxa[r] - 4 * w0;
                             const uint32_t* t0 = stage + lr * kStageWords + (off >> 2);
                             const uint32_t* t1 = t0 + kStageWords;
                             const u4a q0 = {t0[0], t0[1], t0[2], t0[3]};
                             const u4a q1 = {t1[0], t1[1], t1[2], t1[3]};
-                            decode_pair_h(q0, static_cast<uint32_t>(off & 3), src, i0[r], a, b);
-                            decode_pair_h(q1, static_cast<uint32_t>(off & 3), src, i0[r] + fw, c, d);
-                        } else {  // outside the box (rounding beyond the margin): from the frame
-                            uint32_t s0, s1;
-                            const u4a q0 = load_pair_f16(src, i0[r], nbytes, s0);
-                            const u4a q1 = load_pair_f16(src, i0[r] + fw, nbytes, s1);
-                            decode_pair_h(q0, s0, src, i0[r], a, b);
-                            decode_pair_h(q1, s1, src, i0[r] + fw, c, d);
+                            decode_pair_h(q0, static_cast<uint32_t>(off & 3), src, 0, a, b);
+                            decode_pair_h(q1, static_cast<uint32_t>(off & 3), src, 0, c, d);
+                            finish(r, fx[r], fy[r], a, b, c, d);
+                        } else {
+                            farm |= 1u << r;
                         }
-                        finish(r, a, b, c, d);
+                    }
+                    if (__any(farm != 0u)) {  // wave-uniform
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            if (!((farm >> r) & 1u)) continue;
+                            const int ly = wave * R + r;
+                            float px, py;
+                            tile_pos(tm, lx, ly, tile_den(tm, lx, ly), px, py);
+                            const float flx = floorf(px), fly = floorf(py);
+                            const size_t ir = static_cast<size_t>(tm.iy + static_cast<int>(fly)) * fw +
+                                              (tm.ix + static_cast<int>(flx));
+                            _Float16 a[3], b[3], c[3], d[3];
+                            uint32_t s0, s1;
+                            const u4a q0 = load_pair_f16(src, ir, nbytes, s0);
+                            const u4a q1 = load_pair_f16(src, ir + fw, nbytes, s1);
+                            decode_pair_h(q0, s0, src, ir, a, b);
+                            decode_pair_h(q1, s1, src, ir + fw, c, d);
+                            finish(r, px - flx, py - fly, a, b, c, d);
+                        }
                     }
                 } else {
                     u4a q0[R], q1[R];
@@ -627,7 +658,7 @@ __global__ void __launch_bounds__(256) rwarp_kernel(const RItem* __restrict__ it
                         _Float16 a[3], b[3], c[3], d[3];
                         decode_pair_h(q0[r], s0[r], src, i0[r], a, b);
                         decode_pair_h(q1[r], s1[r], src, i0[r] + fw, c, d);
-                        finish(r, a, b, c, d);
+                        finish(r, fx[r], fy[r], a, b, c, d);
                     }
                 }
                 // the tile's coverage (all 1): one dword per thread, runs
