@@ -324,13 +324,24 @@ __global__ void __launch_bounds__(1024) chamfer_kernel(const ChamferWork* __rest
                     b = dmin(b, dadd(i == 0 ? eUp : cur[col(i - 1)], 1.0f));
                     cur[s] = (F || ((exists >> s) & 1u)) ? b : kNone;
                 }
-                float* row = wk.dist + static_cast<size_t>(storeRow(T)) * dp + xs;
+                // BWD rewrites the plane in place: where the strip's four
+                // values equal what the forward pass left (initCur: the
+                // forward bits, or 0 outside the box, where the forward
+                // pass stored 0), the plane already holds the result
+                bool same = !FWD;
+                if constexpr (!FWD) {
 #pragma unroll
-                for (int g = 0; g < S / 4; ++g) {
-                    f4v v;
-                    v.x = __uint_as_float(cur[4 * g]); v.y = __uint_as_float(cur[4 * g + 1]);
-                    v.z = __uint_as_float(cur[4 * g + 2]); v.w = __uint_as_float(cur[4 * g + 3]);
-                    st_f4(row + 4 * g, v);
+                    for (int s = 0; s < S; ++s) same = same && cur[s] == initCur[s];
+                }
+                if (!same) {
+                    float* row = wk.dist + static_cast<size_t>(storeRow(T)) * dp + xs;
+#pragma unroll
+                    for (int g = 0; g < S / 4; ++g) {
+                        f4v v;
+                        v.x = __uint_as_float(cur[4 * g]); v.y = __uint_as_float(cur[4 * g + 1]);
+                        v.z = __uint_as_float(cur[4 * g + 2]); v.w = __uint_as_float(cur[4 * g + 3]);
+                        st_f4(row + 4 * g, v);
+                    }
                 }
                 pprevLast = prev[col(S - 1)];
 #pragma unroll
